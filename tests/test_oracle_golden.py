@@ -1,0 +1,94 @@
+"""Pins the CPU restatement (oracle/pf_oracle.c) before anything is checked against it.
+
+* every fixture written by the reference build (tests/golden/gen_golden.py) is reproduced
+  bit for bit: final u, v, w, p, the divergence diagnostic and the per-step SOR error;
+* the colour decode of the reference's SOR is brute-forced: each interior cell is visited
+  exactly once, and the first sweep is the parity class (i+j+k even in 3D, i+j odd in 2D);
+* closed forms: uniform flow through porosity 1 is an exact fixed point of the step.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden
+
+pyoracle = pytest.importorskip("pyoracle")
+
+
+def _run(p, g):
+    o = pyoracle.OracleRun(p, g["phi_raw"])
+    errs = o.step(p.istep_max)
+    return o, errs
+
+
+FULL = [n for n in golden_names() if not n.startswith("g3d_ref_")]
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_oracle_matches_reference_bitwise(name):
+    p, g = load_golden(name)
+    o, errs = _run(p, g)
+    for f in ("u", "v", "w", "p"):
+        if f in g:
+            np.testing.assert_array_equal(o.interior(f), g[f], err_msg=f"{name}:{f}")
+    np.testing.assert_array_equal(errs, g["errs"], err_msg=f"{name}:errs")
+    np.testing.assert_array_equal(o.divergence(), g["div"], err_msg=f"{name}:div")
+
+
+def test_oracle_matches_reference_room_project():
+    """The reference's own room project (test/room.zip, 64^3), compared by SHA-256."""
+    p, g = load_golden("g3d_ref_room64")
+    o, errs = _run(p, g)
+    np.testing.assert_array_equal(errs, g["errs"])
+    for f in ("u", "v", "w", "p"):
+        a = o.interior(f)
+        np.testing.assert_array_equal(a[31], g[f"{f}_plane031"])
+        assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() == str(g["sha_" + f]), f
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_colour_decode_is_parity_class(dim):
+    for m in range(2, 10):
+        for n in range(2, 10):
+            for l in (range(2, 8) if dim == 3 else [1]):
+                rc, v = pyoracle.decode_first_sweep(dim, m, n, l)
+                assert rc == 0
+                assert set(np.unique(v)) <= {1, 16}, (m, n, l)
+                if dim == 3:
+                    k, j, i = np.meshgrid(np.arange(1, l + 1), np.arange(1, n + 1), np.arange(1, m + 1), indexing="ij")
+                    first = (i + j + k) % 2 == 0
+                else:
+                    j, i = np.meshgrid(np.arange(1, n + 1), np.arange(1, m + 1), indexing="ij")
+                    first = (i + j) % 2 == 1
+                np.testing.assert_array_equal(v == 1, first, err_msg=str((m, n, l)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uniform_flow_is_fixed_point(dim):
+    from pixelflow_amd import cases
+    if dim == 2:
+        p = cases.CaseParams(dim=2, m=20, n=12, width=1.0, height=0.6, time=0.01, istep_max=5, iter_max=7,
+                             inlet_velocity=0.7, outlet_pressure=0.0)
+        phi = np.ones((12, 20))
+    else:
+        p = cases.CaseParams(dim=3, m=12, n=8, l=6, width=1.0, height=0.6, depth=0.5, time=0.01, istep_max=5,
+                             iter_max=7, inlet_velocity=0.7, outlet_pressure=0.0)
+        phi = np.ones((6, 8, 12))
+    o = pyoracle.OracleRun(p, phi)
+    errs = o.step(5)
+    assert np.all(o.interior("u") == 0.7)
+    assert np.all(o.interior("v") == 0.0)
+    assert np.all(o.interior("p") == 0.0)
+    assert np.all(errs == 0.0)
+    assert np.all(o.divergence() == 0.0)
+
+
+def test_fp32_build_is_close_but_not_equal():
+    """The shipped FP32 build differs from FP64 at the 1e-7..1e-5 level (reduced precision
+    alone cannot meet a 1e-6 field target; defect correction is needed)."""
+    p, g = load_golden("g2d_cyl48")
+    for f in ("u", "v", "p"):
+        a64, a32 = g[f], g[f + "_fp32"].astype(np.float64)
+        rel = np.linalg.norm(a32 - a64) / np.linalg.norm(a64)
+        assert 1e-9 < rel < 1e-3, (f, rel)
