@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Benchmark of PixelFlow's time step on one MI355X.
+
+Workload (default): 3D flow around a sphere, 256x128x128 cells over 2x1x1 m, D = 0.15 m,
+2-cell tanh interface, U = 1, xnue = 1e-3, dt = 1e-4, 500 steps, 100 red/black SOR
+iterations per step with omega = 1.7, FP64 -- the reference's 3D uniform-inflow solver
+(src/omp_parallel/ibm_3d_uniform_omp_cpu.f90) reproduced bit for bit.
+
+One "step" is one time step: predictor, Poisson RHS, 100 SOR iterations (200 colour
+sweeps), projection and boundary update, all resident in HBM.  The line printed:
+
+  value       cell-steps/s = m*n*l*steps / wall seconds of the timed steps
+  roofline    the SOR colour-sweep kernel: algorithmic bytes per launch (3 FP64 words per
+              cell: read p, write p of half the cells, read bb of half, read porosity --
+              the streamed model 8*(2d+2 + 6K + 2d+1) B per cell-step) / average launch
+              time measured with HIP events on the solver's stream; traffic = measured
+              HBM bytes per launch from rocprofv3 counters when a profile is committed
+  cpu_baseline the CPU restatement of the reference (oracle/pf_oracle.c, OpenMP) on the
+              host cores for a bounded number of steps of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pixelflow_amd import cases  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s + HBM roofline frac (FP64 vs mixed); batch cases/s; p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def cpu_baseline(p, phi, steps: int, threads: int):
+    """Time the oracle (CPU restatement of the reference) for `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import pyoracle  # noqa: E402  (test infrastructure: the baseline leg only)
+    o = pyoracle.OracleRun(p, phi)
+    o.step(1)                      # warm the pages and the thread pool
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return p.cells * steps / dt, dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--m", type=int, default=256)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--l", type=int, default=128)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from pixelflow_amd.solver import Solver  # the HIP path; raises if the library is missing
+
+    p, phi = cases.sphere_case(m=args.m, n=args.n, l=args.l, steps=max(args.steps, 1))
+    workload = f"sphere3d_{p.m}x{p.n}x{p.l}"
+    log(f"[bench] workload {workload} fp64 iter_max={p.iter_max} steps={args.steps} warmup={args.warmup}")
+    s = Solver(p, phi)
+    if args.warmup:
+        s.step(args.warmup)
+    s.set_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    st = s.step(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    sweep_us, launches = s.last_sweep_time_us()
+    s.close()
+    log(f"[bench] {args.steps} steps in {wall:.3f} s; SOR sweep avg {sweep_us:.2f} us over {launches} launches")
+
+    cells = p.cells
+    value = cells * args.steps / wall
+    sweep_bytes = 3 * 8 * cells
+    achieved = sweep_bytes / (sweep_us * 1e-6) / 1e9 if sweep_us > 0 else 0.0
+    step_bytes = 8 * ((2 * 3 + 2) + 6 * p.iter_max + (2 * 3 + 1)) * cells
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sphere porosity, the reference's CSV rounding)",
+        "config": {"workload": workload, "grid": [p.m, p.n, p.l], "iter_max": p.iter_max, "omega": p.relux_factor,
+                   "dt": p.dt, "xnue": p.xnue, "precision": "fp64"},
+        "sor_cell_updates_per_s": value * p.iter_max,
+        "step_roofline_frac": (step_bytes * args.steps / wall / 1e9) / HBM_PEAK_GBS,
+        "final_sor_error": float(st["sor_error"][-1]),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_sor (red/black colour sweep)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic("k_sor", workload),
+            "bytes_per_launch_algorithmic": sweep_bytes,
+            "avg_launch_us": sweep_us,
+        },
+    }
+    if args.cpu_steps > 0:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")
+        v, dt = cpu_baseline(p, phi, args.cpu_steps, threads)
+        res["cpu_baseline"] = {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_steps} steps of the same {workload} workload ({dt:.1f} s)"}
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

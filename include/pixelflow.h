@@ -1,0 +1,120 @@
+/*
+ * pixelflow.h -- C-ABI of the MI355X (gfx950) solver for PixelFlow's time step.
+ *
+ * The reference has no FFI: its time step is a set of Fortran subroutines called from
+ * each executable's `program main` with explicit-shape arrays dimension(0:md,0:nd[,0:ld])
+ * (src/omp_parallel/ibm_3d_uniform_omp_cpu.f90:81-132, ibm_2d_uniform_omp_cpu.f90:80-125,
+ * ibm_3d_air_condition_omp_cpu.f90:95-146).  Each entry point below names the reference
+ * code it replaces.  Plain C types only; every call returns 0 on success or a negative
+ * PF_E* code, with a message in pf_last_error().  A context owns all device memory; host
+ * pointers are never kept after a call returns.  One host thread at a time per context.
+ *
+ * Host arrays use the reference's storage: Fortran column-major with ghost layers,
+ * element (i,j,k), 0<=i<=m+1, 0<=j<=n+1, 0<=k<=l+1, at offset i + ldi*(j + ldj*k), where
+ * ldi >= m+2 and ldj >= n+2 are the leading dimensions (md+1, nd+1 for the reference's
+ * static arrays; m+2, n+2 for exact-size allocatables).  2D fields ignore k.
+ */
+#ifndef PIXELFLOW_H
+#define PIXELFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_ABI_VERSION 1
+
+enum pf_status {
+    PF_OK = 0,
+    PF_EINVAL = -1,      /* bad argument or descriptor */
+    PF_EHIP = -2,        /* HIP runtime error */
+    PF_ENOMEM = -3,      /* device allocation failed */
+    PF_ESTATE = -4,      /* call out of order (e.g. step before porosity) */
+    PF_EDIVERGED = -5    /* a non-finite value appeared (reported, never aborts) */
+};
+
+enum pf_bc { PF_BC_UNIFORM = 0, PF_BC_AIR = 1 };
+enum pf_precision { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
+enum pf_field { PF_FIELD_U = 0, PF_FIELD_V = 1, PF_FIELD_W = 2, PF_FIELD_P = 3, PF_FIELD_POROSITY = 4 };
+
+/* Everything one case needs: the controlDict variables (src/omp_parallel/lib/global.f90:
+ * 17-26) after the derivations of grid_conditions* (dx = width/(m-1), dt = time/istep_max,
+ * src/omp_parallel/lib/grid.f90:50-52, 163-166, 297-300). */
+typedef struct pf_desc {
+    int32_t dim;              /* 2 or 3 */
+    int32_t m, n, l;          /* interior extents (l = 1 in 2D) */
+    int32_t bc;               /* pf_bc */
+    int32_t nonslip;          /* wall-force term on/off */
+    int32_t iter_max;         /* FP64: fixed SOR iterations per step (reference semantics) */
+    int32_t face[6];          /* air: top,bottom,east,west,north,south; 0 wall,1 inlet,2 outlet */
+    int32_t precision;        /* pf_precision */
+    int32_t inner_iters;      /* mixed: FP32 SOR iterations per defect-correction cycle */
+    int32_t max_cycles;       /* mixed: cap on cycles per step (0 = iter_max/inner_iters) */
+    double dx, dy, dz, dt;
+    double xnue, xlambda, density, thickness;
+    double inlet_velocity, outlet_pressure, aoa;
+    double omega;             /* relux_factor */
+    double residual_tol;      /* mixed: stop when the FP64 residual norm <= tol (0 = fixed count) */
+} pf_desc;
+
+/* Per-step report. sor_error = the reference's "SOR iteration no. .. -- p error:" value
+ * (running max of |p - p_old|, ibm_3d_uniform_omp_cpu.f90:575-583). */
+typedef struct pf_stats {
+    double sor_error;
+    double residual;          /* mixed: final FP64 residual norm of the step (FP64 path: 0) */
+    int32_t sor_iters;        /* SOR iterations executed in the step */
+    int32_t cycles;           /* mixed: defect-correction cycles (FP64 path: 0) */
+} pf_stats;
+
+typedef struct pf_ctx pf_ctx;
+
+/* Replaces the static arrays and set-up of program main (ibm_3d_uniform_omp_cpu.f90:13-19,
+ * 42-59): validates the descriptor and allocates every device field. */
+int pf_create(const pf_desc *desc, pf_ctx **out);
+int pf_destroy(pf_ctx *ctx);
+const char *pf_last_error(const pf_ctx *ctx);   /* NULL ctx: the last pf_create failure */
+int pf_abi_version(void);
+
+/* grid_conditions / grid_conditions_wall / grid_conditions_yz_periodic
+ * (src/omp_parallel/lib/grid.f90:36-47 + 92-106, 151-158 + 215-243, 285-292 + 349-377):
+ * raw interior porosity (i fastest, m*n[*l] values) clamped to threshold, ghosts filled;
+ * then the time-invariant Poisson face coefficients are built once on the device. */
+int pf_set_porosity(pf_ctx *ctx, const double *raw_interior, double threshold);
+
+/* initial_conditions followed by boundary (ibm_3d_uniform_omp_cpu.f90:68-71, :756-793;
+ * 2D :67-70, :542-568; air ibm_3d_air_condition_omp_cpu.f90:1174-1211). */
+int pf_initial(pf_ctx *ctx);
+
+/* Host <-> device copies of one field in the reference's host storage (see top). */
+int pf_upload(pf_ctx *ctx, int field, const double *host, int64_t ldi, int64_t ldj);
+int pf_download(pf_ctx *ctx, int field, double *host, int64_t ldi, int64_t ldj);
+
+/* nsteps iterations of the time loop body (ibm_3d_uniform_omp_cpu.f90:85-128: copy to
+ * *_old, solve_p, projection, boundary; 2D :84-118; air :99-142), device resident.
+ * stats (may be NULL) receives nsteps entries.  Returns after the work is queued and the
+ * statistics are read back. */
+int pf_step(pf_ctx *ctx, int nsteps, pf_stats *stats);
+
+/* output_divergent_2d/3d (src/omp_parallel/lib/output.f90:217-224, 929-940): the
+ * porosity-weighted flux divergence of the current velocity, interior only, i fastest. */
+int pf_divergence(pf_ctx *ctx, double *out_interior);
+
+/* Device bytes the context holds (admission control sizes cases with this). */
+int64_t pf_footprint_bytes(const pf_desc *desc);
+
+/* Wait for all queued work of the context. */
+int pf_sync(pf_ctx *ctx);
+
+/* Measurement hooks (no reference counterpart: the reference prints wall-clock stamps
+ * only, src/omp_parallel/lib/utils.f90:7-16).  With timing on, pf_step brackets each
+ * step's SOR sweeps with HIP events on the context's stream; pf_last_sweep_time_us then
+ * gives the average time per colour-sweep launch of the last pf_step (gaps between the
+ * back-to-back launches included) and the number of launches it covers. */
+int pf_set_timing(pf_ctx *ctx, int on);
+int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,456 @@
+// pf_api.hip -- the C-ABI of include/pixelflow.h: contexts, host<->device staging and the
+// per-step launch sequence on one HIP stream.  Everything stays resident in HBM between
+// calls; the host only moves data at set-up and when a caller asks for a field.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pixelflow.h"
+#include "pf_kernels.h"
+
+struct pf_ctx {
+    pf_desc d{};
+    PfGeom g{};
+    pf::StepArgs a{};
+    hipStream_t st = nullptr;
+    int device = 0;
+    // natural layout
+    double *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
+    double *phi = nullptr, *div = nullptr, *dout = nullptr;
+    // colour-split layout (2 * HS doubles each)
+    double *P = nullptr, *BB = nullptr, *AW = nullptr, *AS = nullptr, *AB = nullptr;
+    unsigned long long *err = nullptr;       // one entry per step of the last pf_step
+    unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
+    int err_cap = 0, nparts = 0;
+    bool have_phi = false, have_init = false;
+    bool timing = false;
+    std::vector<hipEvent_t> evs;   // timing mode: one (start, end) pair per step around the SOR
+    double sweep_us_avg = 0.0;
+    int64_t sweep_launches = 0;
+    std::string msg;
+};
+
+static thread_local std::string g_create_error;
+
+#define PF_TRY_HIP(ctx, call)                                                                   \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (ctx)->msg = std::string(#call) + ": " + hipGetErrorString(e_);                     \
+            return PF_EHIP;                                                                     \
+        }                                                                                       \
+    } while (0)
+
+static int fail(pf_ctx *c, int code, const std::string &m) {
+    c->msg = m;
+    return code;
+}
+
+static bool desc_ok(const pf_desc *d, std::string &why) {
+    if (!d) { why = "null descriptor"; return false; }
+    if (d->dim != 2 && d->dim != 3) { why = "dim must be 2 or 3"; return false; }
+    if (d->m < 3 || d->n < 2 || (d->dim == 3 && d->l < 2)) { why = "extents too small (m>=3, n>=2, l>=2)"; return false; }
+    if (d->bc != PF_BC_UNIFORM && d->bc != PF_BC_AIR) { why = "bc must be PF_BC_UNIFORM or PF_BC_AIR"; return false; }
+    if (d->bc == PF_BC_AIR && d->dim != 3) { why = "the air-conditioning solver is 3D"; return false; }
+    for (int f = 0; f < 6; ++f)
+        if (d->face[f] < 0 || d->face[f] > 2) { why = "face codes must be 0, 1 or 2"; return false; }
+    if (d->iter_max < 0) { why = "iter_max < 0"; return false; }
+    if (d->precision != PF_PREC_FP64) { why = "only PF_PREC_FP64 in this build of the single-case context"; return false; }
+    if (!(d->dx > 0) || !(d->dy > 0) || (d->dim == 3 && !(d->dz > 0)) || !(d->dt > 0) || !(d->density != 0)) {
+        why = "dx, dy, dz, dt must be > 0 and density != 0";
+        return false;
+    }
+    const double cells = (double)d->m * d->n * (d->dim == 3 ? d->l : 1);
+    if (cells > 2.0e9) { why = "grid too large"; return false; }
+    return true;
+}
+
+extern "C" int pf_abi_version(void) { return PF_ABI_VERSION; }
+
+extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
+    std::string why;
+    if (!desc_ok(d, why)) return -1;
+    PfGeom g = pf_make_geom(d->dim, d->m, d->n, d->l);
+    const int nat = (d->dim == 3) ? 8 : 6;
+    const int spl = (d->dim == 3) ? 5 : 4;
+    const int64_t interior = (int64_t)d->m * d->n * (d->dim == 3 ? d->l : 1);
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + 4096;
+}
+
+static int alloc(pf_ctx *c, double **p, int64_t n) {
+    hipError_t e = hipMalloc((void **)p, (size_t)n * sizeof(double));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, PF_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(*p, 0, (size_t)n * sizeof(double), c->st);
+    if (e != hipSuccess) return fail(c, PF_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    return PF_OK;
+}
+
+static void release(pf_ctx *c) {
+    double **all[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->phi, &c->div, &c->dout,
+                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB};
+    for (double **p : all)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (c->err) { (void)hipFree(c->err); c->err = nullptr; }
+    if (c->err_part) { (void)hipFree(c->err_part); c->err_part = nullptr; }
+    for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+    c->evs.clear();
+    if (c->st) (void)hipStreamDestroy(c->st);
+    c->st = nullptr;
+}
+
+extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
+    if (!out) return PF_EINVAL;
+    *out = nullptr;
+    std::string why;
+    if (!desc_ok(desc, why)) {
+        g_create_error = why;
+        return PF_EINVAL;
+    }
+    pf_ctx *c = new (std::nothrow) pf_ctx();
+    if (!c) return PF_ENOMEM;
+    c->d = *desc;
+    c->g = pf_make_geom(desc->dim, desc->m, desc->n, desc->l);
+    pf::StepArgs &a = c->a;
+    a.bc = desc->bc;
+    a.nonslip = desc->nonslip ? 1 : 0;
+    for (int f = 0; f < 6; ++f) a.face[f] = desc->face[f];
+    a.dt = desc->dt; a.dx = desc->dx; a.dy = desc->dy; a.dz = (desc->dim == 3) ? desc->dz : 1.0;
+    a.xnue = desc->xnue; a.xlambda = desc->xlambda; a.density = desc->density; a.thickness = desc->thickness;
+    a.omega = desc->omega;
+    a.omega_c = 1. - desc->omega;
+    // inlet components of boundary (3D :695-696 uses AoA/1300, 2D :478-479 uses AoA/180)
+    const double pi = std::atan(1.) * 4.;
+    const double ang = (desc->dim == 3) ? ((desc->aoa / 1300.) * pi) : ((desc->aoa / 180.) * pi);
+    a.u_in = desc->inlet_velocity * std::cos(ang);
+    a.v_in = desc->inlet_velocity * std::sin(ang);
+    a.inlet_velocity = desc->inlet_velocity;
+    a.outlet_pressure = desc->outlet_pressure;
+
+    int rc;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_error = std::string("HIP init: ") + hipGetErrorString(e);
+        release(c);
+        delete c;
+        return PF_EHIP;
+    }
+    const PfGeom &g = c->g;
+    const int64_t interior = (int64_t)g.m * g.n * g.l;
+    double **nat[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->phi, &c->div};
+    for (double **p : nat) {
+        if ((p == &c->w || p == &c->wo) && g.dim == 2) continue;
+        if ((rc = alloc(c, p, g.NS)) != PF_OK) goto bad;
+    }
+    if ((rc = alloc(c, &c->dout, interior)) != PF_OK) goto bad;
+    {
+        double **spl[] = {&c->P, &c->BB, &c->AW, &c->AS, &c->AB};
+        for (double **p : spl) {
+            if (p == &c->AB && g.dim == 2) continue;
+            if ((rc = alloc(c, p, 2 * g.HS)) != PF_OK) goto bad;
+        }
+    }
+    c->nparts = pf::sor_blocks(g);
+    if (hipMalloc((void **)&c->err_part, sizeof(unsigned long long) * (size_t)c->nparts) != hipSuccess) {
+        rc = fail(c, PF_ENOMEM, "hipMalloc of the error slots failed");
+        goto bad;
+    }
+    pf::launch_clear_u64(c->st, c->err_part, c->nparts);
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        rc = fail(c, PF_EHIP, "stream synchronize failed after allocation");
+        goto bad;
+    }
+    *out = c;
+    return PF_OK;
+bad:
+    g_create_error = c->msg;
+    release(c);
+    delete c;
+    return rc;
+}
+
+extern "C" int pf_destroy(pf_ctx *c) {
+    if (!c) return PF_EINVAL;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    release(c);
+    delete c;
+    return PF_OK;
+}
+
+extern "C" const char *pf_last_error(const pf_ctx *c) {
+    return c ? c->msg.c_str() : g_create_error.c_str();
+}
+
+extern "C" int pf_sync(pf_ctx *c) {
+    if (!c) return PF_EINVAL;
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return PF_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Porosity: grid_conditions* (src/omp_parallel/lib/grid.f90) on the host, once.
+// ------------------------------------------------------------------------------------
+extern "C" int pf_set_porosity(pf_ctx *c, const double *raw, double threshold) {
+    if (!c || !raw) return PF_EINVAL;
+    const PfGeom &g = c->g;
+    const int m = g.m, n = g.n, l = g.l;
+    std::vector<double> phi((size_t)g.NS, 0.0);
+    auto at = [&](int i, int j, int k) -> double & { return phi[(size_t)pf_off(g, i, j, k)]; };
+    const int kmin = (g.dim == 3) ? 1 : 0, kmax = (g.dim == 3) ? l : 0;
+    for (int k = kmin; k <= kmax; ++k)
+        for (int j = 1; j <= n; ++j)
+            for (int i = 1; i <= m; ++i) {
+                const double vraw = raw[(size_t)(i - 1) + (size_t)m * ((size_t)(j - 1) + (size_t)n * (size_t)(k - kmin))];
+                at(i, j, k) = (vraw > threshold) ? vraw : threshold;   // max(poro_val, threshold)
+            }
+    if (g.dim == 2) {                                   // grid.f90:94-105
+        for (int j = 1; j <= n + 1; ++j) { at(0, j, 0) = at(1, j, 0); at(m + 1, j, 0) = at(m, j, 0); }
+        for (int i = 0; i <= m + 1; ++i) { at(i, 0, 0) = at(i, n, 0); at(i, n + 1, 0) = at(i, 1, 0); }
+    } else if (c->d.bc == PF_BC_AIR) {                  // grid.f90:217-242 (zero gradient)
+        for (int j = 0; j <= n + 1; ++j)
+            for (int k = 0; k <= l + 1; ++k) { at(0, j, k) = at(1, j, k); at(m + 1, j, k) = at(m, j, k); }
+        for (int i = 0; i <= m + 1; ++i)
+            for (int k = 0; k <= l + 1; ++k) { at(i, 0, k) = at(i, 1, k); at(i, n + 1, k) = at(i, n, k); }
+        for (int i = 0; i <= m + 1; ++i)
+            for (int j = 0; j <= n + 1; ++j) { at(i, j, 0) = at(i, j, 1); at(i, j, l + 1) = at(i, j, l); }
+    } else {                                            // grid.f90:351-377 (x outlet, y/z periodic)
+        for (int j = 1; j <= n + 1; ++j)
+            for (int k = 1; k <= l + 1; ++k) { at(0, j, k) = at(1, j, k); at(m + 1, j, k) = at(m, j, k); }
+        for (int i = 0; i <= m + 1; ++i)
+            for (int k = 0; k <= l + 1; ++k) { at(i, 0, k) = at(i, n, k); at(i, n + 1, k) = at(i, 1, k); }
+        for (int i = 0; i <= m + 1; ++i)
+            for (int j = 0; j <= n + 1; ++j) { at(i, j, 0) = at(i, j, l); at(i, j, l + 1) = at(i, j, 1); }
+    }
+    PF_TRY_HIP(c, hipMemcpyAsync(c->phi, phi.data(), phi.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
+    PF_TRY_HIP(c, hipGetLastError());
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->have_phi = true;
+    return PF_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Field staging between the reference's host storage and the device layouts.
+// ------------------------------------------------------------------------------------
+static double *field_ptr(pf_ctx *c, int f, bool &split) {
+    split = false;
+    switch (f) {
+        case PF_FIELD_U: return c->u;
+        case PF_FIELD_V: return c->v;
+        case PF_FIELD_W: return c->g.dim == 3 ? c->w : nullptr;
+        case PF_FIELD_P: split = true; return c->P;
+        case PF_FIELD_POROSITY: return c->phi;
+        default: return nullptr;
+    }
+}
+
+extern "C" int pf_upload(pf_ctx *c, int field, const double *host, int64_t ldi, int64_t ldj) {
+    if (!c || !host) return PF_EINVAL;
+    const PfGeom &g = c->g;
+    if (ldi < g.m + 2 || ldj < g.n + 2) return fail(c, PF_EINVAL, "leading dimensions smaller than m+2, n+2");
+    bool split;
+    double *dst = field_ptr(c, field, split);
+    if (!dst) return fail(c, PF_EINVAL, "unknown field");
+    const int64_t size = split ? 2 * g.HS : g.NS;
+    std::vector<double> stage((size_t)size, 0.0);
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    if (split) PF_TRY_HIP(c, hipMemcpy(stage.data(), dst, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.Z; ++k)
+        for (int j = 0; j < g.Y; ++j)
+            for (int i = 0; i < g.m + 2; ++i) {
+                const double val = host[(size_t)i + (size_t)ldi * ((size_t)j + (size_t)ldj * (size_t)k)];
+                stage[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
+            }
+    PF_TRY_HIP(c, hipMemcpy(dst, stage.data(), (size_t)size * sizeof(double), hipMemcpyHostToDevice));
+    if (field == PF_FIELD_POROSITY) {
+        pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        c->have_phi = true;
+    }
+    return PF_OK;
+}
+
+extern "C" int pf_download(pf_ctx *c, int field, double *host, int64_t ldi, int64_t ldj) {
+    if (!c || !host) return PF_EINVAL;
+    const PfGeom &g = c->g;
+    if (ldi < g.m + 2 || ldj < g.n + 2) return fail(c, PF_EINVAL, "leading dimensions smaller than m+2, n+2");
+    bool split;
+    double *src = field_ptr(c, field, split);
+    if (!src) return fail(c, PF_EINVAL, "unknown field");
+    const int64_t size = split ? 2 * g.HS : g.NS;
+    std::vector<double> stage((size_t)size);
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    PF_TRY_HIP(c, hipMemcpy(stage.data(), src, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.Z; ++k)
+        for (int j = 0; j < g.Y; ++j)
+            for (int i = 0; i < g.m + 2; ++i)
+                host[(size_t)i + (size_t)ldi * ((size_t)j + (size_t)ldj * (size_t)k)] =
+                    stage[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))];
+    return PF_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// initial_conditions + boundary
+// ------------------------------------------------------------------------------------
+extern "C" int pf_initial(pf_ctx *c) {
+    if (!c) return PF_EINVAL;
+    if (!c->have_phi) return fail(c, PF_ESTATE, "pf_initial before pf_set_porosity");
+    const PfGeom &g = c->g;
+    const pf_desc &d = c->d;
+    double u0 = 0., v0 = 0.;
+    const double pi = std::atan(1.) * 4.;
+    if (d.bc == PF_BC_UNIFORM) {
+        // 3D initial_conditions uses AoA/360 (:782-783); 2D uses AoA/180 (:560-561)
+        const double ang = (g.dim == 3) ? ((d.aoa / 360.) * pi) : ((d.aoa / 180.) * pi);
+        u0 = d.inlet_velocity * std::cos(ang);
+        v0 = d.inlet_velocity * std::sin(ang);
+    }
+    std::vector<double> nat((size_t)g.NS, 0.0), spl((size_t)(2 * g.HS), 0.0);
+    const int kmin = (g.dim == 3) ? 1 : 0, kmax = (g.dim == 3) ? g.l : 0;
+    auto fill = [&](double val, bool split) {
+        std::vector<double> &buf = split ? spl : nat;
+        std::fill(buf.begin(), buf.end(), 0.0);
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = 1; j <= g.n; ++j)
+                for (int i = 1; i <= g.m; ++i)
+                    buf[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
+    };
+    fill(u0, false);
+    PF_TRY_HIP(c, hipMemcpy(c->u, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
+    fill(v0, false);
+    PF_TRY_HIP(c, hipMemcpy(c->v, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (g.dim == 3) {
+        fill(0.0, false);
+        PF_TRY_HIP(c, hipMemcpy(c->w, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    fill(d.outlet_pressure, true);
+    PF_TRY_HIP(c, hipMemcpy(c->P, spl.data(), spl.size() * sizeof(double), hipMemcpyHostToDevice));
+    pf::launch_boundary(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P);
+    PF_TRY_HIP(c, hipGetLastError());
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    c->have_init = true;
+    return PF_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// The time step
+// ------------------------------------------------------------------------------------
+static void enqueue_sor(pf_ctx *c, unsigned long long *slot) {
+    const PfGeom &g = c->g;
+    const bool periodic = (c->d.bc == PF_BC_UNIFORM);
+    const int fy = periodic && (g.n % 2 == 0);
+    const int fz = periodic && g.dim == 3 && (g.l % 2 == 0);
+    const int ry = periodic && !fy;
+    const int rz = periodic && g.dim == 3 && !fz;
+    // 3D sweeps (i+j+k) even first (:493), 2D sweeps (i+j) odd first (:339)
+    const int first = (g.dim == 3) ? 0 : 1;
+    for (int it = 0; it < c->d.iter_max; ++it) {
+        for (int half = 0; half < 2; ++half) {
+            pf::SweepArgs s;
+            s.colour = half == 0 ? first : 1 - first;
+            s.fuse_y = fy;
+            s.fuse_z = fz;
+            // 3D reduces |dp| after the second sweep only (:575-583); 2D after both (:351, :385)
+            s.track_err = (g.dim == 2) || (half == 1);
+            pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
+            if (ry || rz) pf::launch_ghost_refresh(c->st, g, ry, rz, c->P);
+        }
+    }
+}
+
+extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
+    if (!c || nsteps < 0) return PF_EINVAL;
+    if (!c->have_init) return fail(c, PF_ESTATE, "pf_step before pf_initial");
+    if (nsteps == 0) return PF_OK;
+    const PfGeom &g = c->g;
+    if (c->err_cap < nsteps) {
+        if (c->err) PF_TRY_HIP(c, hipFree(c->err));
+        c->err = nullptr;
+        PF_TRY_HIP(c, hipMalloc((void **)&c->err, sizeof(unsigned long long) * (size_t)nsteps));
+        c->err_cap = nsteps;
+    }
+    pf::launch_clear_u64(c->st, c->err, nsteps);
+    if (c->timing) {
+        while (c->evs.size() < (size_t)(2 * nsteps)) {
+            hipEvent_t e;
+            PF_TRY_HIP(c, hipEventCreate(&e));
+            c->evs.push_back(e);
+        }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        std::swap(c->u, c->uo);
+        std::swap(c->v, c->vo);
+        std::swap(c->w, c->wo);
+        pf::launch_div(c->st, g, c->a, c->uo, c->vo, c->wo, c->div);
+        pf::launch_predict(c->st, g, c->a, c->uo, c->vo, c->wo, c->div, c->phi, c->u, c->v, c->w);
+        pf::launch_rhs(c->st, g, c->a, c->u, c->v, c->w, c->phi, c->AW, c->AS, c->AB, c->P, c->BB);
+        if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s)], c->st));
+        enqueue_sor(c, c->err_part);
+        pf::launch_reduce_err(c->st, c->err_part, c->nparts, c->err + s);
+        if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s + 1)], c->st));
+        pf::launch_project(c->st, g, c->a, c->P, c->u, c->v, c->w);
+        pf::launch_boundary(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P);
+        PF_TRY_HIP(c, hipGetLastError());
+    }
+    std::vector<unsigned long long> e((size_t)nsteps);
+    PF_TRY_HIP(c, hipMemcpyAsync(e.data(), c->err, sizeof(unsigned long long) * (size_t)nsteps, hipMemcpyDeviceToHost, c->st));
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    if (c->timing && c->d.iter_max > 0) {
+        double sweep_ms = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float ms = 0.f;
+            PF_TRY_HIP(c, hipEventElapsedTime(&ms, c->evs[(size_t)(2 * s)], c->evs[(size_t)(2 * s + 1)]));
+            sweep_ms += ms;
+        }
+        c->sweep_launches = (int64_t)nsteps * 2 * c->d.iter_max;
+        c->sweep_us_avg = sweep_ms * 1000.0 / (double)c->sweep_launches;
+    }
+    bool finite = true;
+    for (int s = 0; s < nsteps; ++s) {
+        double v;
+        std::memcpy(&v, &e[(size_t)s], sizeof(double));
+        if (!std::isfinite(v)) finite = false;
+        if (stats) {
+            stats[s].sor_error = v;
+            stats[s].residual = 0.0;
+            stats[s].sor_iters = c->d.iter_max;
+            stats[s].cycles = 0;
+        }
+    }
+    if (!finite) return fail(c, PF_EDIVERGED, "non-finite SOR error: the run diverged");
+    return PF_OK;
+}
+
+extern "C" int pf_divergence(pf_ctx *c, double *out) {
+    if (!c || !out) return PF_EINVAL;
+    const PfGeom &g = c->g;
+    pf::launch_divergence_out(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->dout);
+    PF_TRY_HIP(c, hipGetLastError());
+    const size_t n = (size_t)g.m * g.n * g.l;
+    PF_TRY_HIP(c, hipMemcpyAsync(out, c->dout, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return PF_OK;
+}
+
+extern "C" int pf_set_timing(pf_ctx *c, int on) {
+    if (!c) return PF_EINVAL;
+    c->timing = on != 0;
+    return PF_OK;
+}
+
+extern "C" int pf_last_sweep_time_us(const pf_ctx *c, double *avg_us, int64_t *launches) {
+    if (!c || !avg_us || !launches) return PF_EINVAL;
+    *avg_us = c->sweep_us_avg;
+    *launches = c->sweep_launches;
+    return PF_OK;
+}

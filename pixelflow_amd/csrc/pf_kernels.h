@@ -1,0 +1,56 @@
+// pf_kernels.h -- launch interface of the gfx950 kernels (host side, HIP types only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pf_layout.h"
+
+#ifndef PF_BC_UNIFORM
+#define PF_BC_UNIFORM 0
+#define PF_BC_AIR 1
+#endif
+
+namespace pf {
+
+// Per-case constants of the step, passed by value to every kernel.
+struct StepArgs {
+    int bc;             // PF_BC_UNIFORM / PF_BC_AIR
+    int nonslip;
+    int face[6];        // air face codes: top, bottom, east, west, north, south
+    double dt, dx, dy, dz;
+    double xnue, xlambda, density, thickness;
+    double omega, omega_c;        // relux_factor and (1 - relux_factor)
+    double u_in, v_in;            // inlet velocity components (host libm cos/sin)
+    double inlet_velocity, outlet_pressure;
+};
+
+struct SweepArgs {
+    int colour;         // cells with (i+j+k) & 1 == colour are updated
+    int fuse_y, fuse_z; // write the periodic ghost copies in the sweep (even extents)
+    int track_err;      // reduce max |dp| into the step's error slot
+};
+
+void launch_faces(hipStream_t st, const PfGeom &g, const double *phi, double *AW, double *AS, double *AB,
+                  double dt, double dx, double dy, double dz);
+void launch_div(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *uo, const double *vo,
+                const double *wo, double *div);
+void launch_predict(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *uo, const double *vo,
+                    const double *wo, const double *div, const double *phi, double *u, double *v, double *w);
+void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *u, const double *v,
+                const double *w, const double *phi, const double *AW, const double *AS, const double *AB,
+                const double *P, double *BB);
+void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
+                const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
+                unsigned long long *err_slot);
+void launch_ghost_refresh(hipStream_t st, const PfGeom &g, int do_y, int do_z, double *P);
+void launch_project(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *P, double *u, double *v,
+                    double *w);
+void launch_boundary(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, double *u, double *v,
+                     double *w, double *P);
+void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,
+                           const double *v, const double *w, double *out);
+void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
+int sor_blocks(const PfGeom &g);   // blocks of one colour-sweep launch (= error slots)
+void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err);
+
+}  // namespace pf

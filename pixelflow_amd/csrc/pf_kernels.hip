@@ -1,0 +1,807 @@
+// pf_kernels.hip -- gfx950 kernels of the FP64 time step (2D and 3D, uniform and air).
+//
+// Every arithmetic expression keeps the reference's Fortran evaluation order (a*b/c*d is
+// ((a*b)/c)*d, -a-b is (-a)-b) and is compiled with -ffp-contract=off and IEEE division,
+// so a cell's result is the same IEEE FP64 value the reference computes.  Kernels are
+// HBM-bandwidth bound stencils: one thread per cell, x-contiguous wave rows, no MFMA.
+#include <hip/hip_runtime.h>
+
+#include "pf_kernels.h"
+
+#define PF_SMALL 1.e-6
+#define PF_ALPHA 32.0
+
+namespace pf {
+
+__device__ __forceinline__ double fmax_ref(double a, double b) { return (a > b) ? a : b; }
+
+// ------------------------------------------------------------------------------------
+// Poisson face coefficients (time invariant; solve_p 3D :390-400, 2D :265-268).
+// AW(i) = dt*max(small,(phi(i)+phi(i-1))*0.5)/dx/dx == aw(i) == ae(i-1), likewise AS, AB.
+// ------------------------------------------------------------------------------------
+__global__ void k_faces(PfGeom g, const double *__restrict__ phi, double *__restrict__ AW,
+                        double *__restrict__ AS, double *__restrict__ AB, double dt, double dx,
+                        double dy, double dz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    const int64_t o = pf_off(g, i, j, k);
+    const int64_t so = pf_soff(g, i, j, k);
+    const double pc = phi[o];
+    AW[so] = ((dt * fmax_ref(PF_SMALL, (pc + phi[o - 1]) * 0.5)) / dx) / dx;
+    AS[so] = ((dt * fmax_ref(PF_SMALL, (pc + phi[o - g.PX]) * 0.5)) / dy) / dy;
+    if (g.dim == 3)
+        AB[so] = ((dt * fmax_ref(PF_SMALL, (pc + phi[o - (int64_t)g.PX * g.Y]) * 0.5)) / dz) / dz;
+}
+
+// ------------------------------------------------------------------------------------
+// boundary_matrix row edits, replayed per cell in the reference's order.
+// Uniform: inlet fold at i=1 then outlet row at i=m (3D :636-659, 2D :424-436).
+// Air: top, bottom, east, west, north, south (ibm_3d_air_condition_omp_cpu.f90:685-864).
+// WITH_BB adds the outlet terms to *bb (p ghosts read from the colour-split P); the top
+// outlet assigns bb(i,j,l) = bb(i,j,1) + at*p(i,j,l+1) as written there (:702).
+// ------------------------------------------------------------------------------------
+struct Coef {
+    double ae, aw, an, as, at, ab;
+};
+
+__device__ __forceinline__ void zero6(Coef &c) { c.ae = 0.; c.aw = 0.; c.an = 0.; c.as = 0.; c.at = 0.; c.ab = 0.; }
+
+__device__ __forceinline__ bool is_edge_row(const PfGeom &g, const StepArgs &a, int i, int j, int k) {
+    if (a.bc == PF_BC_UNIFORM) return i == 1 || i == g.m;
+    return i == 1 || i == g.m || j == 1 || j == g.n || k == 1 || k == g.l;
+}
+
+template <bool WITH_BB>
+__device__ void row_rules(const PfGeom &g, const StepArgs &a, int i, int j, int k, Coef &c,
+                          double *bb, const double *P, const double *phi, double bb_k1) {
+    if (a.bc == PF_BC_UNIFORM) {
+        if (i == 1) { c.ae = c.ae + c.aw; c.aw = 0.; }
+        if (i == g.m) {
+            if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, g.m + 1, j, k)]);
+            zero6(c);
+        }
+        return;
+    }
+    const int m = g.m, n = g.n, l = g.l;
+    const double pcell = phi[pf_off(g, i, j, k)];
+    const bool fluid = !(pcell < 0.9);
+    if (k == l) {
+        if (a.face[0] == 0 || a.face[0] == 1 || !fluid) { c.ab = c.ab + c.at; c.at = 0.; }
+        else {
+            if (WITH_BB) *bb = bb_k1 + (c.at * P[pf_soff(g, i, j, l + 1)]);
+            zero6(c);
+        }
+    }
+    if (k == 1) {
+        if (a.face[1] == 0 || a.face[1] == 1 || !fluid) { c.at = c.at + c.ab; c.ab = 0.; }
+        else { if (WITH_BB) *bb = *bb + (c.ab * P[pf_soff(g, i, j, 0)]); zero6(c); }
+    }
+    if (i == m) {
+        if (a.face[2] == 0 || a.face[2] == 1 || !fluid) { c.aw = c.aw + c.ae; c.ae = 0.; }
+        else { if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, m + 1, j, k)]); zero6(c); }
+    }
+    if (i == 1) {
+        if (a.face[3] == 0 || a.face[3] == 1 || !fluid) { c.ae = c.ae + c.aw; c.aw = 0.; }
+        else { if (WITH_BB) *bb = *bb + (c.aw * P[pf_soff(g, 0, j, k)]); zero6(c); }
+    }
+    if (j == n) {
+        if (a.face[4] == 0 || a.face[4] == 1 || !fluid) { c.as = c.as + c.an; c.an = 0.; }
+        else { if (WITH_BB) *bb = *bb + (c.an * P[pf_soff(g, i, n + 1, k)]); zero6(c); }
+    }
+    if (j == 1) {
+        if (a.face[5] == 0 || a.face[5] == 1 || !fluid) { c.an = c.an + c.as; c.as = 0.; }
+        else { if (WITH_BB) *bb = *bb + (c.as * P[pf_soff(g, i, 0, k)]); zero6(c); }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// div of the old velocity with its ghost values (3D :186-222, air :189-237, 2D :172-194).
+// Ghost values are recomputed from the wrapped interior cell rather than copied, so one
+// launch produces interior and ghosts with no intra-kernel dependency.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double div3_at(const PfGeom &g, const StepArgs &a, const double *uo,
+                                          const double *vo, const double *wo, int i, int j, int k) {
+    const int64_t o = pf_off(g, i, j, k);
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    return ((((uo[o + 1] - uo[o - 1]) / a.dx) * 0.5) + (((vo[o + sy] - vo[o - sy]) / a.dy) * 0.5)) +
+           (((wo[o + sz] - wo[o - sz]) / a.dz) * 0.5);
+}
+__device__ __forceinline__ double div2_at(const PfGeom &g, const StepArgs &a, const double *uo,
+                                          const double *vo, int i, int j) {
+    const int64_t o = pf_off(g, i, j, 0);
+    // the 2D code divides the v-difference by dx (ibm_2d_uniform_omp_cpu.f90:175-176)
+    return (((uo[o + 1] - uo[o - 1]) / a.dx) * .5) + (((vo[o + g.PX] - vo[o - g.PX]) / a.dx) * .5);
+}
+
+__global__ void k_div(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
+                      const double *__restrict__ wo, double *__restrict__ div) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = (g.dim == 3) ? (int)blockIdx.z : 0;
+    const int m = g.m, n = g.n, l = g.l;
+    if (i > m + 1 || j > n + 1) return;
+    const bool gi = (i == 0 || i == m + 1), gj = (j == 0 || j == n + 1);
+    const bool gk = (g.dim == 3) && (k == 0 || k == l + 1);
+    const int ng = (int)gi + (int)gj + (int)gk;
+    if (ng == 0) {
+        div[pf_off(g, i, j, k)] = (g.dim == 3) ? div3_at(g, a, uo, vo, wo, i, j, k) : div2_at(g, a, uo, vo, i, j);
+        return;
+    }
+    if (ng > 1) return;  // edges and corners are never written nor read
+    double val = 0.;
+    if (a.bc == PF_BC_UNIFORM && !gi) {  // y / z periodic ghosts; x ghosts are 0
+        const int jj = (j == 0) ? n : (j == n + 1 ? 1 : j);
+        const int kk = (g.dim == 3) ? ((k == 0) ? l : (k == l + 1 ? 1 : k)) : 0;
+        val = (g.dim == 3) ? div3_at(g, a, uo, vo, wo, i, jj, kk) : div2_at(g, a, uo, vo, i, jj);
+    }
+    div[pf_off(g, i, j, k)] = val;
+}
+
+// ------------------------------------------------------------------------------------
+// Momentum predictor (3D :228-381, air :276-428, 2D :200-258).  Writes every cell of the
+// new velocity: interior cells predicted, ghost cells copied from the old velocity (the
+// reference predicts the interior of u after u_old = u, so the ghosts keep u_old's).
+// ------------------------------------------------------------------------------------
+__global__ void k_predict3(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
+                           const double *__restrict__ wo, const double *__restrict__ div,
+                           const double *__restrict__ po, double *__restrict__ u, double *__restrict__ v,
+                           double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    const int64_t o = pf_off(g, i, j, k);
+    if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1 || k == 0 || k == g.l + 1) {
+        u[o] = uo[o]; v[o] = vo[o]; w[o] = wo[o];
+        return;
+    }
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    const double dt = a.dt, dx = a.dx, dy = a.dy, dz = a.dz, xnue = a.xnue, xlambda = a.xlambda;
+    const double pc = po[o];
+    const double pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy], pT = po[o + sz], pB = po[o - sz];
+    const double dc = div[o];
+    const double uc = uo[o], vc = vo[o], wc = wo[o];
+    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy], uT = uo[o + sz], uB = uo[o - sz];
+    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy], vT = vo[o + sz], vB = vo[o - sz];
+    const double wE = wo[o + 1], wW = wo[o - 1], wN = wo[o + sy], wS = wo[o - sy], wT = wo[o + sz], wB = wo[o - sz];
+    const double wall = PF_ALPHA;
+    {   // u (:233-271)
+        double r = uc - ((((dt * uc) * (uE - uW)) / dx) * 0.5);
+        r = r - ((((dt * vc) * (uN - uS)) / dy) * 0.5);
+        r = r - ((((dt * wc) * (uT - uB)) / dz) * 0.5);
+        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
+        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
+        r = r + ((((dt * xnue) * ((uT - (2. * uc)) + uB)) / dz) / dz);
+        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * 0.5);
+        const double t1 = (((((((uE - uW) / dx) * 0.5) + (((uE - uW) / dx) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
+        const double t2 = (((((((uN - uS) / dy) * 0.5) + (((vE - vW) / dx) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
+        const double t3 = (((((((uT - uB) / dz) * 0.5) + (((wE - wW) / dx) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
+        const double t4 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) {
+            const double th = a.thickness * dx;
+            r = r - (((((((dt * xnue) * uc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
+        }
+        u[o] = r;
+    }
+    {   // v (:286-323)
+        double r = vc - ((((dt * uc) * (vE - vW)) / dx) * 0.5);
+        r = r - ((((dt * vc) * (vN - vS)) / dy) * 0.5);
+        r = r - ((((dt * wc) * (vT - vB)) / dz) * 0.5);
+        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
+        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
+        r = r + ((((dt * xnue) * ((vT - (2. * vc)) + vB)) / dz) / dz);
+        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * 0.5);
+        const double t1 = (((((((vE - vW) / dx) * 0.5) + (((uN - uS) / dy) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
+        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
+        const double t3 = (((((((vT - vB) / dz) * .5) + (((wN - wS) / dy) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
+        const double t4 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) {
+            const double th = a.thickness * dy;
+            r = r - (((((((dt * xnue) * vc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
+        }
+        v[o] = r;
+    }
+    {   // w (:339-376)
+        double r = wc - ((((dt * uc) * (wE - wW)) / dx) * 0.5);
+        r = r - ((((dt * vc) * (wN - wS)) / dy) * 0.5);
+        r = r - ((((dt * wc) * (wT - wB)) / dz) * 0.5);
+        r = r + ((((dt * xnue) * ((wE - (2. * wc)) + wW)) / dx) / dx);
+        r = r + ((((dt * xnue) * ((wN - (2. * wc)) + wS)) / dy) / dy);
+        r = r + ((((dt * xnue) * ((wT - (2. * wc)) + wB)) / dz) / dz);
+        r = r + ((((dt * (xnue + xlambda)) * (div[o + sz] - div[o - sz])) / dz) * 0.5);
+        const double t1 = (((((((wE - wW) / dx) * 0.5) + (((uT - uB) / dz) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
+        const double t2 = (((((((wN - wS) / dy) * 0.5) + (((vT - vB) / dz) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
+        const double t3 = (((((((wT - wB) / dz) * 0.5) + (((wT - wB) / dz) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
+        const double t4 = (((dc * (pT - pB)) / dz) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) {
+            const double th = a.thickness * dz;
+            r = r - (((((((dt * xnue) * wc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
+        }
+        w[o] = r;
+    }
+}
+
+__global__ void k_predict2(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
+                           const double *__restrict__ div, const double *__restrict__ po,
+                           double *__restrict__ u, double *__restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    const int64_t o = pf_off(g, i, j, 0);
+    if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1) {
+        u[o] = uo[o]; v[o] = vo[o];
+        return;
+    }
+    const int64_t sy = g.PX;
+    const double dt = a.dt, dx = a.dx, dy = a.dy, xnue = a.xnue, xlambda = a.xlambda;
+    const double pc = po[o], pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy];
+    const double dc = div[o];
+    const double uc = uo[o], vc = vo[o];
+    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy];
+    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy];
+    {   // u (ibm_2d_uniform_omp_cpu.f90:204-224)
+        double r = uc - (dt * (((uc * (uE - uW)) / dx) / 2.));
+        r = r - (dt * (((vc * (uN - uS)) / dy) / 2.));
+        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
+        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
+        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * .5);
+        const double t1 = (((((((uE - uW) / dx) * .5) + (((uE - uW) / dx) * .5)) * xnue) * (pE - pW)) / dx) * .5;
+        const double t2 = (((((((uN - uS) / dy) * .5) + (((vE - vW) / dx) * .5)) * xnue) * (pN - pS)) / dy) * .5;
+        const double t3 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
+        r = r + ((dt * ((t1 + t2) + t3)) / pc);
+        if (a.nonslip) {
+            const double th = a.thickness * dx;
+            r = r - (((((((dt * xnue) * uc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
+        }
+        u[o] = r;
+    }
+    {   // v (:235-255); the wall force uses dx here as well (:254)
+        double r = vc - (dt * (((uc * (vE - vW)) / dx) / 2.));
+        r = r - (dt * (((vc * (vN - vS)) / dy) / 2.));
+        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
+        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
+        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * .5);
+        const double t1 = (((((((vE - vW) / dx) * .5) + (((uN - uS) / dy) * .5)) * xnue) * (pE - pW)) / dx) * .5;
+        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * .5)) * xnue) * (pN - pS)) / dy) * .5;
+        const double t3 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
+        r = r + ((dt * ((t1 + t2) + t3)) / pc);
+        if (a.nonslip) {
+            const double th = a.thickness * dx;
+            r = r - (((((((dt * xnue) * vc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
+        }
+        v[o] = r;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// RHS bb of the predicted velocity (3D :404-409, 2D :271-274) with the outlet terms of
+// boundary_matrix, written in the colour-split layout.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
+                                         const double *w, const double *po, int i, int j, int k) {
+    const int64_t o = pf_off(g, i, j, k);
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    const double pc = po[o], density = a.density;
+    return (((((((po[o + 1] * u[o]) + (pc * u[o + 1])) * 0.5) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) * 0.5)) * density) / a.dx) +
+            ((((((po[o + sy] * v[o]) + (pc * v[o + sy])) * 0.5) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) * 0.5)) * density) / a.dy)) +
+           ((((((po[o + sz] * w[o]) + (pc * w[o + sz])) * 0.5) - (((po[o - sz] * w[o]) + (pc * w[o - sz])) * 0.5)) * density) / a.dz);
+}
+__device__ __forceinline__ double bb2_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
+                                         const double *po, int i, int j) {
+    const int64_t o = pf_off(g, i, j, 0);
+    const int64_t sy = g.PX;
+    const double pc = po[o], density = a.density;
+    return ((((((po[o + 1] * u[o]) + (pc * u[o + 1])) * 0.5) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) * 0.5)) * density) / a.dx) +
+           ((((((po[o + sy] * v[o]) + (pc * v[o + sy])) * 0.5) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) * 0.5)) * density) / a.dy);
+}
+
+__global__ void k_rhs(PfGeom g, StepArgs a, const double *__restrict__ u, const double *__restrict__ v,
+                      const double *__restrict__ w, const double *__restrict__ po, const double *__restrict__ AW,
+                      const double *__restrict__ AS, const double *__restrict__ AB, const double *__restrict__ P,
+                      double *__restrict__ BB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m || j > g.n) return;
+    double bb = (g.dim == 3) ? bb3_at(g, a, u, v, w, po, i, j, k) : bb2_at(g, a, u, v, po, i, j);
+    if (is_edge_row(g, a, i, j, k)) {
+        Coef c;
+        const int64_t so = pf_soff(g, i, j, k);
+        c.aw = AW[so];
+        c.ae = AW[pf_soff(g, i + 1, j, k)];
+        c.as = AS[so];
+        c.an = AS[pf_soff(g, i, j + 1, k)];
+        c.ab = (g.dim == 3) ? AB[so] : 0.;
+        c.at = (g.dim == 3) ? AB[pf_soff(g, i, j, k + 1)] : 0.;
+        double bbk1 = 0.;
+        if (a.bc == PF_BC_AIR && k == g.l && a.face[0] == 2) bbk1 = bb3_at(g, a, u, v, w, po, i, j, 1);
+        row_rules<true>(g, a, i, j, k, c, &bb, P, po, bbk1);
+    }
+    BB[pf_soff(g, i, j, k)] = bb;
+}
+
+// ------------------------------------------------------------------------------------
+// Red/black SOR half-sweep (3D :492-516 / :549-572, 2D :339-352 / :373-386).
+// One launch updates every interior cell of colour c in place.  This equals the
+// reference's p_old form: all neighbours of a colour-c cell have colour 1-c and are not
+// written during the sweep; ghost cells are read as stored.  With an even periodic extent
+// the ghost copy of a freshly updated boundary cell has the same colour and no cell of
+// this sweep reads it, so the refresh of the reference (:463-480) is fused here as a
+// second store; odd extents use k_ghost_refresh after the sweep instead.
+// The SOR error diagnostic is the max |p - p_old| over the updated cells, reduced per
+// wave with cross-lane shuffles, per block through LDS, then one atomic max per block.
+// ------------------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
+                                             double *__restrict__ Pown, const double *__restrict__ AWo,
+                                             const double *__restrict__ AWx, const double *__restrict__ ASo,
+                                             const double *__restrict__ ASx, const double *__restrict__ ABo,
+                                             const double *__restrict__ ABx, const double *__restrict__ BBo,
+                                             const double *__restrict__ phi, unsigned long long *err_slot) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (DIM == 3) ? (int)blockIdx.z + 1 : 0;
+    const int sh = (s.colour + j + k) & 1;
+    // thread t of a row takes the t-th interior cell of the colour: i = 2t+1 (sh = 1) or 2t+2
+    const int h = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - sh;
+    const int i = 2 * h + sh;
+    const bool active = (j <= g.n) && (i <= g.m);
+    double err = 0.;
+    if (active) {
+        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+        const int64_t o = (int64_t)h + HP * ((int64_t)j + (int64_t)g.Y * k);
+        const double pc = Pown[o];
+        const double pW = Poth[o - 1 + sh], pE = Poth[o + sh];
+        const double pS = Poth[o - HP], pN = Poth[o + HP];
+        Coef c;
+        c.aw = AWo[o];
+        c.ae = AWx[o + sh];
+        c.as = ASo[o];
+        c.an = ASx[o + HP];
+        double pB = 0., pT = 0.;
+        if (DIM == 3) {
+            pB = Poth[o - HPY];
+            pT = Poth[o + HPY];
+            c.ab = ABo[o];
+            c.at = ABx[o + HPY];
+        } else {
+            c.ab = 0.;
+            c.at = 0.;
+        }
+        double ap;
+        if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
+        else ap = (((-c.ae) - c.aw) - c.an) - c.as;
+        if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+        const double bb = BBo[o];
+        double num;
+        if (DIM == 3)
+            num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
+        else
+            num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
+        const double pn = ((num / ap) * a.omega) + (pc * a.omega_c);
+        Pown[o] = pn;
+        if (s.fuse_y) {
+            if (j == g.n) Pown[o - HP * (int64_t)g.n] = pn;          // ghost (i, 0, k)
+            if (j == 1) Pown[o + HP * (int64_t)g.n] = pn;            // ghost (i, n+1, k)
+        }
+        if (DIM == 3 && s.fuse_z) {
+            if (k == g.l) Pown[o - HPY * (int64_t)g.l] = pn;         // ghost (i, j, 0)
+            if (k == 1) Pown[o + HPY * (int64_t)g.l] = pn;           // ghost (i, j, l+1)
+        }
+        err = fabs(pn - pc);
+    }
+    if (!s.track_err) return;
+    // |dp| >= 0, so its IEEE bit pattern orders like an unsigned integer
+    unsigned long long e = __double_as_longlong(err);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) red[lane >> 6] = e;
+    __syncthreads();
+    if (lane == 0) {
+        unsigned long long b = red[0];
+        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+        for (int q = 1; q < nw; ++q) b = (red[q] > b) ? red[q] : b;
+        // one slot per block: no two blocks of a launch touch the same address
+        const int64_t bid = (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        unsigned long long *slot = err_slot + bid;
+        if (b > *slot) *slot = b;
+    }
+}
+
+// Max of the per-block slots of one step into that step's entry; the slots are cleared
+// for the next step.  One block.
+__global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned long long *step_err) {
+    unsigned long long e = 0ull;
+    for (int t = threadIdx.x; t < nslots; t += blockDim.x) {
+        const unsigned long long v = slots[t];
+        e = (v > e) ? v : e;
+        slots[t] = 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = (red[q] > b) ? red[q] : b;
+        *step_err = b;
+    }
+}
+
+// Periodic ghost refresh of p for odd extents (3D :463-480 / :520-537 / :588-605,
+// 2D :322-327 / :356-361 / :392-397): y ghosts for i in 1..m, k in 1..l; z ghosts for
+// i in 1..m, j in 1..n.
+__global__ void k_ghost_refresh(PfGeom g, int do_y, int do_z, double *__restrict__ P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int t = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.m) return;
+    if (do_y && t <= g.l && (g.dim == 3 || t == 1)) {
+        const int k = (g.dim == 3) ? t : 0;
+        P[pf_soff(g, i, 0, k)] = P[pf_soff(g, i, g.n, k)];
+        P[pf_soff(g, i, g.n + 1, k)] = P[pf_soff(g, i, 1, k)];
+    }
+    if (do_z && g.dim == 3 && t <= g.n) {
+        P[pf_soff(g, i, t, 0)] = P[pf_soff(g, i, t, g.l)];
+        P[pf_soff(g, i, t, g.l + 1)] = P[pf_soff(g, i, t, 1)];
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Projection (3D main :115-123, 2D main :108-113).
+// ------------------------------------------------------------------------------------
+__global__ void k_project(PfGeom g, StepArgs a, const double *__restrict__ P, double *__restrict__ u,
+                          double *__restrict__ v, double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m || j > g.n) return;
+    const int64_t o = pf_off(g, i, j, k);
+    const double f = a.dt / a.density;
+    u[o] = u[o] - (((f * (P[pf_soff(g, i + 1, j, k)] - P[pf_soff(g, i - 1, j, k)])) / a.dx) * 0.5);
+    v[o] = v[o] - (((f * (P[pf_soff(g, i, j + 1, k)] - P[pf_soff(g, i, j - 1, k)])) / a.dy) * 0.5);
+    if (g.dim == 3)
+        w[o] = w[o] - (((f * (P[pf_soff(g, i, j, k + 1)] - P[pf_soff(g, i, j, k - 1)])) / a.dz) * 0.5);
+}
+
+// ------------------------------------------------------------------------------------
+// boundary for the uniform-inflow solvers (3D :691-748, 2D :477-501).  The reference
+// applies inlet and outlet rows for interior (j,k), then copies y-periodic ghosts over the
+// full i and k ranges, then z-periodic over full i and j ranges.  The final value of any
+// boundary cell is therefore V(i, wrap(j), wrap(k)), V being the state after the x rows;
+// each thread evaluates that directly, so one launch has no intra-kernel dependency.
+// The launch covers the x planes i in {0,1,m+1}, the y planes and the z planes.
+// ------------------------------------------------------------------------------------
+__global__ void k_boundary_uniform(PfGeom g, StepArgs a, double *__restrict__ u, double *__restrict__ v,
+                                   double *__restrict__ w, double *__restrict__ P) {
+    const int m = g.m, n = g.n, l = g.l;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t Y = g.Y, Z = g.Z, X = m + 2;
+    const int64_t nx = 3 * Y * Z, ny = 2 * X * Z, nz = (g.dim == 3) ? 2 * X * Y : 0;
+    int i, j, k;
+    if (t < nx) {
+        const int sel = (int)(t / (Y * Z));
+        const int64_t r = t % (Y * Z);
+        i = (sel == 0) ? 0 : (sel == 1 ? 1 : m + 1);
+        j = (int)(r % Y);
+        k = (int)(r / Y);
+    } else if (t < nx + ny) {
+        const int64_t r = t - nx;
+        const int sel = (int)(r / (X * Z));
+        const int64_t q = r % (X * Z);
+        j = (sel == 0) ? 0 : n + 1;
+        i = (int)(q % X);
+        k = (int)(q / X);
+    } else if (t < nx + ny + nz) {
+        const int64_t r = t - nx - ny;
+        const int sel = (int)(r / (X * Y));
+        const int64_t q = r % (X * Y);
+        k = (sel == 0) ? 0 : l + 1;
+        i = (int)(q % X);
+        j = (int)(q / X);
+    } else {
+        return;
+    }
+    if (g.dim == 2) k = 0;
+    // an i = 1 cell with interior (j, k) only takes the inlet velocity (its p is untouched)
+    const int jj = (j == 0) ? n : (j == n + 1 ? 1 : j);
+    const int kk = (g.dim == 3) ? ((k == 0) ? l : (k == l + 1 ? 1 : k)) : 0;
+    double uu, vv, ww, pp;
+    if (i == 0 || i == 1) {
+        uu = a.u_in; vv = a.v_in; ww = 0.;
+        pp = (i == 0) ? P[pf_soff(g, 2, jj, kk)] : P[pf_soff(g, 1, jj, kk)];
+    } else if (i == m + 1) {
+        if (m - 1 == 1) { uu = a.u_in; vv = a.v_in; ww = 0.; }
+        else {
+            const int64_t s = pf_off(g, m - 1, jj, kk);
+            uu = u[s]; vv = v[s]; ww = (g.dim == 3) ? w[s] : 0.;
+        }
+        pp = a.outlet_pressure;
+    } else {
+        const int64_t s = pf_off(g, i, jj, kk);
+        uu = u[s]; vv = v[s]; ww = (g.dim == 3) ? w[s] : 0.;
+        pp = P[pf_soff(g, i, jj, kk)];
+    }
+    const int64_t o = pf_off(g, i, j, k);
+    u[o] = uu;
+    v[o] = vv;
+    if (g.dim == 3) w[o] = ww;
+    P[pf_soff(g, i, j, k)] = pp;
+}
+
+// boundary of the air-conditioning solver, one face per launch in the reference's order
+// top, bottom, west, east, north, south (ibm_3d_air_condition_omp_cpu.f90:890-1167);
+// each launch covers the face's full index range, ghost rows included.
+__global__ void k_boundary_air(PfGeom g, StepArgs a, int face, const double *__restrict__ po, double *__restrict__ u,
+                               double *__restrict__ v, double *__restrict__ w, double *__restrict__ P) {
+    const int m = g.m, n = g.n, l = g.l;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y * blockDim.y + threadIdx.y;
+    const double U = a.inlet_velocity, pout = a.outlet_pressure;
+#define OFF(ii, jj, kk) pf_off(g, ii, jj, kk)
+#define SOF(ii, jj, kk) pf_soff(g, ii, jj, kk)
+    if (face == 0 || face == 1) {          // top (k = l) / bottom (k = 1): r = i, q = j
+        const int i = r, j = q;
+        if (i > m + 1 || j > n + 1) return;
+        if (face == 0) {
+            const int code = a.face[0];
+            const bool wall = (code == 0) || (po[OFF(i, j, l)] < 0.9);
+            if (wall) {
+                u[OFF(i, j, l)] = 0.; v[OFF(i, j, l)] = 0.; w[OFF(i, j, l)] = 0.;
+                w[OFF(i, j, l + 1)] = -w[OFF(i, j, l - 1)];
+                P[SOF(i, j, l + 1)] = P[SOF(i, j, l - 1)];
+            } else if (code == 1) {
+                u[OFF(i, j, l)] = 0.; v[OFF(i, j, l)] = 0.; w[OFF(i, j, l)] = -U;
+                u[OFF(i, j, l + 1)] = 0.; v[OFF(i, j, l + 1)] = 0.; w[OFF(i, j, l + 1)] = -U;
+                P[SOF(i, j, l + 1)] = P[SOF(i, j, l - 1)];
+            } else {
+                u[OFF(i, j, l + 1)] = u[OFF(i, j, l - 1)];
+                v[OFF(i, j, l + 1)] = v[OFF(i, j, l - 1)];
+                w[OFF(i, j, l + 1)] = w[OFF(i, j, l - 1)];
+                P[SOF(i, j, l + 1)] = pout;
+            }
+        } else {
+            const int code = a.face[1];
+            // the inlet branch tests porosity(i,j,l) as written at :948
+            const bool wall = (code == 0) || (code == 1 && po[OFF(i, j, l)] < 0.9) ||
+                              (code == 2 && po[OFF(i, j, 1)] < 0.9);
+            if (wall) {
+                u[OFF(i, j, 1)] = 0.; v[OFF(i, j, 1)] = 0.; w[OFF(i, j, 1)] = 0.;
+                w[OFF(i, j, 0)] = -w[OFF(i, j, 2)];
+                P[SOF(i, j, 0)] = P[SOF(i, j, 2)];
+            } else if (code == 1) {
+                u[OFF(i, j, 1)] = 0.; v[OFF(i, j, 1)] = 0.; w[OFF(i, j, 1)] = U;
+                u[OFF(i, j, 0)] = 0.; v[OFF(i, j, 0)] = 0.; w[OFF(i, j, 0)] = U;
+                P[SOF(i, j, 0)] = P[SOF(i, j, 2)];
+            } else {
+                u[OFF(i, j, 0)] = u[OFF(i, j, 1)];
+                v[OFF(i, j, 0)] = v[OFF(i, j, 1)];
+                w[OFF(i, j, 0)] = w[OFF(i, j, 1)];
+                P[SOF(i, j, 0)] = pout;
+            }
+        }
+        return;
+    }
+    if (face == 2 || face == 3) {          // west (i = 1) / east (i = m): r = j, q = k
+        const int j = r, k = q;
+        if (j > n + 1 || k > l + 1) return;
+        if (face == 2) {
+            const int code = a.face[3];
+            const bool wall = (code == 0) || (po[OFF(1, j, k)] < 0.9);
+            if (wall) {
+                u[OFF(1, j, k)] = 0.; v[OFF(1, j, k)] = 0.; w[OFF(1, j, k)] = 0.;
+                u[OFF(0, j, k)] = -u[OFF(2, j, k)];
+                P[SOF(0, j, k)] = P[SOF(2, j, k)];
+            } else if (code == 1) {
+                u[OFF(1, j, k)] = U; v[OFF(1, j, k)] = 0.; w[OFF(1, j, k)] = 0.;
+                u[OFF(0, j, k)] = U; v[OFF(0, j, k)] = 0.; w[OFF(0, j, k)] = 0.;
+                P[SOF(0, j, k)] = P[SOF(2, j, k)];
+            } else {
+                u[OFF(0, j, k)] = u[OFF(1, j, k)];
+                v[OFF(0, j, k)] = v[OFF(1, j, k)];
+                w[OFF(0, j, k)] = w[OFF(1, j, k)];
+                P[SOF(0, j, k)] = pout;
+            }
+        } else {
+            const int code = a.face[2];
+            const bool wall = (code == 0) || (po[OFF(m, j, k)] < 0.9);
+            if (wall) {
+                u[OFF(m, j, k)] = 0.; v[OFF(m, j, k)] = 0.; w[OFF(m, j, k)] = 0.;
+                u[OFF(m + 1, j, k)] = -u[OFF(m - 1, j, k)];
+                P[SOF(m + 1, j, k)] = P[SOF(m - 1, j, k)];
+            } else if (code == 1) {
+                u[OFF(m, j, k)] = -U; v[OFF(m, j, k)] = 0.; w[OFF(m, j, k)] = 0.;
+                u[OFF(m + 1, j, k)] = -U; v[OFF(m + 1, j, k)] = 0.; w[OFF(m + 1, j, k)] = 0.;
+                P[SOF(m + 1, j, k)] = P[SOF(m - 1, j, k)];
+            } else {
+                u[OFF(m + 1, j, k)] = u[OFF(m, j, k)];
+                v[OFF(m + 1, j, k)] = v[OFF(m, j, k)];
+                w[OFF(m + 1, j, k)] = w[OFF(m, j, k)];
+                P[SOF(m + 1, j, k)] = pout;
+            }
+        }
+        return;
+    }
+    {                                      // north (j = n) / south (j = 1): r = i, q = k
+        const int i = r, k = q;
+        if (i > m + 1 || k > l + 1) return;
+        if (face == 4) {
+            const int code = a.face[4];
+            const bool wall = (code == 0) || (po[OFF(i, n, k)] < 0.9);
+            if (wall) {
+                u[OFF(i, n, k)] = 0.; v[OFF(i, n, k)] = 0.; w[OFF(i, n, k)] = 0.;
+                u[OFF(i, n + 1, k)] = -u[OFF(i, n - 1, k)];   // u, as written at :1087
+                P[SOF(i, n + 1, k)] = P[SOF(i, n - 1, k)];
+            } else if (code == 1) {
+                u[OFF(i, n, k)] = -U; v[OFF(i, n, k)] = 0.; w[OFF(i, n, k)] = 0.;
+                u[OFF(i, n + 1, k)] = -U; v[OFF(i, n + 1, k)] = 0.; w[OFF(i, n + 1, k)] = 0.;
+                P[SOF(i, n + 1, k)] = P[SOF(i, n - 1, k)];
+            } else {
+                u[OFF(i, n + 1, k)] = u[OFF(i, n, k)];
+                v[OFF(i, n + 1, k)] = v[OFF(i, n, k)];
+                w[OFF(i, n + 1, k)] = w[OFF(i, n, k)];
+                P[SOF(i, n + 1, k)] = pout;
+            }
+        } else {
+            const int code = a.face[5];
+            const bool wall = (code == 0) || (po[OFF(i, 1, k)] < 0.9);
+            if (wall) {
+                u[OFF(i, 1, k)] = 0.; v[OFF(i, 1, k)] = 0.; w[OFF(i, 1, k)] = 0.;
+                v[OFF(i, 0, k)] = -v[OFF(i, 2, k)];
+                P[SOF(i, 0, k)] = P[SOF(i, 2, k)];
+            } else if (code == 1) {
+                u[OFF(i, 1, k)] = U; v[OFF(i, 1, k)] = 0.; w[OFF(i, 1, k)] = 0.;
+                u[OFF(i, 0, k)] = U; v[OFF(i, 0, k)] = 0.; w[OFF(i, 0, k)] = 0.;
+                P[SOF(i, 0, k)] = P[SOF(i, 2, k)];
+            } else {
+                u[OFF(i, 0, k)] = u[OFF(i, 2, k)];    // (i,2,k), as written at :1153
+                v[OFF(i, 0, k)] = v[OFF(i, 2, k)];
+                w[OFF(i, 0, k)] = w[OFF(i, 2, k)];
+                P[SOF(i, 0, k)] = pout;
+            }
+        }
+    }
+#undef OFF
+#undef SOF
+}
+
+// ------------------------------------------------------------------------------------
+// output_divergent_2d/3d (output.f90:217-224, 929-940) into a dense interior array.
+// ------------------------------------------------------------------------------------
+__global__ void k_divergence_out(PfGeom g, StepArgs a, const double *__restrict__ po, const double *__restrict__ u,
+                                 const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m || j > g.n) return;
+    const int64_t o = pf_off(g, i, j, k);
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    const double pc = po[o];
+    double r = ((((po[o + 1] * u[o]) + (pc * u[o + 1])) / 2.) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) / 2.)) / a.dx;
+    r = r + (((((po[o + sy] * v[o]) + (pc * v[o + sy])) / 2.) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) / 2.)) / a.dy);
+    if (g.dim == 3)
+        r = r + (((((po[o + sz] * w[o]) + (pc * w[o + sz])) / 2.) - (((po[o - sz] * w[o]) + (pc * w[o - sz])) / 2.)) / a.dz);
+    const int64_t d = (int64_t)(i - 1) + (int64_t)g.m * ((int64_t)(j - 1) + (int64_t)g.n * (g.dim == 3 ? k - 1 : 0));
+    out[d] = r;
+}
+
+// Clear the per-step diagnostics with a kernel (graph-capture safe, unlike a memset node).
+__global__ void k_clear_u64(unsigned long long *p, int count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0ull;
+}
+
+// ------------------------------------------------------------------------------------
+// Launch wrappers
+// ------------------------------------------------------------------------------------
+static inline dim3 grid2(int nx, int ny, int nz, dim3 b) {
+    return dim3((unsigned)((nx + (int)b.x - 1) / (int)b.x), (unsigned)((ny + (int)b.y - 1) / (int)b.y), (unsigned)nz);
+}
+
+void launch_faces(hipStream_t st, const PfGeom &g, const double *phi, double *AW, double *AS, double *AB,
+                  double dt, double dx, double dy, double dz) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_faces, grid2(g.m + 1, g.n + 1, g.dim == 3 ? g.l + 1 : 1, b), b, 0, st, g, phi, AW, AS, AB,
+                       dt, dx, dy, dz);
+}
+
+void launch_div(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *uo, const double *vo,
+                const double *wo, double *div) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_div, grid2(g.m + 2, g.n + 2, g.dim == 3 ? g.l + 2 : 1, b), b, 0, st, g, a, uo, vo, wo, div);
+}
+
+void launch_predict(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *uo, const double *vo,
+                    const double *wo, const double *div, const double *phi, double *u, double *v, double *w) {
+    dim3 b(64, 4, 1);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_predict3, grid2(g.m + 2, g.n + 2, g.l + 2, b), b, 0, st, g, a, uo, vo, wo, div, phi, u, v, w);
+    else
+        hipLaunchKernelGGL(k_predict2, grid2(g.m + 2, g.n + 2, 1, b), b, 0, st, g, a, uo, vo, div, phi, u, v);
+}
+
+void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *u, const double *v,
+                const double *w, const double *phi, const double *AW, const double *AS, const double *AB,
+                const double *P, double *BB) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_rhs, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, u, v, w, phi, AW, AS, AB, P, BB);
+}
+
+void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
+                const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
+                unsigned long long *err_slot) {
+    const int c = s.colour;
+    const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
+    dim3 b(64, 4, 1);
+    dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_sor<3>, gr, b, 0, st, g, a, s, P + oth, P + own, AW + own, AW + oth, AS + own, AS + oth,
+                           AB + own, AB + oth, BB + own, phi, err_slot);
+    else
+        hipLaunchKernelGGL(k_sor<2>, gr, b, 0, st, g, a, s, P + oth, P + own, AW + own, AW + oth, AS + own, AS + oth,
+                           AW + own, AW + oth, BB + own, phi, err_slot);
+}
+
+void launch_ghost_refresh(hipStream_t st, const PfGeom &g, int do_y, int do_z, double *P) {
+    dim3 b(64, 4, 1);
+    const int t = (g.dim == 3) ? (g.l > g.n ? g.l : g.n) : 1;
+    hipLaunchKernelGGL(k_ghost_refresh, grid2(g.m, t, 1, b), b, 0, st, g, do_y, do_z, P);
+}
+
+void launch_project(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *P, double *u, double *v,
+                    double *w) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_project, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, P, u, v, w);
+}
+
+void launch_boundary(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, double *u, double *v,
+                     double *w, double *P) {
+    if (a.bc == PF_BC_UNIFORM) {
+        const int64_t X = g.m + 2;
+        const int64_t total = 3 * (int64_t)g.Y * g.Z + 2 * X * g.Z + (g.dim == 3 ? 2 * X * g.Y : 0);
+        const int b = 256;
+        hipLaunchKernelGGL(k_boundary_uniform, dim3((unsigned)((total + b - 1) / b)), dim3(b), 0, st, g, a, u, v, w, P);
+        return;
+    }
+    dim3 b(64, 4, 1);
+    const int order[6] = {0, 1, 2, 3, 4, 5};  // top, bottom, west, east, north, south
+    for (int f : order) {
+        dim3 gr;
+        if (f <= 1) gr = grid2(g.m + 2, g.n + 2, 1, b);
+        else if (f <= 3) gr = grid2(g.n + 2, g.l + 2, 1, b);
+        else gr = grid2(g.m + 2, g.l + 2, 1, b);
+        hipLaunchKernelGGL(k_boundary_air, gr, b, 0, st, g, a, f, phi, u, v, w, P);
+    }
+}
+
+void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,
+                           const double *v, const double *w, double *out) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_divergence_out, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, phi, u, v, w, out);
+}
+
+int sor_blocks(const PfGeom &g) {
+    dim3 b(64, 4, 1);
+    dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
+    return (int)(gr.x * gr.y * gr.z);
+}
+
+void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err) {
+    hipLaunchKernelGGL(k_reduce_err, dim3(1), dim3(1024), 0, st, slots, nslots, step_err);
+}
+
+void launch_clear_u64(hipStream_t st, unsigned long long *p, int count) {
+    hipLaunchKernelGGL(k_clear_u64, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, count);
+}
+
+}  // namespace pf

@@ -1,0 +1,199 @@
+"""Python host over the C-ABI (include/pixelflow.h): one case resident on the GPU.
+
+Mirrors the reference's time loop: ``Solver(params, porosity)`` does what program main
+does before the loop (read_settings / grid_conditions* / initial_conditions / boundary,
+e.g. src/omp_parallel/ibm_3d_uniform_omp_cpu.f90:42-71), ``step(n)`` runs n iterations of
+the loop body (:81-132 without the VTK output), ``field()`` / ``divergence()`` give what
+the output writers read.  The HIP library is required: there is no CPU fallback, and a
+missing library raises at import of the first solver.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .cases import CaseParams
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "_build", "libpixelflow.so")
+
+PF_BC_UNIFORM, PF_BC_AIR = 0, 1
+PF_PREC_FP64, PF_PREC_MIXED = 0, 1
+FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "porosity": 4}
+ERRORS = {-1: "PF_EINVAL", -2: "PF_EHIP", -3: "PF_ENOMEM", -4: "PF_ESTATE", -5: "PF_EDIVERGED"}
+
+
+class PfDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("l", ctypes.c_int32),
+        ("bc", ctypes.c_int32), ("nonslip", ctypes.c_int32), ("iter_max", ctypes.c_int32),
+        ("face", ctypes.c_int32 * 6), ("precision", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
+        ("max_cycles", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+        ("xnue", ctypes.c_double), ("xlambda", ctypes.c_double), ("density", ctypes.c_double),
+        ("thickness", ctypes.c_double), ("inlet_velocity", ctypes.c_double),
+        ("outlet_pressure", ctypes.c_double), ("aoa", ctypes.c_double), ("omega", ctypes.c_double),
+        ("residual_tol", ctypes.c_double),
+    ]
+
+
+class PfStats(ctypes.Structure):
+    _fields_ = [("sor_error", ctypes.c_double), ("residual", ctypes.c_double),
+                ("sor_iters", ctypes.c_int32), ("cycles", ctypes.c_int32)]
+
+
+class PixelFlowError(RuntimeError):
+    pass
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+# Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
+EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
+           "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_footprint_bytes", "pf_sync",
+           "pf_set_timing", "pf_last_sweep_time_us")
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpixelflow.so (build with ``python -m pixelflow_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PixelFlowError(f"HIP library not built: {LIB_PATH} (run `python -m pixelflow_amd.build`)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+    L.pf_abi_version.restype = ctypes.c_int
+    L.pf_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.POINTER(vp)]
+    L.pf_destroy.argtypes = [vp]
+    L.pf_last_error.restype = ctypes.c_char_p
+    L.pf_last_error.argtypes = [vp]
+    L.pf_set_porosity.argtypes = [vp, dp, ctypes.c_double]
+    L.pf_initial.argtypes = [vp]
+    L.pf_upload.argtypes = [vp, ctypes.c_int, dp, i64, i64]
+    L.pf_download.argtypes = [vp, ctypes.c_int, dp, i64, i64]
+    L.pf_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PfStats)]
+    L.pf_divergence.argtypes = [vp, dp]
+    L.pf_footprint_bytes.restype = ctypes.c_int64
+    L.pf_footprint_bytes.argtypes = [ctypes.POINTER(PfDesc)]
+    L.pf_sync.argtypes = [vp]
+    L.pf_set_timing.argtypes = [vp, ctypes.c_int]
+    L.pf_last_sweep_time_us.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0, max_cycles: int = 0,
+              residual_tol: float = 0.0) -> PfDesc:
+    d = PfDesc()
+    d.dim, d.m, d.n, d.l = p.dim, p.m, p.n, (p.l if p.dim == 3 else 1)
+    d.bc = PF_BC_AIR if p.bc == "air" else PF_BC_UNIFORM
+    d.nonslip = 1 if p.nonslip else 0
+    d.iter_max = p.iter_max
+    for i, f in enumerate(p.faces):
+        d.face[i] = int(f)
+    d.precision = precision
+    d.inner_iters = inner_iters
+    d.max_cycles = max_cycles
+    d.dx, d.dy, d.dz, d.dt = p.dx, p.dy, (p.dz if p.dim == 3 else 1.0), p.dt
+    d.xnue, d.xlambda, d.density, d.thickness = p.xnue, p.xlambda, p.density, p.thickness
+    d.inlet_velocity, d.outlet_pressure, d.aoa = p.inlet_velocity, p.outlet_pressure, p.aoa
+    d.omega = p.relux_factor
+    d.residual_tol = residual_tol
+    return d
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    """One case on the GPU (FP64 path: reproduces the reference bit for bit)."""
+
+    def __init__(self, p: CaseParams, phi_raw_interior: np.ndarray, precision: int = PF_PREC_FP64, **kw):
+        self.p = p
+        self.desc = make_desc(p, precision=precision, **kw)
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.pf_create(ctypes.byref(self.desc), ctypes.byref(h))
+        if rc != 0:
+            raise PixelFlowError(f"pf_create: {ERRORS.get(rc, rc)}: {L.pf_last_error(None).decode()}")
+        self._h = h
+        raw = np.ascontiguousarray(phi_raw_interior, dtype=np.float64).reshape(-1)
+        if raw.size != p.cells:
+            raise ValueError(f"porosity has {raw.size} values, the grid {p.cells}")
+        self._check(L.pf_set_porosity(self._h, _dp(raw), float(p.threshold)), "pf_set_porosity")
+        self._check(L.pf_initial(self._h), "pf_initial")
+        self.steps_done = 0
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = lib().pf_last_error(self._h).decode()
+            raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def step(self, nsteps: int = 1) -> Dict[str, np.ndarray]:
+        st = (PfStats * max(nsteps, 1))()
+        self._check(lib().pf_step(self._h, int(nsteps), st), "pf_step")
+        self.steps_done += nsteps
+        return {
+            "sor_error": np.array([s.sor_error for s in st[:nsteps]]),
+            "residual": np.array([s.residual for s in st[:nsteps]]),
+            "sor_iters": np.array([s.sor_iters for s in st[:nsteps]]),
+            "cycles": np.array([s.cycles for s in st[:nsteps]]),
+        }
+
+    def field(self, name: str) -> np.ndarray:
+        """Full field with ghosts, numpy shape (l+2, n+2, m+2) or (n+2, m+2)."""
+        out = np.zeros(self.p.full_shape(), dtype=np.float64)
+        self._check(lib().pf_download(self._h, FIELDS[name], _dp(out), self.p.m + 2, self.p.n + 2), "pf_download")
+        return out
+
+    def set_field(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.p.full_shape())
+        self._check(lib().pf_upload(self._h, FIELDS[name], _dp(a), self.p.m + 2, self.p.n + 2), "pf_upload")
+
+    def interior(self, name: str) -> np.ndarray:
+        a = self.field(name)
+        return a[1:-1, 1:-1, 1:-1].copy() if self.p.dim == 3 else a[1:-1, 1:-1].copy()
+
+    def divergence(self) -> np.ndarray:
+        out = np.zeros(self.p.interior_shape(), dtype=np.float64)
+        self._check(lib().pf_divergence(self._h, _dp(out)), "pf_divergence")
+        return out
+
+    def sync(self) -> None:
+        self._check(lib().pf_sync(self._h), "pf_sync")
+
+    def set_timing(self, on: bool) -> None:
+        self._check(lib().pf_set_timing(self._h, 1 if on else 0), "pf_set_timing")
+
+    def last_sweep_time_us(self):
+        us = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(lib().pf_last_sweep_time_us(self._h, ctypes.byref(us), ctypes.byref(n)), "pf_last_sweep_time_us")
+        return us.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().pf_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint_bytes(p: CaseParams, precision: int = PF_PREC_FP64) -> int:
+    d = make_desc(p, precision=precision)
+    return int(lib().pf_footprint_bytes(ctypes.byref(d)))
