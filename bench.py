@@ -75,7 +75,15 @@ def main(argv=None):
     ap.add_argument("--m", type=int, default=256)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--l", type=int, default=128)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain device-copy calibration kernel and print its GB/s")
     args = ap.parse_args(argv)
+
+    if args.copy_probe:
+        from pixelflow_amd.solver import probe_copy_gbps
+        out = {"copy_probe_GBps": {f"{mb}MB": probe_copy_gbps(mb << 20, 20) for mb in (64, 128, 1024, 4096)}}
+        print(json.dumps(out))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -99,6 +107,8 @@ def main(argv=None):
     sweep_us, launches = s.last_sweep_time_us()
     s.close()
     log(f"[bench] {args.steps} steps in {wall:.3f} s; SOR sweep avg {sweep_us:.2f} us over {launches} launches")
+    from pixelflow_amd.solver import probe_copy_gbps
+    copy_gbps = {"128MB": probe_copy_gbps(128 << 20, 20), "2GB": probe_copy_gbps(2048 << 20, 10)}
 
     cells = p.cells
     value = cells * args.steps / wall
@@ -123,14 +133,15 @@ def main(argv=None):
         "sor_cell_updates_per_s": value * p.iter_max,
         "step_roofline_frac": (step_bytes * args.steps / wall / 1e9) / HBM_PEAK_GBS,
         "final_sor_error": float(st["sor_error"][-1]),
+        "device_copy_GBps": copy_gbps,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_sor (red/black colour sweep)",
+            "kernel": "k_sor_phi (red/black colour sweep)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_sor", workload),
+            "traffic": measured_traffic("k_sor_phi", workload),
             "bytes_per_launch_algorithmic": sweep_bytes,
             "avg_launch_us": sweep_us,
         },

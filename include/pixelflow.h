@@ -112,6 +112,13 @@ int pf_sync(pf_ctx *ctx);
  * gives the average time per colour-sweep launch of the last pf_step (gaps between the
  * back-to-back launches included) and the number of launches it covers. */
 int pf_set_timing(pf_ctx *ctx, int on);
+/* Which colour-sweep kernel the context uses (0 stored face coefficients, 1 faces rebuilt
+ * from porosity with IEEE division, 2 the same with the constant-divisor division that
+ * pf_set_porosity verified face by face).  All three give identical results. */
+int pf_sor_kernel(const pf_ctx *ctx);
+/* Bandwidth calibration (no reference counterpart): a plain device-to-device copy of
+ * `bytes`, `iters` times; *gbps = (read + written bytes) / second. */
+int pf_probe_copy(int64_t bytes, int iters, double *gbps);
 int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
 
 #ifdef __cplusplus

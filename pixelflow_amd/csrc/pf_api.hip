@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -25,7 +26,13 @@ struct pf_ctx {
     double *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     double *phi = nullptr, *div = nullptr, *dout = nullptr;
     // colour-split layout (2 * HS doubles each)
-    double *P = nullptr, *BB = nullptr, *AW = nullptr, *AS = nullptr, *AB = nullptr;
+    double *P = nullptr, *BB = nullptr, *AW = nullptr, *AS = nullptr, *AB = nullptr, *PHIs = nullptr;
+    // SOR sweep kernel: 0 stored face coefficients, 1 faces rebuilt from porosity with IEEE
+    // division, 2 the same with the checked constant-divisor division (chosen when
+    // k_faces_check finds every face identical to IEEE division).
+    int sor_variant = 2;
+    int sor_forced = -1;
+    int xcd_remap = 1;
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
     unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
     int err_cap = 0, nparts = 0;
@@ -79,7 +86,7 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     if (!desc_ok(d, why)) return -1;
     PfGeom g = pf_make_geom(d->dim, d->m, d->n, d->l);
     const int nat = (d->dim == 3) ? 8 : 6;
-    const int spl = (d->dim == 3) ? 5 : 4;
+    const int spl = (d->dim == 3) ? 6 : 5;
     const int64_t interior = (int64_t)d->m * d->n * (d->dim == 3 ? d->l : 1);
     return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + 4096;
 }
@@ -97,7 +104,7 @@ static int alloc(pf_ctx *c, double **p, int64_t n) {
 
 static void release(pf_ctx *c) {
     double **all[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->phi, &c->div, &c->dout,
-                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB};
+                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs};
     for (double **p : all)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (c->err) { (void)hipFree(c->err); c->err = nullptr; }
@@ -125,6 +132,7 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     a.nonslip = desc->nonslip ? 1 : 0;
     for (int f = 0; f < 6; ++f) a.face[f] = desc->face[f];
     a.dt = desc->dt; a.dx = desc->dx; a.dy = desc->dy; a.dz = (desc->dim == 3) ? desc->dz : 1.0;
+    a.rdx = 1.0 / a.dx; a.rdy = 1.0 / a.dy; a.rdz = 1.0 / a.dz;
     a.xnue = desc->xnue; a.xlambda = desc->xlambda; a.density = desc->density; a.thickness = desc->thickness;
     a.omega = desc->omega;
     a.omega_c = 1. - desc->omega;
@@ -154,7 +162,7 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     }
     if ((rc = alloc(c, &c->dout, interior)) != PF_OK) goto bad;
     {
-        double **spl[] = {&c->P, &c->BB, &c->AW, &c->AS, &c->AB};
+        double **spl[] = {&c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs};
         for (double **p : spl) {
             if (p == &c->AB && g.dim == 2) continue;
             if ((rc = alloc(c, p, 2 * g.HS)) != PF_OK) goto bad;
@@ -169,6 +177,12 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         rc = fail(c, PF_EHIP, "stream synchronize failed after allocation");
         goto bad;
+    }
+    {
+        const char *v = std::getenv("PF_SOR_VARIANT");
+        if (v && *v) c->sor_forced = std::atoi(v);
+        const char *x = std::getenv("PF_XCD_REMAP");
+        if (x && *x) c->xcd_remap = std::atoi(x) != 0;
     }
     *out = c;
     return PF_OK;
@@ -200,6 +214,28 @@ extern "C" int pf_sync(pf_ctx *c) {
 // ------------------------------------------------------------------------------------
 // Porosity: grid_conditions* (src/omp_parallel/lib/grid.f90) on the host, once.
 // ------------------------------------------------------------------------------------
+// Time-invariant data derived from porosity: stored faces, the colour-split porosity and
+// the choice of sweep kernel.
+static int derive_from_porosity(pf_ctx *c) {
+    const PfGeom &g = c->g;
+    pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
+    pf::launch_split(c->st, g, c->phi, c->PHIs);
+    int *bad = nullptr;
+    PF_TRY_HIP(c, hipMalloc((void **)&bad, sizeof(int)));
+    int zero = 0, hbad = 1;
+    PF_TRY_HIP(c, hipMemcpyAsync(bad, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
+    pf::launch_faces_check(c->st, g, c->phi, c->AW, c->AS, c->AB, c->a, bad);
+    hipError_t e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(bad);
+    PF_TRY_HIP(c, e);
+    PF_TRY_HIP(c, hipGetLastError());
+    c->sor_variant = (hbad == 0) ? 2 : 1;
+    if (c->sor_forced >= 0 && c->sor_forced <= 2 && !(c->sor_forced == 2 && hbad != 0)) c->sor_variant = c->sor_forced;
+    c->have_phi = true;
+    return PF_OK;
+}
+
 extern "C" int pf_set_porosity(pf_ctx *c, const double *raw, double threshold) {
     if (!c || !raw) return PF_EINVAL;
     const PfGeom &g = c->g;
@@ -232,11 +268,7 @@ extern "C" int pf_set_porosity(pf_ctx *c, const double *raw, double threshold) {
             for (int j = 0; j <= n + 1; ++j) { at(i, j, 0) = at(i, j, l); at(i, j, l + 1) = at(i, j, 1); }
     }
     PF_TRY_HIP(c, hipMemcpyAsync(c->phi, phi.data(), phi.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
-    pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
-    PF_TRY_HIP(c, hipGetLastError());
-    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
-    c->have_phi = true;
-    return PF_OK;
+    return derive_from_porosity(c);
 }
 
 // ------------------------------------------------------------------------------------
@@ -272,11 +304,7 @@ extern "C" int pf_upload(pf_ctx *c, int field, const double *host, int64_t ldi, 
                 stage[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
             }
     PF_TRY_HIP(c, hipMemcpy(dst, stage.data(), (size_t)size * sizeof(double), hipMemcpyHostToDevice));
-    if (field == PF_FIELD_POROSITY) {
-        pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
-        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
-        c->have_phi = true;
-    }
+    if (field == PF_FIELD_POROSITY) return derive_from_porosity(c);
     return PF_OK;
 }
 
@@ -362,7 +390,12 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot) {
             s.fuse_z = fz;
             // 3D reduces |dp| after the second sweep only (:575-583); 2D after both (:351, :385)
             s.track_err = (g.dim == 2) || (half == 1);
-            pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
+            s.xcd_remap = c->xcd_remap;
+            s.gx = s.gy = 1;
+            if (c->sor_variant == 0)
+                pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
+            else
+                pf::launch_sor_phi(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, slot, c->sor_variant == 2);
             if (ry || rz) pf::launch_ghost_refresh(c->st, g, ry, rz, c->P);
         }
     }
@@ -453,4 +486,38 @@ extern "C" int pf_last_sweep_time_us(const pf_ctx *c, double *avg_us, int64_t *l
     *avg_us = c->sweep_us_avg;
     *launches = c->sweep_launches;
     return PF_OK;
+}
+
+extern "C" int pf_sor_kernel(const pf_ctx *c) { return c ? c->sor_variant : PF_EINVAL; }
+
+// Bandwidth calibration: copies `bytes` (rounded down to 16) device to device `iters` times
+// on a private stream and returns the average GB/s (read + write bytes per second).
+extern "C" int pf_probe_copy(int64_t bytes, int iters, double *gbps) {
+    if (bytes < 16 || iters < 1 || !gbps) return PF_EINVAL;
+    const int64_t n = (bytes / 16) * 2;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PF_OK;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return PF_EHIP;
+    if (hipMalloc((void **)&a, (size_t)n * 8) != hipSuccess || hipMalloc((void **)&b, (size_t)n * 8) != hipSuccess) {
+        rc = PF_ENOMEM;
+    } else if (hipMemsetAsync(a, 0, (size_t)n * 8, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        rc = PF_EHIP;
+    } else {
+        pf::launch_copy_probe(st, a, b, n);  // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int t = 0; t < iters; ++t) pf::launch_copy_probe(st, a, b, n);
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PF_EHIP;
+        else *gbps = (double)(2 * n * 8) * iters / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return rc;
 }

@@ -18,6 +18,7 @@ struct StepArgs {
     int nonslip;
     int face[6];        // air face codes: top, bottom, east, west, north, south
     double dt, dx, dy, dz;
+    double rdx, rdy, rdz;         // RN(1/dx) etc. for the checked constant-divisor path
     double xnue, xlambda, density, thickness;
     double omega, omega_c;        // relux_factor and (1 - relux_factor)
     double u_in, v_in;            // inlet velocity components (host libm cos/sin)
@@ -28,6 +29,8 @@ struct SweepArgs {
     int colour;         // cells with (i+j+k) & 1 == colour are updated
     int fuse_y, fuse_z; // write the periodic ghost copies in the sweep (even extents)
     int track_err;      // reduce max |dp| into the step's error slot
+    int xcd_remap;      // renumber blocks XCD-contiguously (k_sor_phi)
+    int gx, gy;         // block grid extents of the sweep (set by the launcher)
 };
 
 void launch_faces(hipStream_t st, const PfGeom &g, const double *phi, double *AW, double *AS, double *AB,
@@ -42,6 +45,12 @@ void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double
 void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                 const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
                 unsigned long long *err_slot);
+void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
+                    const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
+                    bool fast);
+void launch_faces_check(hipStream_t st, const PfGeom &g, const double *phi, const double *AW, const double *AS,
+                        const double *AB, const StepArgs &a, int *bad);
+void launch_split(hipStream_t st, const PfGeom &g, const double *src, double *dst);
 void launch_ghost_refresh(hipStream_t st, const PfGeom &g, int do_y, int do_z, double *P);
 void launch_project(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *P, double *u, double *v,
                     double *w);
@@ -50,7 +59,8 @@ void launch_boundary(hipStream_t st, const PfGeom &g, const StepArgs &a, const d
 void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
-int sor_blocks(const PfGeom &g);   // blocks of one colour-sweep launch (= error slots)
+int sor_blocks(const PfGeom &g);
+void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n);   // blocks of one colour-sweep launch (= error slots)
 void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err);
 
 }  // namespace pf

@@ -439,6 +439,158 @@ __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned lon
     }
 }
 
+// XCD-aware block order.  The hardware hands workgroup L of a 1D grid to XCD L % 8; each
+// XCD has its own L2.  Renumbering so that XCD x receives the contiguous range
+// [x*B/8, (x+1)*B/8) of logical blocks keeps the rows and planes that neighbouring blocks
+// share (the j-1/j+1 rows and k-1/k+1 planes of a stencil) in one L2.
+__device__ __forceinline__ int xcd_logical_block(int L, int B) {
+    const int q = B >> 3, rem = B & 7;
+    const int x = L & 7, r = L >> 3;
+    return (x < rem) ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+
+// Variant of the sweep that rebuilds the six face coefficients of each updated cell from
+// porosity instead of reading them: AW(i) = dt*max(small,(phi(i)+phi(i-1))*0.5)/dx/dx is
+// evaluated with the same operands and order as solve_p (3D :390-400, 2D :265-268), so the
+// values are the stored ones bit for bit, while a half-sweep streams 3 words per cell
+// (p of both colours, porosity of both colours, bb of the colour, p written) instead of 5.
+// PHIo / PHIx: porosity in the colour-split layout (own / other colour).
+__device__ __forceinline__ double face_coef(double pa, double pb, double dt, double d) {
+    return ((dt * fmax_ref(PF_SMALL, (pa + pb) * 0.5)) / d) / d;
+}
+
+// a / d for a constant divisor d with rd = RN(1/d) from the host: q0 = a*rd is within one
+// ulp of a/d, the residual a - q0*d is exact in an FMA, and one FMA correction gives the
+// correctly rounded quotient (Markstein's theorem; 3 instructions instead of the ~10 of a
+// general FP64 division).  k_faces_check verifies every face of a case against true
+// division before this path is used.
+__device__ __forceinline__ double div_const(double a, double d, double rd) {
+    const double q0 = a * rd;
+    const double e = __builtin_fma(-q0, d, a);
+    return __builtin_fma(e, rd, q0);
+}
+__device__ __forceinline__ double face_coef_fast(double pa, double pb, double dt, double d, double rd) {
+    return div_const(div_const(dt * fmax_ref(PF_SMALL, (pa + pb) * 0.5), d, rd), d, rd);
+}
+
+template <bool FAST>
+__device__ __forceinline__ double face_any(double pa, double pb, double dt, double d, double rd) {
+    return FAST ? face_coef_fast(pa, pb, dt, d, rd) : face_coef(pa, pb, dt, d);
+}
+
+template <int DIM, bool FAST>
+__global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
+                                                 double *__restrict__ Pown, const double *__restrict__ PHIo,
+                                                 const double *__restrict__ PHIx, const double *__restrict__ BBo,
+                                                 const double *__restrict__ phi, unsigned long long *err_slot) {
+    // 1D grid of gx*gy*gz blocks (s.gx, s.gy), renumbered XCD-contiguously when s.xcd_remap
+    const int nb = (int)gridDim.x;
+    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
+    const int bx = L % s.gx, byz = L / s.gx;
+    const int by = byz % s.gy, bz = byz / s.gy;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int k = (DIM == 3) ? bz + 1 : 0;
+    const int sh = (s.colour + j + k) & 1;
+    const int h = (int)(bx * blockDim.x + threadIdx.x) + 1 - sh;
+    const int i = 2 * h + sh;
+    const bool active = (j <= g.n) && (i <= g.m);
+    double err = 0.;
+    if (active) {
+        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+        const int64_t o = (int64_t)h + HP * ((int64_t)j + (int64_t)g.Y * k);
+        const double pc = Pown[o];
+        const double pW = Poth[o - 1 + sh], pE = Poth[o + sh];
+        const double pS = Poth[o - HP], pN = Poth[o + HP];
+        const double fc = PHIo[o];
+        const double fW = PHIx[o - 1 + sh], fE = PHIx[o + sh];
+        const double fS = PHIx[o - HP], fN = PHIx[o + HP];
+        const double dt = a.dt;
+        Coef c;
+        c.ae = face_any<FAST>(fE, fc, dt, a.dx, a.rdx);
+        c.aw = face_any<FAST>(fc, fW, dt, a.dx, a.rdx);
+        c.an = face_any<FAST>(fN, fc, dt, a.dy, a.rdy);
+        c.as = face_any<FAST>(fc, fS, dt, a.dy, a.rdy);
+        double pB = 0., pT = 0.;
+        if (DIM == 3) {
+            pB = Poth[o - HPY];
+            pT = Poth[o + HPY];
+            const double fB = PHIx[o - HPY], fT = PHIx[o + HPY];
+            c.at = face_any<FAST>(fT, fc, dt, a.dz, a.rdz);
+            c.ab = face_any<FAST>(fc, fB, dt, a.dz, a.rdz);
+        } else {
+            c.at = 0.;
+            c.ab = 0.;
+        }
+        double ap;
+        if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
+        else ap = (((-c.ae) - c.aw) - c.an) - c.as;
+        if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+        const double bb = BBo[o];
+        double num;
+        if (DIM == 3)
+            num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
+        else
+            num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
+        const double pn = ((num / ap) * a.omega) + (pc * a.omega_c);
+        Pown[o] = pn;
+        if (s.fuse_y) {
+            if (j == g.n) Pown[o - HP * (int64_t)g.n] = pn;
+            if (j == 1) Pown[o + HP * (int64_t)g.n] = pn;
+        }
+        if (DIM == 3 && s.fuse_z) {
+            if (k == g.l) Pown[o - HPY * (int64_t)g.l] = pn;
+            if (k == 1) Pown[o + HPY * (int64_t)g.l] = pn;
+        }
+        err = fabs(pn - pc);
+    }
+    if (!s.track_err) return;
+    unsigned long long e = __double_as_longlong(err);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) red[lane >> 6] = e;
+    __syncthreads();
+    if (lane == 0) {
+        unsigned long long b = red[0];
+        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+        for (int q = 1; q < nw; ++q) b = (red[q] > b) ? red[q] : b;
+        unsigned long long *slot = err_slot + blockIdx.x;
+        if (b > *slot) *slot = b;
+    }
+}
+
+// Porosity (natural layout, ghosts included) -> colour-split copy.
+__global__ void k_split(PfGeom g, const double *__restrict__ src, double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    dst[pf_soff(g, i, j, k)] = src[pf_off(g, i, j, k)];
+}
+
+// Compares the fast face formula with IEEE division for every face k_faces stored; any
+// difference sets *bad (never expected; the solver then keeps true division).
+__global__ void k_faces_check(PfGeom g, const double *__restrict__ phi, const double *__restrict__ AW,
+                              const double *__restrict__ AS, const double *__restrict__ AB, StepArgs a,
+                              int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    const int64_t o = pf_off(g, i, j, k);
+    const int64_t so = pf_soff(g, i, j, k);
+    const double pc = phi[o];
+    bool ok = __double_as_longlong(face_coef_fast(pc, phi[o - 1], a.dt, a.dx, a.rdx)) == __double_as_longlong(AW[so]);
+    ok = ok && __double_as_longlong(face_coef_fast(pc, phi[o - g.PX], a.dt, a.dy, a.rdy)) == __double_as_longlong(AS[so]);
+    if (g.dim == 3)
+        ok = ok && __double_as_longlong(face_coef_fast(pc, phi[o - (int64_t)g.PX * g.Y], a.dt, a.dz, a.rdz)) ==
+                       __double_as_longlong(AB[so]);
+    if (!ok) atomicOr(bad, 1);
+}
+
 // Periodic ghost refresh of p for odd extents (3D :463-480 / :520-537 / :588-605,
 // 2D :322-327 / :356-361 / :392-397): y ghosts for i in 1..m, k in 1..l; z ghosts for
 // i in 1..m, j in 1..n.
@@ -752,6 +904,39 @@ void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepA
                            AW + own, AW + oth, BB + own, phi, err_slot);
 }
 
+void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
+                    const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
+                    bool fast) {
+    const int c = s.colour;
+    const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
+    dim3 b(64, 4, 1);
+    dim3 g3 = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
+    SweepArgs s1 = s;
+    s1.gx = (int)g3.x;
+    s1.gy = (int)g3.y;
+    dim3 gr((unsigned)(g3.x * g3.y * g3.z));
+#define PF_SOR_PHI(D, F) hipLaunchKernelGGL((k_sor_phi<D, F>), gr, b, 0, st, g, a, s1, P + oth, P + own, PHIs + own, \
+                                            PHIs + oth, BB + own, phi, err_slot)
+    if (g.dim == 3) {
+        if (fast) PF_SOR_PHI(3, true); else PF_SOR_PHI(3, false);
+    } else {
+        if (fast) PF_SOR_PHI(2, true); else PF_SOR_PHI(2, false);
+    }
+#undef PF_SOR_PHI
+}
+
+void launch_faces_check(hipStream_t st, const PfGeom &g, const double *phi, const double *AW, const double *AS,
+                        const double *AB, const StepArgs &a, int *bad) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_faces_check, grid2(g.m + 1, g.n + 1, g.dim == 3 ? g.l + 1 : 1, b), b, 0, st, g, phi, AW, AS,
+                       AB, a, bad);
+}
+
+void launch_split(hipStream_t st, const PfGeom &g, const double *src, double *dst) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_split, grid2(g.m + 2, g.n + 2, g.Z, b), b, 0, st, g, src, dst);
+}
+
 void launch_ghost_refresh(hipStream_t st, const PfGeom &g, int do_y, int do_z, double *P) {
     dim3 b(64, 4, 1);
     const int t = (g.dim == 3) ? (g.l > g.n ? g.l : g.n) : 1;
@@ -802,6 +987,17 @@ void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, un
 
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count) {
     hipLaunchKernelGGL(k_clear_u64, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, count);
+}
+
+// Plain device copy (bandwidth calibration: a kernel whose HBM bytes are known exactly).
+__global__ void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n2) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (int64_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n) {
+    const int64_t n2 = n / 2;
+    hipLaunchKernelGGL(k_copy_probe, dim3(256 * 8 * 4), dim3(256), 0, st, (const double2 *)src, (double2 *)dst, n2);
 }
 
 }  // namespace pf

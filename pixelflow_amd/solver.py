@@ -54,7 +54,7 @@ _lib: Optional[ctypes.CDLL] = None
 # Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
 EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
            "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_footprint_bytes", "pf_sync",
-           "pf_set_timing", "pf_last_sweep_time_us")
+           "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_probe_copy")
 
 
 def lib() -> ctypes.CDLL:
@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
     L.pf_sync.argtypes = [vp]
     L.pf_set_timing.argtypes = [vp, ctypes.c_int]
     L.pf_last_sweep_time_us.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.pf_sor_kernel.argtypes = [vp]
+    L.pf_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -176,6 +178,9 @@ class Solver:
         self._check(lib().pf_last_sweep_time_us(self._h, ctypes.byref(us), ctypes.byref(n)), "pf_last_sweep_time_us")
         return us.value, n.value
 
+    def sor_kernel(self) -> int:
+        return int(lib().pf_sor_kernel(self._h))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().pf_destroy(self._h)
@@ -197,3 +202,12 @@ class Solver:
 def footprint_bytes(p: CaseParams, precision: int = PF_PREC_FP64) -> int:
     d = make_desc(p, precision=precision)
     return int(lib().pf_footprint_bytes(ctypes.byref(d)))
+
+
+def probe_copy_gbps(nbytes: int, iters: int = 20) -> float:
+    """Device-to-device copy bandwidth (read + write bytes per second, GB/s)."""
+    out = ctypes.c_double()
+    rc = lib().pf_probe_copy(int(nbytes), int(iters), ctypes.byref(out))
+    if rc != 0:
+        raise PixelFlowError(f"pf_probe_copy: {ERRORS.get(rc, rc)}")
+    return out.value

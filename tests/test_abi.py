@@ -35,9 +35,9 @@ def test_library_exports_every_declared_symbol():
 def test_footprint_and_descriptor_validation():
     p, _ = cases.sphere_case(steps=10)
     fb = solver.footprint_bytes(p)
-    # 8 natural FP64 fields + 5 colour-split ones, each >= one value per cell
-    assert fb > 13 * 8 * p.cells
-    assert fb < 16 * 8 * p.cells
+    # 8 natural FP64 fields + 6 colour-split ones, each >= one value per cell
+    assert fb > 14 * 8 * p.cells
+    assert fb < 18 * 8 * p.cells
     bad = cases.CaseParams(dim=3, m=2, n=2, l=2)
     assert solver.footprint_bytes(bad) == -1
 

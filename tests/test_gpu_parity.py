@@ -111,3 +111,24 @@ def test_gpu_step_chunks_are_consistent():
         np.testing.assert_array_equal(ea, eb)
         for f in ("u", "v", "w", "p"):
             np.testing.assert_array_equal(a.field(f), b.field(f))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("name", ["g2d_weird_odd", "g3d_sph32", "g3d_odd_33_17_15", "g3d_air_room24", "g3d_weird"])
+def test_gpu_sweep_kernels_agree(name, variant, monkeypatch):
+    """The three colour-sweep kernels (stored faces; faces from porosity with IEEE division;
+    with the checked constant-divisor division) reproduce the reference identically."""
+    monkeypatch.setenv("PF_SOR_VARIANT", str(variant))
+    p, g = load_golden(name)
+    with _solver(p, g["phi_raw"]) as s:
+        assert s.sor_kernel() == variant
+        st = s.step(p.istep_max)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), g[f], err_msg=f"{name}:{f}:variant{variant}")
+        np.testing.assert_array_equal(st["sor_error"], g["errs"])
+
+
+def test_gpu_default_sweep_is_checked_fast_path():
+    p, phi = cases.sphere_case(m=64, n=32, l=32, steps=1)
+    with _solver(p, phi) as s:
+        assert s.sor_kernel() == 2
