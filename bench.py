@@ -65,6 +65,44 @@ def cpu_baseline(p, phi, steps: int, threads: int):
     return p.cells * steps / dt, dt
 
 
+def mixed_comparison(p, phi, steps: int, warmup: int, inner: int, fp64_value: float):
+    """Same workload in mixed precision (fixed-count mode: the reference's iter_max), timed,
+    then the fields after `steps` steps compared with an FP64 run of the same steps."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    log(f"[bench] mixed precision: {steps} steps, inner={inner or p.iter_max}")
+    b = Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=inner)
+    if warmup:
+        b.step(warmup)
+    b.set_timing(True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    sweep_us, _ = b.last_sweep_time_us()
+    a = Solver(p, phi)
+    a.step(warmup + steps)
+    errs = {}
+    for f in ("u", "v", "w", "p"):
+        x, y = a.interior(f), b.interior(f)
+        errs[f] = float(np.linalg.norm(y - x) / max(np.linalg.norm(x), 1e-300))
+    ddiv = float(np.max(np.abs(a.divergence() - b.divergence())))
+    a.close()
+    b.close()
+    v = p.cells * steps / wall
+    # mixed bytes per cell-iteration: 2 FP32 half-sweeps of 3 words, per cycle an FP64
+    # residual (p 8, bb 8, porosity 8 read, r 4 + e 4 written) and an update (e 4, p 8 + 8)
+    K = inner or p.iter_max
+    cyc = (p.iter_max + K - 1) // K
+    sor_bytes_cell = p.iter_max * 2 * 3 * 4 + cyc * (8 * 3 + 4 + 4 + 4 + 16)
+    return {"value": v, "unit": "cell-steps/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
+            "speedup_vs_fp64": v / fp64_value, "sor_cell_updates_per_s": v * p.iter_max,
+            "inner_iters": K, "cycles_per_step": cyc, "avg_solve_us_per_iteration_half": sweep_us,
+            "solve_roofline_frac": (sor_bytes_cell * p.cells / (sweep_us * 2 * p.iter_max * 1e-6) / 1e9) / HBM_PEAK_GBS,
+            "field_rel_l2_vs_fp64": errs, "max_abs_divergence_diff_vs_fp64": ddiv,
+            "mode": "fixed-count (iter_max FP32 SOR iterations in FP64 defect-correction cycles)"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +113,9 @@ def main(argv=None):
     ap.add_argument("--m", type=int, default=256)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--l", type=int, default=128)
+    ap.add_argument("--mixed-steps", type=int, default=100,
+                    help="steps of the mixed-precision comparison on the same workload (0 = skip)")
+    ap.add_argument("--mixed-inner", type=int, default=0, help="FP32 iterations per defect-correction cycle")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain device-copy calibration kernel and print its GB/s")
     args = ap.parse_args(argv)
@@ -146,6 +187,8 @@ def main(argv=None):
             "avg_launch_us": sweep_us,
         },
     }
+    if args.mixed_steps > 0:
+        res["mixed"] = mixed_comparison(p, phi, args.mixed_steps, args.warmup, args.mixed_inner, value)
     if args.cpu_steps > 0:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")

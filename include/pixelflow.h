@@ -50,19 +50,24 @@ typedef struct pf_desc {
     int32_t face[6];          /* air: top,bottom,east,west,north,south; 0 wall,1 inlet,2 outlet */
     int32_t precision;        /* pf_precision */
     int32_t inner_iters;      /* mixed: FP32 SOR iterations per defect-correction cycle */
-    int32_t max_cycles;       /* mixed: cap on cycles per step (0 = iter_max/inner_iters) */
+    int32_t max_cycles;       /* mixed, tolerance mode: cap on cycles per step */
+    int32_t report_residual;  /* mixed, fixed-count mode: also evaluate the final FP64 residual */
     double dx, dy, dz, dt;
     double xnue, xlambda, density, thickness;
     double inlet_velocity, outlet_pressure, aoa;
     double omega;             /* relux_factor */
-    double residual_tol;      /* mixed: stop when the FP64 residual norm <= tol (0 = fixed count) */
+    double residual_tol;      /* mixed: 0 = run iter_max iterations in cycles of inner_iters (the
+                                 reference's fixed count); > 0 = cycles of inner_iters until the
+                                 FP64 residual max|r/ap| <= residual_tol (or max_cycles) */
 } pf_desc;
 
 /* Per-step report. sor_error = the reference's "SOR iteration no. .. -- p error:" value
  * (running max of |p - p_old|, ibm_3d_uniform_omp_cpu.f90:575-583). */
 typedef struct pf_stats {
     double sor_error;
-    double residual;          /* mixed: final FP64 residual norm of the step (FP64 path: 0) */
+    double residual;          /* mixed: FP64 residual max|r/ap| (tolerance mode: the last one
+                                 evaluated; fixed-count mode: after the step if report_residual,
+                                 else -1); FP64 path: -1 */
     int32_t sor_iters;        /* SOR iterations executed in the step */
     int32_t cycles;           /* mixed: defect-correction cycles (FP64 path: 0) */
 } pf_stats;

@@ -372,7 +372,8 @@ def sphere_case(m: int = 256, n: int = 128, l: int = 128, steps: int = 500, iter
 
 
 def boxes_case(m: int = 512, n: int = 256, l: int = 256, steps: int = 200, seed: int = 7,
-               iter_max: int = 100, binary: bool = False) -> Tuple[CaseParams, np.ndarray]:
+               iter_max: int = 100, binary: bool = False, nboxes: int = 40,
+               side_range: Tuple[int, int] = (4, 32)) -> Tuple[CaseParams, np.ndarray]:
     """3D mixed-precision case: 512x256x256, 40 seeded boxes with sides 4-32 cells, U = 1,
     xnue = 5e-4, dt = 5e-5.  ``binary=True`` hands the 0/1 voxels to the solver as they
     are (clamped to the threshold, the step diverges within a few steps in the reference
@@ -380,7 +381,7 @@ def boxes_case(m: int = 512, n: int = 256, l: int = 256, steps: int = 200, seed:
     voxels with the reference's tanh interface (:func:`voxel_porosity`)."""
     p = CaseParams(dim=3, m=m, n=n, l=l, width=2.0, height=1.0, depth=1.0, xnue=0.0005, time=5e-5 * steps,
                    istep_max=steps, inlet_velocity=1.0, iter_max=iter_max, relux_factor=1.7)
-    vox = random_boxes_3d(m, n, l, 40, (4, 32), seed)
+    vox = random_boxes_3d(m, n, l, nboxes, side_range, seed)
     return p, (vox if binary else voxel_porosity(vox, 1.5))
 
 

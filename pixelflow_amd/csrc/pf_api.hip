@@ -35,12 +35,18 @@ struct pf_ctx {
     int xcd_remap = 1;
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
     unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
+    unsigned long long *res_part = nullptr;  // mixed: per-block max |r/ap|
+    unsigned long long *res_val = nullptr;   // mixed: reduced residual of the last evaluation
+    float *E32 = nullptr, *R32 = nullptr, *PHI32 = nullptr;   // mixed: colour-split FP32 arrays
     int err_cap = 0, nparts = 0;
     bool have_phi = false, have_init = false;
     bool timing = false;
     std::vector<hipEvent_t> evs;   // timing mode: one (start, end) pair per step around the SOR
     double sweep_us_avg = 0.0;
     int64_t sweep_launches = 0;
+    std::vector<double> mixed_res;
+    std::vector<int> mixed_cycles, mixed_iters;
+    double solve_us_avg = 0.0;      // timing mode: average pressure-solve time per step
     std::string msg;
 };
 
@@ -69,7 +75,11 @@ static bool desc_ok(const pf_desc *d, std::string &why) {
     for (int f = 0; f < 6; ++f)
         if (d->face[f] < 0 || d->face[f] > 2) { why = "face codes must be 0, 1 or 2"; return false; }
     if (d->iter_max < 0) { why = "iter_max < 0"; return false; }
-    if (d->precision != PF_PREC_FP64) { why = "only PF_PREC_FP64 in this build of the single-case context"; return false; }
+    if (d->precision != PF_PREC_FP64 && d->precision != PF_PREC_MIXED) { why = "precision must be FP64 or MIXED"; return false; }
+    if (d->precision == PF_PREC_MIXED && (d->inner_iters < 0 || d->max_cycles < 0 || !(d->residual_tol >= 0))) {
+        why = "mixed precision: inner_iters, max_cycles >= 0 and residual_tol >= 0";
+        return false;
+    }
     if (!(d->dx > 0) || !(d->dy > 0) || (d->dim == 3 && !(d->dz > 0)) || !(d->dt > 0) || !(d->density != 0)) {
         why = "dx, dy, dz, dt must be > 0 and density != 0";
         return false;
@@ -88,7 +98,8 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     const int nat = (d->dim == 3) ? 8 : 6;
     const int spl = (d->dim == 3) ? 6 : 5;
     const int64_t interior = (int64_t)d->m * d->n * (d->dim == 3 ? d->l : 1);
-    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + 4096;
+    const int64_t mixed = (d->precision == PF_PREC_MIXED) ? (int64_t)4 * 3 * 2 * g.HS : 0;
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + 4096;
 }
 
 static int alloc(pf_ctx *c, double **p, int64_t n) {
@@ -109,6 +120,11 @@ static void release(pf_ctx *c) {
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (c->err) { (void)hipFree(c->err); c->err = nullptr; }
     if (c->err_part) { (void)hipFree(c->err_part); c->err_part = nullptr; }
+    if (c->res_part) { (void)hipFree(c->res_part); c->res_part = nullptr; }
+    if (c->res_val) { (void)hipFree(c->res_val); c->res_val = nullptr; }
+    float **f32[] = {&c->E32, &c->R32, &c->PHI32};
+    for (float **p : f32)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     c->evs.clear();
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -169,11 +185,27 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         }
     }
     c->nparts = pf::sor_blocks(g);
-    if (hipMalloc((void **)&c->err_part, sizeof(unsigned long long) * (size_t)c->nparts) != hipSuccess) {
-        rc = fail(c, PF_ENOMEM, "hipMalloc of the error slots failed");
+    if (hipMalloc((void **)&c->err_part, sizeof(unsigned long long) * (size_t)c->nparts) != hipSuccess ||
+        hipMalloc((void **)&c->res_part, sizeof(unsigned long long) * (size_t)c->nparts) != hipSuccess ||
+        hipMalloc((void **)&c->res_val, sizeof(unsigned long long)) != hipSuccess) {
+        rc = fail(c, PF_ENOMEM, "hipMalloc of the reduction slots failed");
         goto bad;
     }
     pf::launch_clear_u64(c->st, c->err_part, c->nparts);
+    pf::launch_clear_u64(c->st, c->res_part, c->nparts);
+    if (desc->precision == PF_PREC_MIXED) {
+        float **f32[] = {&c->E32, &c->R32, &c->PHI32};
+        for (float **p : f32) {
+            if (hipMalloc((void **)p, sizeof(float) * (size_t)(2 * g.HS)) != hipSuccess) {
+                rc = fail(c, PF_ENOMEM, "hipMalloc of the FP32 arrays failed");
+                goto bad;
+            }
+            if (hipMemsetAsync(*p, 0, sizeof(float) * (size_t)(2 * g.HS), c->st) != hipSuccess) {
+                rc = fail(c, PF_EHIP, "hipMemsetAsync failed");
+                goto bad;
+            }
+        }
+    }
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         rc = fail(c, PF_EHIP, "stream synchronize failed after allocation");
         goto bad;
@@ -220,6 +252,7 @@ static int derive_from_porosity(pf_ctx *c) {
     const PfGeom &g = c->g;
     pf::launch_faces(c->st, g, c->phi, c->AW, c->AS, c->AB, c->d.dt, c->d.dx, c->d.dy, g.dim == 3 ? c->d.dz : 1.0);
     pf::launch_split(c->st, g, c->phi, c->PHIs);
+    if (c->PHI32) pf::launch_split32(c->st, g, c->phi, c->PHI32);
     int *bad = nullptr;
     PF_TRY_HIP(c, hipMalloc((void **)&bad, sizeof(int)));
     int zero = 0, hbad = 1;
@@ -373,7 +406,7 @@ extern "C" int pf_initial(pf_ctx *c) {
 // ------------------------------------------------------------------------------------
 // The time step
 // ------------------------------------------------------------------------------------
-static void enqueue_sor(pf_ctx *c, unsigned long long *slot) {
+static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
     const PfGeom &g = c->g;
     const bool periodic = (c->d.bc == PF_BC_UNIFORM);
     const int fy = periodic && (g.n % 2 == 0);
@@ -382,7 +415,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot) {
     const int rz = periodic && g.dim == 3 && !fz;
     // 3D sweeps (i+j+k) even first (:493), 2D sweeps (i+j) odd first (:339)
     const int first = (g.dim == 3) ? 0 : 1;
-    for (int it = 0; it < c->d.iter_max; ++it) {
+    for (int it = 0; it < iters; ++it) {
         for (int half = 0; half < 2; ++half) {
             pf::SweepArgs s;
             s.colour = half == 0 ? first : 1 - first;
@@ -401,6 +434,105 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot) {
     }
 }
 
+// Mixed precision pressure solve of one step (see pf_mixed.hip).  Returns cycles run;
+// *res receives the last evaluated residual (or -1).
+static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *iters) {
+    const PfGeom &g = c->g;
+    const bool periodic = (c->d.bc == PF_BC_UNIFORM);
+    const int fy = periodic && (g.n % 2 == 0);
+    const int fz = periodic && g.dim == 3 && (g.l % 2 == 0);
+    const int ry = periodic && !fy;
+    const int rz = periodic && g.dim == 3 && !fz;
+    const int first = (g.dim == 3) ? 0 : 1;
+    const bool fast = (c->sor_variant == 2);
+    const int K = c->d.inner_iters > 0 ? c->d.inner_iters : (c->d.iter_max > 0 ? c->d.iter_max : 1);
+    const bool tol_mode = c->d.residual_tol > 0;
+    const int maxc = tol_mode ? (c->d.max_cycles > 0 ? c->d.max_cycles : 1000) : (c->d.iter_max + K - 1) / K;
+    int remaining = c->d.iter_max, cycles = 0;
+    *res = -1.0;
+    *iters = 0;
+    auto residual = [&](bool norm) -> int {
+        for (int half = 0; half < 2; ++half) {
+            pf::SweepArgs s{};
+            s.colour = half;
+            s.fuse_y = fy;
+            s.fuse_z = fz;
+            s.track_err = norm ? 1 : 0;
+            s.xcd_remap = c->xcd_remap;
+            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, c->R32, c->E32, c->res_part, fast);
+        }
+        if (!norm) return PF_OK;
+        pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
+        unsigned long long bits = 0;
+        PF_TRY_HIP(c, hipMemcpyAsync(&bits, c->res_val, sizeof(bits), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        std::memcpy(res, &bits, sizeof(double));
+        return PF_OK;
+    };
+    for (int cyc = 0; cyc < maxc; ++cyc) {
+        int rc;
+        if (tol_mode) {
+            if ((rc = residual(true)) != PF_OK) return rc;
+            if (*res <= c->d.residual_tol) break;
+        } else {
+            if (remaining <= 0) break;
+            if ((rc = residual(false)) != PF_OK) return rc;
+        }
+        const int kk = tol_mode ? K : (remaining < K ? remaining : K);
+        for (int it = 0; it < kk; ++it) {
+            for (int half = 0; half < 2; ++half) {
+                pf::SweepArgs s{};
+                s.colour = half == 0 ? first : 1 - first;
+                s.fuse_y = fy;
+                s.fuse_z = fz;
+                s.track_err = (g.dim == 2) || (half == 1);
+                s.xcd_remap = c->xcd_remap;
+                pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot);
+                if (ry || rz) pf::launch_ghost_refresh32(c->st, g, ry, rz, c->E32);
+            }
+        }
+        pf::launch_update(c->st, g, c->E32, c->P);
+        remaining -= kk;
+        *iters += kk;
+        ++cycles;
+    }
+    if (!tol_mode && c->d.report_residual) {
+        int rc = residual(true);
+        if (rc != PF_OK) return rc;
+    }
+    return cycles;
+}
+
+// FP64 solve to a residual tolerance: cycles of inner_iters reference SOR iterations with an
+// FP64 residual check before each (the FP64 counterpart of the mixed tolerance mode).
+static int enqueue_fp64_tol(pf_ctx *c, double *res, int *iters) {
+    const PfGeom &g = c->g;
+    const int K = c->d.inner_iters > 0 ? c->d.inner_iters : (c->d.iter_max > 0 ? c->d.iter_max : 1);
+    const int maxc = c->d.max_cycles > 0 ? c->d.max_cycles : 1000;
+    const bool fast = (c->sor_variant == 2);
+    int cycles = 0;
+    *iters = 0;
+    for (int cyc = 0; cyc <= maxc; ++cyc) {
+        for (int half = 0; half < 2; ++half) {
+            pf::SweepArgs s{};
+            s.colour = half;
+            s.track_err = 1;
+            s.xcd_remap = c->xcd_remap;
+            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, nullptr, nullptr, c->res_part, fast);
+        }
+        pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
+        unsigned long long bits = 0;
+        PF_TRY_HIP(c, hipMemcpyAsync(&bits, c->res_val, sizeof(bits), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        std::memcpy(res, &bits, sizeof(double));
+        if (*res <= c->d.residual_tol || cyc == maxc) break;
+        enqueue_sor(c, c->err_part, K);
+        *iters += K;
+        ++cycles;
+    }
+    return cycles;
+}
+
 extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
     if (!c || nsteps < 0) return PF_EINVAL;
     if (!c->have_init) return fail(c, PF_ESTATE, "pf_step before pf_initial");
@@ -413,6 +545,9 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         c->err_cap = nsteps;
     }
     pf::launch_clear_u64(c->st, c->err, nsteps);
+    c->mixed_res.clear();
+    c->mixed_cycles.clear();
+    c->mixed_iters.clear();
     if (c->timing) {
         while (c->evs.size() < (size_t)(2 * nsteps)) {
             hipEvent_t e;
@@ -428,7 +563,25 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         pf::launch_predict(c->st, g, c->a, c->uo, c->vo, c->wo, c->div, c->phi, c->u, c->v, c->w);
         pf::launch_rhs(c->st, g, c->a, c->u, c->v, c->w, c->phi, c->AW, c->AS, c->AB, c->P, c->BB);
         if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s)], c->st));
-        enqueue_sor(c, c->err_part);
+        if (c->d.precision == PF_PREC_MIXED) {
+            double res = -1.0;
+            int iters = 0;
+            const int cyc = enqueue_mixed(c, c->err_part, &res, &iters);
+            if (cyc < 0) return cyc;
+            c->mixed_res.push_back(res);
+            c->mixed_cycles.push_back(cyc);
+            c->mixed_iters.push_back(iters);
+        } else if (c->d.residual_tol > 0) {
+            double res = -1.0;
+            int iters = 0;
+            const int cyc = enqueue_fp64_tol(c, &res, &iters);
+            if (cyc < 0) return cyc;
+            c->mixed_res.push_back(res);
+            c->mixed_cycles.push_back(cyc);
+            c->mixed_iters.push_back(iters);
+        } else {
+            enqueue_sor(c, c->err_part, c->d.iter_max);
+        }
         pf::launch_reduce_err(c->st, c->err_part, c->nparts, c->err + s);
         if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s + 1)], c->st));
         pf::launch_project(c->st, g, c->a, c->P, c->u, c->v, c->w);
@@ -445,6 +598,7 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
             PF_TRY_HIP(c, hipEventElapsedTime(&ms, c->evs[(size_t)(2 * s)], c->evs[(size_t)(2 * s + 1)]));
             sweep_ms += ms;
         }
+        c->solve_us_avg = sweep_ms * 1000.0 / (double)nsteps;
         c->sweep_launches = (int64_t)nsteps * 2 * c->d.iter_max;
         c->sweep_us_avg = sweep_ms * 1000.0 / (double)c->sweep_launches;
     }
@@ -454,10 +608,13 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         std::memcpy(&v, &e[(size_t)s], sizeof(double));
         if (!std::isfinite(v)) finite = false;
         if (stats) {
+            const bool mixed = (c->d.precision == PF_PREC_MIXED);
             stats[s].sor_error = v;
-            stats[s].residual = 0.0;
-            stats[s].sor_iters = c->d.iter_max;
-            stats[s].cycles = 0;
+            (void)mixed;
+            const bool cyc = mixed || c->d.residual_tol > 0;
+            stats[s].residual = cyc ? c->mixed_res[(size_t)s] : -1.0;
+            stats[s].sor_iters = cyc ? c->mixed_iters[(size_t)s] : c->d.iter_max;
+            stats[s].cycles = cyc ? c->mixed_cycles[(size_t)s] : 0;
         }
     }
     if (!finite) return fail(c, PF_EDIVERGED, "non-finite SOR error: the run diverged");

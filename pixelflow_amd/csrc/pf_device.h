@@ -1,0 +1,117 @@
+// pf_device.h -- device helpers shared by the FP64 and mixed-precision kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pf_kernels.h"
+
+#define PF_SMALL 1.e-6
+#define PF_ALPHA 32.0
+
+namespace pf {
+
+__device__ __forceinline__ double fmax_ref(double a, double b) { return (a > b) ? a : b; }
+
+// ------------------------------------------------------------------------------------
+// boundary_matrix row edits, replayed per cell in the reference's order.
+// Uniform: inlet fold at i=1 then outlet row at i=m (3D :636-659, 2D :424-436).
+// Air: top, bottom, east, west, north, south (ibm_3d_air_condition_omp_cpu.f90:685-864).
+// WITH_BB adds the outlet terms to *bb (p ghosts read from the colour-split P); the top
+// outlet assigns bb(i,j,l) = bb(i,j,1) + at*p(i,j,l+1) as written there (:702).
+// ------------------------------------------------------------------------------------
+template <typename T>
+struct CoefT {
+    T ae, aw, an, as, at, ab;
+};
+using Coef = CoefT<double>;
+
+template <typename T>
+__device__ __forceinline__ void zero6(CoefT<T> &c) { c.ae = T(0); c.aw = T(0); c.an = T(0); c.as = T(0); c.at = T(0); c.ab = T(0); }
+
+__device__ __forceinline__ bool is_edge_row(const PfGeom &g, const StepArgs &a, int i, int j, int k) {
+    if (a.bc == PF_BC_UNIFORM) return i == 1 || i == g.m;
+    return i == 1 || i == g.m || j == 1 || j == g.n || k == 1 || k == g.l;
+}
+
+template <bool WITH_BB, typename T = double>
+__device__ void row_rules(const PfGeom &g, const StepArgs &a, int i, int j, int k, CoefT<T> &c,
+                          double *bb, const double *P, const double *phi, double bb_k1) {
+    if (a.bc == PF_BC_UNIFORM) {
+        if (i == 1) { c.ae = c.ae + c.aw; c.aw = T(0); }
+        if (i == g.m) {
+            if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, g.m + 1, j, k)]);
+            zero6(c);
+        }
+        return;
+    }
+    const int m = g.m, n = g.n, l = g.l;
+    const double pcell = phi[pf_off(g, i, j, k)];
+    const bool fluid = !(pcell < 0.9);
+    if (k == l) {
+        if (a.face[0] == 0 || a.face[0] == 1 || !fluid) { c.ab = c.ab + c.at; c.at = T(0); }
+        else {
+            if (WITH_BB) *bb = bb_k1 + (c.at * P[pf_soff(g, i, j, l + 1)]);
+            zero6(c);
+        }
+    }
+    if (k == 1) {
+        if (a.face[1] == 0 || a.face[1] == 1 || !fluid) { c.at = c.at + c.ab; c.ab = T(0); }
+        else { if (WITH_BB) *bb = *bb + (c.ab * P[pf_soff(g, i, j, 0)]); zero6(c); }
+    }
+    if (i == m) {
+        if (a.face[2] == 0 || a.face[2] == 1 || !fluid) { c.aw = c.aw + c.ae; c.ae = T(0); }
+        else { if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, m + 1, j, k)]); zero6(c); }
+    }
+    if (i == 1) {
+        if (a.face[3] == 0 || a.face[3] == 1 || !fluid) { c.ae = c.ae + c.aw; c.aw = T(0); }
+        else { if (WITH_BB) *bb = *bb + (c.aw * P[pf_soff(g, 0, j, k)]); zero6(c); }
+    }
+    if (j == n) {
+        if (a.face[4] == 0 || a.face[4] == 1 || !fluid) { c.as = c.as + c.an; c.an = T(0); }
+        else { if (WITH_BB) *bb = *bb + (c.an * P[pf_soff(g, i, n + 1, k)]); zero6(c); }
+    }
+    if (j == 1) {
+        if (a.face[5] == 0 || a.face[5] == 1 || !fluid) { c.an = c.an + c.as; c.as = T(0); }
+        else { if (WITH_BB) *bb = *bb + (c.as * P[pf_soff(g, i, 0, k)]); zero6(c); }
+    }
+}
+
+// XCD-aware block order.  The hardware hands workgroup L of a 1D grid to XCD L % 8; each
+// XCD has its own L2.  Renumbering so that XCD x receives the contiguous range
+// [x*B/8, (x+1)*B/8) of logical blocks keeps the rows and planes that neighbouring blocks
+// share (the j-1/j+1 rows and k-1/k+1 planes of a stencil) in one L2.
+__device__ __forceinline__ int xcd_logical_block(int L, int B) {
+    const int q = B >> 3, rem = B & 7;
+    const int x = L & 7, r = L >> 3;
+    return (x < rem) ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+
+// Variant of the sweep that rebuilds the six face coefficients of each updated cell from
+// porosity instead of reading them: AW(i) = dt*max(small,(phi(i)+phi(i-1))*0.5)/dx/dx is
+// evaluated with the same operands and order as solve_p (3D :390-400, 2D :265-268), so the
+// values are the stored ones bit for bit, while a half-sweep streams 3 words per cell
+// (p of both colours, porosity of both colours, bb of the colour, p written) instead of 5.
+// PHIo / PHIx: porosity in the colour-split layout (own / other colour).
+__device__ __forceinline__ double face_coef(double pa, double pb, double dt, double d) {
+    return ((dt * fmax_ref(PF_SMALL, (pa + pb) * 0.5)) / d) / d;
+}
+
+// a / d for a constant divisor d with rd = RN(1/d) from the host: q0 = a*rd is within one
+// ulp of a/d, the residual a - q0*d is exact in an FMA, and one FMA correction gives the
+// correctly rounded quotient (Markstein's theorem; 3 instructions instead of the ~10 of a
+// general FP64 division).  k_faces_check verifies every face of a case against true
+// division before this path is used.
+__device__ __forceinline__ double div_const(double a, double d, double rd) {
+    const double q0 = a * rd;
+    const double e = __builtin_fma(-q0, d, a);
+    return __builtin_fma(e, rd, q0);
+}
+__device__ __forceinline__ double face_coef_fast(double pa, double pb, double dt, double d, double rd) {
+    return div_const(div_const(dt * fmax_ref(PF_SMALL, (pa + pb) * 0.5), d, rd), d, rd);
+}
+
+template <bool FAST>
+__device__ __forceinline__ double face_any(double pa, double pb, double dt, double d, double rd) {
+    return FAST ? face_coef_fast(pa, pb, dt, d, rd) : face_coef(pa, pb, dt, d);
+}
+
+}  // namespace pf

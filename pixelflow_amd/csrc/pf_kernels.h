@@ -60,7 +60,16 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
 int sor_blocks(const PfGeom &g);
-void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n);   // blocks of one colour-sweep launch (= error slots)
+void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n);
+// mixed precision (pf_mixed.hip)
+void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
+                     const double *PHIs, const double *BB, const double *phi, float *R, float *E,
+                     unsigned long long *slots, bool fast);
+void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
+                  const float *R, const double *phi, unsigned long long *err_slot);
+void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P);
+void launch_ghost_refresh32(hipStream_t st, const PfGeom &g, int do_y, int do_z, float *E);
+void launch_split32(hipStream_t st, const PfGeom &g, const double *src, float *dst);   // blocks of one colour-sweep launch (= error slots)
 void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err);
 
 }  // namespace pf

@@ -31,7 +31,7 @@ class PfDesc(ctypes.Structure):
         ("dim", ctypes.c_int32), ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("l", ctypes.c_int32),
         ("bc", ctypes.c_int32), ("nonslip", ctypes.c_int32), ("iter_max", ctypes.c_int32),
         ("face", ctypes.c_int32 * 6), ("precision", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
-        ("max_cycles", ctypes.c_int32),
+        ("max_cycles", ctypes.c_int32), ("report_residual", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
         ("xnue", ctypes.c_double), ("xlambda", ctypes.c_double), ("density", ctypes.c_double),
         ("thickness", ctypes.c_double), ("inlet_velocity", ctypes.c_double),
@@ -89,7 +89,7 @@ def lib() -> ctypes.CDLL:
 
 
 def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0, max_cycles: int = 0,
-              residual_tol: float = 0.0) -> PfDesc:
+              residual_tol: float = 0.0, report_residual: bool = False) -> PfDesc:
     d = PfDesc()
     d.dim, d.m, d.n, d.l = p.dim, p.m, p.n, (p.l if p.dim == 3 else 1)
     d.bc = PF_BC_AIR if p.bc == "air" else PF_BC_UNIFORM
@@ -100,6 +100,7 @@ def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0
     d.precision = precision
     d.inner_iters = inner_iters
     d.max_cycles = max_cycles
+    d.report_residual = 1 if report_residual else 0
     d.dx, d.dy, d.dz, d.dt = p.dx, p.dy, (p.dz if p.dim == 3 else 1.0), p.dt
     d.xnue, d.xlambda, d.density, d.thickness = p.xnue, p.xlambda, p.density, p.thickness
     d.inlet_velocity, d.outlet_pressure, d.aoa = p.inlet_velocity, p.outlet_pressure, p.aoa

@@ -1,0 +1,82 @@
+"""Mixed precision (FP32 SOR sweeps inside FP64 defect correction) against the FP64 path.
+
+The FP64 path is bit-identical to the reference (test_gpu_parity.py), so it is the FP64
+oracle here.
+
+* Tolerance mode (cycles until the FP64 residual max|r/ap| <= tol) against the FP64 path
+  run to the same stopping rule: relative L2 error <= 1e-6 in every field.
+* Fixed-count mode (the reference's iter_max, cycles of K FP32 iterations): velocities
+  within 1e-6 (relative L2), the output_divergent diagnostic within 1e-6 of its own
+  magnitude; the pressure of an unconverged SOR iterate carries FP32 rounding that grows
+  with K (measured 1e-4 at K = 100, 1e-5 at K = 10 on the 2D cylinder), held to 2e-4.
+  "Final divergence <= 1e-8" is the FP64 Poisson residual max|r/ap|, which tolerance mode
+  drives below 1e-8 (checked in the last test).
+"""
+import numpy as np
+import pytest
+
+from pixelflow_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-6
+P_FIXED_TOL = 2e-4
+DIV_REL_TOL = 1e-6
+
+
+def _pair(p, phi, steps, **mixed_kw):
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    with Solver(p, phi) as a, Solver(p, phi, precision=PF_PREC_MIXED, **mixed_kw) as b:
+        sa = a.step(steps)
+        sb = b.step(steps)
+        fields = ("u", "v", "w", "p") if p.dim == 3 else ("u", "v", "p")
+        errs = {}
+        for f in fields:
+            x, y = a.interior(f), b.interior(f)
+            errs[f] = float(np.linalg.norm(y - x) / max(np.linalg.norm(x), 1e-300))
+        da, db = a.divergence(), b.divergence()
+        ddiv = float(np.max(np.abs(da - db)) / max(np.max(np.abs(da)), 1e-300))
+        return errs, ddiv, sa, sb
+
+
+CASES = {
+    "3d_sphere_64x32x32": lambda: cases.sphere_case(m=64, n=32, l=32, steps=20),
+    "3d_odd_45x27x19": lambda: (cases.CaseParams(dim=3, m=45, n=27, l=19, width=1.0, height=0.59, depth=0.41,
+                                                 time=20e-3, istep_max=20, iter_max=100),
+                                cases.sphere_3d(45, 27, 19, 1.0, 0.59, 0.41, (0.3, 0.3, 0.2), 0.2)),
+    "2d_c1_256x128": lambda: cases.cylinder_case(steps=30),
+    "3d_air_room_40x36x32": lambda: cases.room_case(4, steps=15, iter_max=100, size=(40, 36, 32)),
+    "3d_boxes_96x48x48": lambda: cases.boxes_case(m=96, n=48, l=48, steps=10, nboxes=6, side_range=(4, 12)),
+}
+
+
+@pytest.mark.parametrize("inner", [10, 25])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_fixed_count_matches_fp64(name, inner):
+    p, phi = CASES[name]()
+    print(f"[gpu] mixed {name} inner={inner}", flush=True)
+    errs, ddiv, sa, sb = _pair(p, phi, p.istep_max, inner_iters=inner, report_residual=True)
+    print(f"[gpu] mixed {name} inner={inner} rel-L2 {errs} max|ddiv| {ddiv:.3e}", flush=True)
+    assert all(e <= FIELD_TOL for f, e in errs.items() if f != "p"), errs
+    assert errs["p"] <= P_FIXED_TOL, errs
+    assert ddiv <= DIV_REL_TOL
+    assert np.all(sb["sor_iters"] == p.iter_max)
+    assert np.all(sb["residual"] >= 0)
+    # the SOR diagnostic tracks the FP64 one (same iteration, FP32 increments)
+    np.testing.assert_allclose(sb["sor_error"], sa["sor_error"], rtol=1e-3)
+
+
+def test_mixed_tolerance_mode_matches_fp64_tolerance_mode():
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, phi = cases.sphere_case(m=48, n=24, l=24, steps=5)
+    tol = 1e-8
+    kw = dict(inner_iters=20, residual_tol=tol, max_cycles=400)
+    with Solver(p, phi, **kw) as a, Solver(p, phi, precision=PF_PREC_MIXED, **kw) as b:
+        sa = a.step(5)
+        sb = b.step(5)
+        assert np.all(sa["residual"] <= tol) and np.all(sb["residual"] <= tol), (sa["residual"], sb["residual"])
+        for f in ("u", "v", "w", "p"):
+            x, y = a.interior(f), b.interior(f)
+            rel = np.linalg.norm(y - x) / np.linalg.norm(x)
+            assert rel <= FIELD_TOL, (f, rel)
+        print(f"[gpu] tolerance mode iterations fp64 {sa['sor_iters']} mixed {sb['sor_iters']}", flush=True)

@@ -59,7 +59,7 @@ CASES = {
     "3d_odd_45x27x19": lambda: (cases.CaseParams(dim=3, m=45, n=27, l=19, width=1.0, height=0.59, depth=0.41,
                                                  time=3e-3, istep_max=3, iter_max=30),
                                 cases.sphere_3d(45, 27, 19, 1.0, 0.59, 0.41, (0.3, 0.3, 0.2), 0.2)),
-    "3d_boxes_48x32x24": lambda: cases.boxes_case(m=48, n=32, l=24, steps=3, iter_max=50),
+    "3d_boxes_48x32x24": lambda: cases.boxes_case(m=48, n=32, l=24, steps=3, iter_max=50, nboxes=4, side_range=(3, 10)),
     "3d_air_room_40x36x32": lambda: cases.room_case(4, steps=4, iter_max=50, size=(40, 36, 32)),
 }
 
