@@ -126,6 +126,48 @@ int pf_sor_kernel(const pf_ctx *ctx);
 int pf_probe_copy(int64_t bytes, int iters, double *gbps);
 int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
 
+/* ------------------------------------------------------------------------------------
+ * Batches: many independent cases of uneven size advanced together, one launch per stage
+ * of the step for all of them, the step captured in a hipGraph.  No reference
+ * counterpart (the reference runs one case per process); every case of a batch computes
+ * exactly what pf_step computes for it alone.  Cases of one batch share dim, bc, precision
+ * and the iteration settings (iter_max, inner_iters, max_cycles, residual_tol); they may be
+ * added and removed between steps (continuous batching).
+ * ------------------------------------------------------------------------------------ */
+typedef struct pf_batch pf_batch;
+
+typedef struct pf_batch_config {
+    int32_t persistent_blocks;  /* 0: one block per tile; > 0: this many blocks walk the tiles */
+    int32_t use_graph;          /* capture the fixed-count step in a hipGraph and replay it */
+    int32_t inner_iters;        /* mixed precision: FP32 iterations per defect-correction cycle */
+    int32_t reserved;
+} pf_batch_config;
+
+/* kind: the shared settings; arena_bytes: device memory for the cases' fields. */
+int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena_bytes, pf_batch **out);
+int pf_batch_destroy(pf_batch *b);
+const char *pf_batch_last_error(const pf_batch *b);
+/* Device bytes one case takes in a batch arena (-1 for an invalid descriptor). */
+int64_t pf_batch_case_bytes(const pf_desc *desc);
+/* Largest free contiguous range of the arena. */
+int64_t pf_batch_free_bytes(const pf_batch *b);
+/* Add a case (grid_conditions + initial_conditions + boundary, as pf_set_porosity and
+ * pf_initial); *slot receives its slot number in [0, max_cases). */
+int pf_batch_add(pf_batch *b, const pf_desc *desc, const double *raw_interior, double threshold, int *slot);
+int pf_batch_remove(pf_batch *b, int slot);
+int pf_batch_count(const pf_batch *b);
+/* nsteps steps of every case; stats[s * max_cases + slot] for the occupied slots. */
+int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats);
+int pf_batch_download(pf_batch *b, int slot, int field, double *host, int64_t ldi, int64_t ldj);
+int pf_batch_divergence(pf_batch *b, int slot, double *out_interior);
+int pf_batch_set_config(pf_batch *b, const pf_batch_config *cfg);
+int pf_batch_get_config(const pf_batch *b, pf_batch_config *cfg);
+int pf_batch_sync(pf_batch *b);
+/* Steps the case in `slot` has taken since it was added (-1 if the slot is empty). */
+int64_t pf_batch_case_steps(const pf_batch *b, int slot);
+/* Free and total device memory (hipMemGetInfo), for admission control. */
+int pf_device_memory(int64_t *free_bytes, int64_t *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

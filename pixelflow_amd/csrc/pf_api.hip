@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/pixelflow.h"
+#include "pf_host.h"
 #include "pf_kernels.h"
 
 struct pf_ctx {
@@ -33,6 +34,7 @@ struct pf_ctx {
     int sor_variant = 2;
     int sor_forced = -1;
     int xcd_remap = 1;
+    int sweep_kb = 1;               // 3D sweeps: cells per thread (1, 2 or 4)
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
     unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
     unsigned long long *res_part = nullptr;  // mixed: per-block max |r/ap|
@@ -66,28 +68,7 @@ static int fail(pf_ctx *c, int code, const std::string &m) {
     return code;
 }
 
-static bool desc_ok(const pf_desc *d, std::string &why) {
-    if (!d) { why = "null descriptor"; return false; }
-    if (d->dim != 2 && d->dim != 3) { why = "dim must be 2 or 3"; return false; }
-    if (d->m < 3 || d->n < 2 || (d->dim == 3 && d->l < 2)) { why = "extents too small (m>=3, n>=2, l>=2)"; return false; }
-    if (d->bc != PF_BC_UNIFORM && d->bc != PF_BC_AIR) { why = "bc must be PF_BC_UNIFORM or PF_BC_AIR"; return false; }
-    if (d->bc == PF_BC_AIR && d->dim != 3) { why = "the air-conditioning solver is 3D"; return false; }
-    for (int f = 0; f < 6; ++f)
-        if (d->face[f] < 0 || d->face[f] > 2) { why = "face codes must be 0, 1 or 2"; return false; }
-    if (d->iter_max < 0) { why = "iter_max < 0"; return false; }
-    if (d->precision != PF_PREC_FP64 && d->precision != PF_PREC_MIXED) { why = "precision must be FP64 or MIXED"; return false; }
-    if (d->precision == PF_PREC_MIXED && (d->inner_iters < 0 || d->max_cycles < 0 || !(d->residual_tol >= 0))) {
-        why = "mixed precision: inner_iters, max_cycles >= 0 and residual_tol >= 0";
-        return false;
-    }
-    if (!(d->dx > 0) || !(d->dy > 0) || (d->dim == 3 && !(d->dz > 0)) || !(d->dt > 0) || !(d->density != 0)) {
-        why = "dx, dy, dz, dt must be > 0 and density != 0";
-        return false;
-    }
-    const double cells = (double)d->m * d->n * (d->dim == 3 ? d->l : 1);
-    if (cells > 2.0e9) { why = "grid too large"; return false; }
-    return true;
-}
+static bool desc_ok(const pf_desc *d, std::string &why) { return pf::desc_valid(d, why); }
 
 extern "C" int pf_abi_version(void) { return PF_ABI_VERSION; }
 
@@ -143,22 +124,7 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     if (!c) return PF_ENOMEM;
     c->d = *desc;
     c->g = pf_make_geom(desc->dim, desc->m, desc->n, desc->l);
-    pf::StepArgs &a = c->a;
-    a.bc = desc->bc;
-    a.nonslip = desc->nonslip ? 1 : 0;
-    for (int f = 0; f < 6; ++f) a.face[f] = desc->face[f];
-    a.dt = desc->dt; a.dx = desc->dx; a.dy = desc->dy; a.dz = (desc->dim == 3) ? desc->dz : 1.0;
-    a.rdx = 1.0 / a.dx; a.rdy = 1.0 / a.dy; a.rdz = 1.0 / a.dz;
-    a.xnue = desc->xnue; a.xlambda = desc->xlambda; a.density = desc->density; a.thickness = desc->thickness;
-    a.omega = desc->omega;
-    a.omega_c = 1. - desc->omega;
-    // inlet components of boundary (3D :695-696 uses AoA/1300, 2D :478-479 uses AoA/180)
-    const double pi = std::atan(1.) * 4.;
-    const double ang = (desc->dim == 3) ? ((desc->aoa / 1300.) * pi) : ((desc->aoa / 180.) * pi);
-    a.u_in = desc->inlet_velocity * std::cos(ang);
-    a.v_in = desc->inlet_velocity * std::sin(ang);
-    a.inlet_velocity = desc->inlet_velocity;
-    a.outlet_pressure = desc->outlet_pressure;
+    c->a = pf::make_step_args(*desc);
 
     int rc;
     hipError_t e = hipGetDevice(&c->device);
@@ -213,6 +179,11 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     {
         const char *v = std::getenv("PF_SOR_VARIANT");
         if (v && *v) c->sor_forced = std::atoi(v);
+        const char *kb = std::getenv("PF_SWEEP_KB");
+        if (kb && *kb) {
+            const int v = std::atoi(kb);
+            c->sweep_kb = (v == 1 || v == 2 || v == 4) ? v : 1;
+        }
         const char *x = std::getenv("PF_XCD_REMAP");
         if (x && *x) c->xcd_remap = std::atoi(x) != 0;
     }
@@ -271,35 +242,8 @@ static int derive_from_porosity(pf_ctx *c) {
 
 extern "C" int pf_set_porosity(pf_ctx *c, const double *raw, double threshold) {
     if (!c || !raw) return PF_EINVAL;
-    const PfGeom &g = c->g;
-    const int m = g.m, n = g.n, l = g.l;
-    std::vector<double> phi((size_t)g.NS, 0.0);
-    auto at = [&](int i, int j, int k) -> double & { return phi[(size_t)pf_off(g, i, j, k)]; };
-    const int kmin = (g.dim == 3) ? 1 : 0, kmax = (g.dim == 3) ? l : 0;
-    for (int k = kmin; k <= kmax; ++k)
-        for (int j = 1; j <= n; ++j)
-            for (int i = 1; i <= m; ++i) {
-                const double vraw = raw[(size_t)(i - 1) + (size_t)m * ((size_t)(j - 1) + (size_t)n * (size_t)(k - kmin))];
-                at(i, j, k) = (vraw > threshold) ? vraw : threshold;   // max(poro_val, threshold)
-            }
-    if (g.dim == 2) {                                   // grid.f90:94-105
-        for (int j = 1; j <= n + 1; ++j) { at(0, j, 0) = at(1, j, 0); at(m + 1, j, 0) = at(m, j, 0); }
-        for (int i = 0; i <= m + 1; ++i) { at(i, 0, 0) = at(i, n, 0); at(i, n + 1, 0) = at(i, 1, 0); }
-    } else if (c->d.bc == PF_BC_AIR) {                  // grid.f90:217-242 (zero gradient)
-        for (int j = 0; j <= n + 1; ++j)
-            for (int k = 0; k <= l + 1; ++k) { at(0, j, k) = at(1, j, k); at(m + 1, j, k) = at(m, j, k); }
-        for (int i = 0; i <= m + 1; ++i)
-            for (int k = 0; k <= l + 1; ++k) { at(i, 0, k) = at(i, 1, k); at(i, n + 1, k) = at(i, n, k); }
-        for (int i = 0; i <= m + 1; ++i)
-            for (int j = 0; j <= n + 1; ++j) { at(i, j, 0) = at(i, j, 1); at(i, j, l + 1) = at(i, j, l); }
-    } else {                                            // grid.f90:351-377 (x outlet, y/z periodic)
-        for (int j = 1; j <= n + 1; ++j)
-            for (int k = 1; k <= l + 1; ++k) { at(0, j, k) = at(1, j, k); at(m + 1, j, k) = at(m, j, k); }
-        for (int i = 0; i <= m + 1; ++i)
-            for (int k = 0; k <= l + 1; ++k) { at(i, 0, k) = at(i, n, k); at(i, n + 1, k) = at(i, 1, k); }
-        for (int i = 0; i <= m + 1; ++i)
-            for (int j = 0; j <= n + 1; ++j) { at(i, j, 0) = at(i, j, l); at(i, j, l + 1) = at(i, j, 1); }
-    }
+    std::vector<double> phi;
+    pf::host_porosity(c->g, c->d.bc, raw, threshold, phi);
     PF_TRY_HIP(c, hipMemcpyAsync(c->phi, phi.data(), phi.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
     return derive_from_porosity(c);
 }
@@ -330,12 +274,7 @@ extern "C" int pf_upload(pf_ctx *c, int field, const double *host, int64_t ldi, 
     std::vector<double> stage((size_t)size, 0.0);
     PF_TRY_HIP(c, hipStreamSynchronize(c->st));
     if (split) PF_TRY_HIP(c, hipMemcpy(stage.data(), dst, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
-    for (int k = 0; k < g.Z; ++k)
-        for (int j = 0; j < g.Y; ++j)
-            for (int i = 0; i < g.m + 2; ++i) {
-                const double val = host[(size_t)i + (size_t)ldi * ((size_t)j + (size_t)ldj * (size_t)k)];
-                stage[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
-            }
+    pf::from_host(g, split, host, ldi, ldj, stage);
     PF_TRY_HIP(c, hipMemcpy(dst, stage.data(), (size_t)size * sizeof(double), hipMemcpyHostToDevice));
     if (field == PF_FIELD_POROSITY) return derive_from_porosity(c);
     return PF_OK;
@@ -352,11 +291,7 @@ extern "C" int pf_download(pf_ctx *c, int field, double *host, int64_t ldi, int6
     std::vector<double> stage((size_t)size);
     PF_TRY_HIP(c, hipStreamSynchronize(c->st));
     PF_TRY_HIP(c, hipMemcpy(stage.data(), src, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
-    for (int k = 0; k < g.Z; ++k)
-        for (int j = 0; j < g.Y; ++j)
-            for (int i = 0; i < g.m + 2; ++i)
-                host[(size_t)i + (size_t)ldi * ((size_t)j + (size_t)ldj * (size_t)k)] =
-                    stage[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))];
+    pf::to_host(g, split, stage, host, ldi, ldj);
     return PF_OK;
 }
 
@@ -368,24 +303,10 @@ extern "C" int pf_initial(pf_ctx *c) {
     if (!c->have_phi) return fail(c, PF_ESTATE, "pf_initial before pf_set_porosity");
     const PfGeom &g = c->g;
     const pf_desc &d = c->d;
-    double u0 = 0., v0 = 0.;
-    const double pi = std::atan(1.) * 4.;
-    if (d.bc == PF_BC_UNIFORM) {
-        // 3D initial_conditions uses AoA/360 (:782-783); 2D uses AoA/180 (:560-561)
-        const double ang = (g.dim == 3) ? ((d.aoa / 360.) * pi) : ((d.aoa / 180.) * pi);
-        u0 = d.inlet_velocity * std::cos(ang);
-        v0 = d.inlet_velocity * std::sin(ang);
-    }
-    std::vector<double> nat((size_t)g.NS, 0.0), spl((size_t)(2 * g.HS), 0.0);
-    const int kmin = (g.dim == 3) ? 1 : 0, kmax = (g.dim == 3) ? g.l : 0;
-    auto fill = [&](double val, bool split) {
-        std::vector<double> &buf = split ? spl : nat;
-        std::fill(buf.begin(), buf.end(), 0.0);
-        for (int k = kmin; k <= kmax; ++k)
-            for (int j = 1; j <= g.n; ++j)
-                for (int i = 1; i <= g.m; ++i)
-                    buf[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
-    };
+    double u0, v0;
+    pf::initial_values(d, u0, v0);
+    std::vector<double> nat, spl;
+    auto fill = [&](double val, bool split) { pf::fill_interior(g, val, split, split ? spl : nat); };
     fill(u0, false);
     PF_TRY_HIP(c, hipMemcpy(c->u, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
     fill(v0, false);
@@ -425,6 +346,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
             s.track_err = (g.dim == 2) || (half == 1);
             s.xcd_remap = c->xcd_remap;
             s.gx = s.gy = 1;
+            s.kb = c->sweep_kb;
             if (c->sor_variant == 0)
                 pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
             else
@@ -487,6 +409,7 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
                 s.fuse_z = fz;
                 s.track_err = (g.dim == 2) || (half == 1);
                 s.xcd_remap = c->xcd_remap;
+                s.kb = c->sweep_kb;
                 pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot);
                 if (ry || rz) pf::launch_ghost_refresh32(c->st, g, ry, rz, c->E32);
             }
@@ -561,7 +484,7 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         std::swap(c->w, c->wo);
         pf::launch_div(c->st, g, c->a, c->uo, c->vo, c->wo, c->div);
         pf::launch_predict(c->st, g, c->a, c->uo, c->vo, c->wo, c->div, c->phi, c->u, c->v, c->w);
-        pf::launch_rhs(c->st, g, c->a, c->u, c->v, c->w, c->phi, c->AW, c->AS, c->AB, c->P, c->BB);
+        pf::launch_rhs(c->st, g, c->a, c->u, c->v, c->w, c->phi, c->P, c->BB);
         if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s)], c->st));
         if (c->d.precision == PF_PREC_MIXED) {
             double res = -1.0;

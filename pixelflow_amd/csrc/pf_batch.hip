@@ -1,0 +1,884 @@
+// pf_batch.hip -- many independent cases of uneven size advanced together.
+//
+// All cases of a batch live in one device arena; a table of BatchCase records gives each
+// one's geometry, constants and field addresses.  Every stage of the step is ONE launch for
+// the whole batch: the work is cut into tiles (64 x 4 threads; natural-cell tiles for the
+// stencil stages, colour-slot tiles for the sweeps), tiles of all cases are numbered
+// consecutively (prefix sums per case), and a persistent grid of blocks walks the tile list
+// (block b takes tiles b, b + G, b + 2G, ... after an XCD-contiguous renumbering, so each
+// XCD's L2 sees neighbouring tiles).  The per-cell arithmetic is pf_cells.h, shared with the
+// single-case path, so a case gives the same bits alone and inside any batch.
+// The launch sequence of one step is captured once into a hipGraph per velocity parity and
+// replayed; per-step diagnostics go to a history indexed by a device step counter.
+// Cases can be added and removed between steps (continuous batching): the arena has a
+// first-fit allocator, and the tables and graphs are rebuilt when membership changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pixelflow.h"
+#include "pf_batch.h"
+#include "pf_cells.h"
+#include "pf_device.h"
+#include "pf_host.h"
+#include "pf_kernels.h"
+
+namespace pf {
+
+__device__ __forceinline__ int find_case(const int *prefix, int ncases, int t) {
+    int lo = 0, hi = ncases - 1;   // largest c with prefix[c] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Block max of v into slots[t]; all threads of the block call it once per tile.
+__device__ __forceinline__ void tile_max_to_slot(double v, unsigned long long *slots, int t) {
+    unsigned long long e = __double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) red[lane >> 6] = e;
+    __syncthreads();
+    if (lane == 0) {
+        unsigned long long b = red[0];
+        for (int q = 1; q < 4; ++q) b = (red[q] > b) ? red[q] : b;
+        if (b > slots[t]) slots[t] = b;
+    }
+    __syncthreads();   // red[] is reused by the next tile
+}
+
+enum NatOp {
+    NAT_DIV = 0,
+    NAT_PREDICT = 1,
+    NAT_RHS = 2,
+    NAT_PROJECT = 3,
+    NAT_BOUNDARY_UNIFORM = 4,
+    NAT_BOUNDARY_AIR = 5,   // aux = face (0 top, 1 bottom, 2 west, 3 east, 4 north, 5 south)
+    NAT_UPDATE = 6,         // p += e (mixed)
+    NAT_REFRESH_P = 7,      // periodic ghost refresh for odd extents
+    NAT_REFRESH_E = 8
+};
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int aux) {
+    const int G = (int)gridDim.x;
+    const int L = xcd_logical_block((int)blockIdx.x, G);
+    for (int t = L; t < v.nat_tiles; t += G) {
+        const int ci = find_case(v.nat_prefix, v.ncases, t);
+        if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) continue;
+        const BatchCase &C = v.cases[ci];
+        const PfGeom &g = C.g;
+        if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) continue;
+        const int local = t - v.nat_prefix[ci];
+        const int tx = local % C.nat_gx, r = local / C.nat_gx;
+        const int ty = r % C.nat_gy, tz = r / C.nat_gy;
+        const int i = tx * 64 + (int)threadIdx.x, j = ty * 4 + (int)threadIdx.y, k = tz;
+        if (i > g.m + 1 || j > g.n + 1) continue;
+        const int p = parity, q = 1 - parity;
+        if (OP == NAT_DIV) {
+            div_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, i, j, k);
+        } else if (OP == NAT_PREDICT) {
+            if (g.dim == 3) predict3_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, C.phi, C.U[p], C.V[p], C.W[p], i, j, k);
+            else predict2_cell(g, C.a, C.U[q], C.V[q], C.div, C.phi, C.U[p], C.V[p], i, j);
+        } else if (OP == NAT_RHS || OP == NAT_PROJECT) {
+            const bool interior = i >= 1 && i <= g.m && j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
+            if (!interior) continue;
+            if (OP == NAT_RHS) rhs_cell(g, C.a, C.U[p], C.V[p], C.W[p], C.phi, C.P, C.BB, i, j, k);
+            else project_cell(g, C.a, C.P, C.U[p], C.V[p], C.W[p], i, j, k);
+        } else if (OP == NAT_BOUNDARY_UNIFORM) {
+            const bool bnd = (i == 0 || i == 1 || i == g.m + 1 || j == 0 || j == g.n + 1 ||
+                              (g.dim == 3 && (k == 0 || k == g.l + 1)));
+            if (bnd) boundary_uniform_cell(g, C.a, C.U[p], C.V[p], C.W[p], C.P, i, j, k);
+        } else if (OP == NAT_BOUNDARY_AIR) {
+            int rr = -1, qq = -1;
+            if (aux == 0 && k == g.l) { rr = i; qq = j; }
+            else if (aux == 1 && k == 1) { rr = i; qq = j; }
+            else if (aux == 2 && i == 1) { rr = j; qq = k; }
+            else if (aux == 3 && i == g.m) { rr = j; qq = k; }
+            else if (aux == 4 && j == g.n) { rr = i; qq = k; }
+            else if (aux == 5 && j == 1) { rr = i; qq = k; }
+            if (rr >= 0) boundary_air_cell(g, C.a, aux, C.phi, C.U[p], C.V[p], C.W[p], C.P, rr, qq);
+        } else if (OP == NAT_UPDATE) {
+            const int64_t so = pf_soff(g, i, j, k);
+            C.P[so] = C.P[so] + (double)C.E[so];
+        } else if (OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) {
+            const bool iin = i >= 1 && i <= g.m;
+            int sj = -1, sk = -1;
+            if (C.ref_y && iin && (j == 0 || j == g.n + 1) && (g.dim == 2 || (k >= 1 && k <= g.l))) {
+                sj = (j == 0) ? g.n : 1;
+                sk = k;
+            } else if (C.ref_z && g.dim == 3 && iin && j >= 1 && j <= g.n && (k == 0 || k == g.l + 1)) {
+                sj = j;
+                sk = (k == 0) ? g.l : 1;
+            }
+            if (sj < 0) continue;
+            if (OP == NAT_REFRESH_P) C.P[pf_soff(g, i, j, k)] = C.P[pf_soff(g, i, sj, sk)];
+            else C.E[pf_soff(g, i, j, k)] = C.E[pf_soff(g, i, sj, sk)];
+        }
+    }
+}
+
+enum SwpKind { SWP_FP64 = 0, SWP_FP32 = 1, SWP_RESIDUAL = 2 };
+
+template <int DIM, int KIND>
+__global__ __launch_bounds__(256) void k_batch_swp(BatchView v, int colour, int track, unsigned long long *slots) {
+    const int G = (int)gridDim.x;
+    const int L = xcd_logical_block((int)blockIdx.x, G);
+    for (int t = L; t < v.swp_tiles; t += G) {
+        const int ci = find_case(v.swp_prefix, v.ncases, t);
+        if (!v.active[ci]) continue;   // block-uniform: every thread skips the tile together
+        const BatchCase &C = v.cases[ci];
+        const PfGeom &g = C.g;
+        const int local = t - v.swp_prefix[ci];
+        const int bx = local % C.swp_gx, r = local / C.swp_gx;
+        const int by = r % C.swp_gy, bz = r / C.swp_gy;
+        int j, k;
+        if (DIM == 3) {
+            j = by * 2 + (int)(threadIdx.y >> 1) + 1;
+            k = bz * 2 + 1 + (int)(threadIdx.y & 1);
+        } else {
+            j = by * 4 + (int)threadIdx.y + 1;
+            k = 0;
+        }
+        const int sh = (colour + j + k) & 1;
+        const int h = bx * 64 + (int)threadIdx.x + 1 - sh;
+        const int i = 2 * h + sh;
+        double e = 0.;
+        if (j <= g.n && i <= g.m && (DIM == 2 || k <= g.l)) {
+            const int64_t o = (int64_t)h + (int64_t)g.HP * ((int64_t)j + (int64_t)g.Y * k);
+            const int64_t own = (int64_t)colour * g.HS, oth = (int64_t)(1 - colour) * g.HS;
+            SweepArgs s{};
+            s.colour = colour;
+            s.fuse_y = C.fuse_y;
+            s.fuse_z = C.fuse_z;
+            if (KIND == SWP_FP64) {
+                if (C.fast)
+                    e = sor_phi_cell<DIM, true>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
+                                                C.PHIs + oth, C.BB + own, C.phi);
+                else
+                    e = sor_phi_cell<DIM, false>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
+                                                 C.PHIs + oth, C.BB + own, C.phi);
+            } else if (KIND == SWP_FP32) {
+                const float cdx = (float)(C.a.dt / (C.a.dx * C.a.dx)), cdy = (float)(C.a.dt / (C.a.dy * C.a.dy)),
+                            cdz = (float)(C.a.dt / (C.a.dz * C.a.dz));
+                e = sor32_cell<DIM>(g, C.a, s, i, j, k, sh, o, C.E + oth, C.E + own, C.PHI32 + own, C.PHI32 + oth,
+                                    C.R + own, C.phi, cdx, cdy, cdz);
+            } else {
+                if (C.fast)
+                    e = residual_cell<DIM, true>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
+                else
+                    e = residual_cell<DIM, false>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
+            }
+        }
+        if (track) tile_max_to_slot(e, slots, t);
+    }
+}
+
+// Per case: max over its tile slots -> hist[step * stride + slot] (step from *d_step, or
+// the fixed row `row` when row >= 0); the slots are cleared.  One block per case.
+__global__ void k_batch_reduce(BatchView v, unsigned long long *slots, unsigned long long *hist, const int *d_step,
+                               int row, int stride) {
+    const int ci = (int)blockIdx.x;
+    if (ci >= v.ncases) return;
+    const int t0 = v.swp_prefix[ci], t1 = v.swp_prefix[ci + 1];
+    unsigned long long e = 0ull;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) {
+        const unsigned long long x = slots[t];
+        e = (x > e) ? x : e;
+        slots[t] = 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = red[0];
+        for (int q = 1; q < 4; ++q) b = (red[q] > b) ? red[q] : b;
+        const int r = (row >= 0) ? row : *d_step;
+        hist[(int64_t)r * stride + v.cases[ci].slot] = b;
+    }
+}
+
+__global__ void k_set_int(int *p, int val) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = val;
+}
+__global__ void k_inc_int(int *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + 1;
+}
+
+}  // namespace pf
+
+// ======================================================================================
+// Host side
+// ======================================================================================
+using pf::BatchCase;
+
+struct pf_batch {
+    pf_desc kind{};                 // settings shared by every case
+    int max_cases = 0;
+    hipStream_t st = nullptr;
+    char *arena = nullptr;
+    int64_t arena_bytes = 0;
+    std::map<int64_t, int64_t> free_list;   // offset -> size
+    struct Slot {
+        bool used = false;
+        pf_desc d{};
+        int64_t off = 0, bytes = 0;
+        BatchCase bc{};
+        int64_t steps = 0;
+    };
+    std::vector<Slot> slots;
+    std::vector<int> order;         // compact case index -> slot
+    BatchCase *d_cases = nullptr;
+    int *d_nat_prefix = nullptr, *d_swp_prefix = nullptr, *d_active = nullptr, *d_step = nullptr;
+    int nat_tiles = 0, swp_tiles = 0, table_cap = 0, tile_cap = 0;
+    unsigned long long *d_tile_slots = nullptr;   // per-tile SOR error maxima
+    unsigned long long *d_res_slots = nullptr;    // per-tile residual maxima
+    unsigned long long *d_err_hist = nullptr, *d_res_hist = nullptr;
+    int hist_rows = 0;
+    bool dirty = true;
+    bool any_refresh = false;
+    int parity = 0;
+    pf_batch_config cfg{};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::string msg;
+};
+
+#define PFB_TRY(b, call)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (b)->msg = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+            return PF_EHIP;                                                                     \
+        }                                                                                       \
+    } while (0)
+
+static thread_local std::string g_batch_create_error;
+
+static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// Bytes of one case inside a batch arena (velocity pairs, porosity, div, p/bb/porosity in
+// colour-split FP64, FP32 e/r/porosity for mixed precision, the interior output buffer).
+static int64_t case_bytes(const pf_desc &d) {
+    const PfGeom g = pf_make_geom(d.dim, d.m, d.n, d.l);
+    const int nat = (d.dim == 3) ? 8 : 6;
+    int64_t b = 0;
+    b += nat * align256(8 * g.NS);
+    b += 3 * align256(8 * 2 * g.HS);
+    if (d.precision == PF_PREC_MIXED) b += 3 * align256(4 * 2 * g.HS);
+    b += align256(8 * (int64_t)d.m * d.n * (d.dim == 3 ? d.l : 1));
+    return b;
+}
+
+extern "C" int64_t pf_batch_case_bytes(const pf_desc *d) {
+    std::string why;
+    if (!pf::desc_valid(d, why)) return -1;
+    return case_bytes(*d);
+}
+
+static bool same_kind(const pf_desc &a, const pf_desc &b) {
+    return a.dim == b.dim && a.bc == b.bc && a.precision == b.precision && a.iter_max == b.iter_max &&
+           a.inner_iters == b.inner_iters && a.max_cycles == b.max_cycles && a.residual_tol == b.residual_tol;
+}
+
+static void drop_graphs(pf_batch *b) {
+    for (int p = 0; p < 2; ++p) {
+        if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+        if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+        b->gexec[p] = nullptr;
+        b->graph[p] = nullptr;
+    }
+}
+
+extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena_bytes, pf_batch **out) {
+    if (!out) return PF_EINVAL;
+    *out = nullptr;
+    std::string why;
+    if (!pf::desc_valid(kind, why)) { g_batch_create_error = why; return PF_EINVAL; }
+    if (max_cases < 1 || arena_bytes < 4096) { g_batch_create_error = "max_cases >= 1, arena_bytes >= 4096"; return PF_EINVAL; }
+    pf_batch *b = new (std::nothrow) pf_batch();
+    if (!b) return PF_ENOMEM;
+    b->kind = *kind;
+    b->max_cases = max_cases;
+    b->slots.resize((size_t)max_cases);
+    b->cfg.persistent_blocks = 0;
+    b->cfg.use_graph = 1;
+    b->cfg.inner_iters = kind->inner_iters;
+    b->arena_bytes = align256(arena_bytes);
+    hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->arena, (size_t)b->arena_bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_step, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_active, sizeof(int) * (size_t)max_cases);
+    if (e != hipSuccess) {
+        g_batch_create_error = std::string("batch allocation: ") + hipGetErrorString(e);
+        if (b->arena) (void)hipFree(b->arena);
+        if (b->d_step) (void)hipFree(b->d_step);
+        if (b->d_active) (void)hipFree(b->d_active);
+        if (b->st) (void)hipStreamDestroy(b->st);
+        delete b;
+        return PF_ENOMEM;
+    }
+    b->free_list[0] = b->arena_bytes;
+    *out = b;
+    return PF_OK;
+}
+
+extern "C" int pf_batch_destroy(pf_batch *b) {
+    if (!b) return PF_EINVAL;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    drop_graphs(b);
+    void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_active, b->d_step,
+                    b->d_tile_slots, b->d_res_slots, b->d_err_hist, b->d_res_hist};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return PF_OK;
+}
+
+extern "C" const char *pf_batch_last_error(const pf_batch *b) { return b ? b->msg.c_str() : g_batch_create_error.c_str(); }
+
+static int64_t arena_alloc(pf_batch *b, int64_t bytes) {
+    for (auto it = b->free_list.begin(); it != b->free_list.end(); ++it) {
+        if (it->second >= bytes) {
+            const int64_t off = it->first, rest = it->second - bytes;
+            b->free_list.erase(it);
+            if (rest > 0) b->free_list[off + bytes] = rest;
+            return off;
+        }
+    }
+    return -1;
+}
+
+static void arena_free(pf_batch *b, int64_t off, int64_t bytes) {
+    auto it = b->free_list.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != b->free_list.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        b->free_list.erase(nx);
+    }
+    if (it != b->free_list.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            b->free_list.erase(it);
+        }
+    }
+}
+
+extern "C" int64_t pf_batch_free_bytes(const pf_batch *b) {
+    if (!b) return -1;
+    int64_t best = 0;
+    for (auto &kv : b->free_list) best = std::max(best, kv.second);
+    return best;
+}
+
+static int rebuild_tables(pf_batch *b) {
+    b->order.clear();
+    for (int s = 0; s < b->max_cases; ++s)
+        if (b->slots[(size_t)s].used) b->order.push_back(s);
+    const int nc = (int)b->order.size();
+    std::vector<BatchCase> cs;
+    std::vector<int> np(1, 0), sp(1, 0);
+    b->any_refresh = false;
+    for (int s : b->order) {
+        BatchCase bc = b->slots[(size_t)s].bc;
+        const PfGeom &g = bc.g;
+        bc.nat_gx = (g.m + 2 + 63) / 64;
+        bc.nat_gy = (g.n + 2 + 3) / 4;
+        const pf::SweepShape sh = pf::sweep_shape(g, 1);
+        bc.swp_gx = sh.gx;
+        bc.swp_gy = sh.gy;
+        np.push_back(np.back() + bc.nat_gx * bc.nat_gy * g.Z);
+        sp.push_back(sp.back() + sh.blocks());
+        b->any_refresh = b->any_refresh || bc.ref_y || bc.ref_z;
+        b->slots[(size_t)s].bc = bc;
+        cs.push_back(bc);
+    }
+    b->nat_tiles = np.back();
+    b->swp_tiles = sp.back();
+    if (b->table_cap < nc + 1) {
+        if (b->d_cases) (void)hipFree(b->d_cases);
+        if (b->d_nat_prefix) (void)hipFree(b->d_nat_prefix);
+        if (b->d_swp_prefix) (void)hipFree(b->d_swp_prefix);
+        b->d_cases = nullptr;
+        b->d_nat_prefix = b->d_swp_prefix = nullptr;
+        const int cap = std::max(nc + 1, b->max_cases + 1);
+        PFB_TRY(b, hipMalloc((void **)&b->d_cases, sizeof(BatchCase) * (size_t)cap));
+        PFB_TRY(b, hipMalloc((void **)&b->d_nat_prefix, sizeof(int) * (size_t)cap));
+        PFB_TRY(b, hipMalloc((void **)&b->d_swp_prefix, sizeof(int) * (size_t)cap));
+        b->table_cap = cap;
+    }
+    if (b->tile_cap < b->swp_tiles + 1) {
+        if (b->d_tile_slots) (void)hipFree(b->d_tile_slots);
+        if (b->d_res_slots) (void)hipFree(b->d_res_slots);
+        b->d_tile_slots = b->d_res_slots = nullptr;
+        const int cap = b->swp_tiles + 1024;
+        PFB_TRY(b, hipMalloc((void **)&b->d_tile_slots, sizeof(unsigned long long) * (size_t)cap));
+        PFB_TRY(b, hipMalloc((void **)&b->d_res_slots, sizeof(unsigned long long) * (size_t)cap));
+        PFB_TRY(b, hipMemsetAsync(b->d_tile_slots, 0, sizeof(unsigned long long) * (size_t)cap, b->st));
+        PFB_TRY(b, hipMemsetAsync(b->d_res_slots, 0, sizeof(unsigned long long) * (size_t)cap, b->st));
+        b->tile_cap = cap;
+    }
+    if (nc > 0) {
+        PFB_TRY(b, hipMemcpyAsync(b->d_cases, cs.data(), sizeof(BatchCase) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+        PFB_TRY(b, hipMemcpyAsync(b->d_nat_prefix, np.data(), sizeof(int) * np.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY(b, hipMemcpyAsync(b->d_swp_prefix, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice, b->st));
+        std::vector<int> ones((size_t)nc, 1);
+        PFB_TRY(b, hipMemcpyAsync(b->d_active, ones.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+    }
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    drop_graphs(b);
+    b->dirty = false;
+    return PF_OK;
+}
+
+static pf::BatchView view(const pf_batch *b) {
+    pf::BatchView v;
+    v.cases = b->d_cases;
+    v.nat_prefix = b->d_nat_prefix;
+    v.swp_prefix = b->d_swp_prefix;
+    v.active = b->d_active;
+    v.ncases = (int)b->order.size();
+    v.nat_tiles = b->nat_tiles;
+    v.swp_tiles = b->swp_tiles;
+    return v;
+}
+
+extern "C" int pf_batch_add(pf_batch *b, const pf_desc *d, const double *raw, double threshold, int *slot_out) {
+    if (!b || !d || !raw || !slot_out) return PF_EINVAL;
+    std::string why;
+    if (!pf::desc_valid(d, why)) { b->msg = why; return PF_EINVAL; }
+    if (!same_kind(b->kind, *d)) {
+        b->msg = "case differs from the batch kind (dim, bc, precision, iteration settings)";
+        return PF_EINVAL;
+    }
+    int slot = -1;
+    for (int s = 0; s < b->max_cases; ++s)
+        if (!b->slots[(size_t)s].used) { slot = s; break; }
+    if (slot < 0) { b->msg = "batch is full (max_cases)"; return PF_ENOMEM; }
+    const int64_t bytes = case_bytes(*d);
+    const int64_t off = arena_alloc(b, bytes);
+    if (off < 0) { b->msg = "batch arena has no room for the case"; return PF_ENOMEM; }
+    pf_batch::Slot &S = b->slots[(size_t)slot];
+    S.d = *d;
+    S.off = off;
+    S.bytes = bytes;
+    S.steps = 0;
+    BatchCase &bc = S.bc;
+    bc = BatchCase{};
+    bc.g = pf_make_geom(d->dim, d->m, d->n, d->l);
+    bc.a = pf::make_step_args(*d);
+    bc.slot = slot;
+    const PfGeom &g = bc.g;
+    char *p = b->arena + off;
+    auto takeD = [&](int64_t n) { double *r = (double *)p; p += align256(8 * n); return r; };
+    auto takeF = [&](int64_t n) { float *r = (float *)p; p += align256(4 * n); return r; };
+    bc.U[0] = takeD(g.NS); bc.U[1] = takeD(g.NS);
+    bc.V[0] = takeD(g.NS); bc.V[1] = takeD(g.NS);
+    if (g.dim == 3) { bc.W[0] = takeD(g.NS); bc.W[1] = takeD(g.NS); }
+    bc.phi = takeD(g.NS);
+    bc.div = takeD(g.NS);
+    bc.P = takeD(2 * g.HS);
+    bc.BB = takeD(2 * g.HS);
+    bc.PHIs = takeD(2 * g.HS);
+    if (d->precision == PF_PREC_MIXED) {
+        bc.E = takeF(2 * g.HS);
+        bc.R = takeF(2 * g.HS);
+        bc.PHI32 = takeF(2 * g.HS);
+    }
+    bc.dout = takeD((int64_t)g.m * g.n * g.l);
+    const bool periodic = d->bc == PF_BC_UNIFORM;
+    bc.fuse_y = periodic && (g.n % 2 == 0);
+    bc.fuse_z = periodic && g.dim == 3 && (g.l % 2 == 0);
+    bc.ref_y = periodic && !bc.fuse_y;
+    bc.ref_z = periodic && g.dim == 3 && !bc.fuse_z;
+    PFB_TRY(b, hipMemsetAsync(b->arena + off, 0, (size_t)bytes, b->st));
+    // porosity, its colour-split copies, the fast-division check
+    std::vector<double> phi;
+    pf::host_porosity(g, d->bc, raw, threshold, phi);
+    PFB_TRY(b, hipMemcpyAsync(bc.phi, phi.data(), phi.size() * 8, hipMemcpyHostToDevice, b->st));
+    pf::launch_split(b->st, g, bc.phi, bc.PHIs);
+    if (bc.PHI32) pf::launch_split32(b->st, g, bc.phi, bc.PHI32);
+    int *bad = nullptr, hbad = 1, zero = 0;
+    PFB_TRY(b, hipMalloc((void **)&bad, sizeof(int)));
+    hipError_t e = hipMemcpyAsync(bad, &zero, sizeof(int), hipMemcpyHostToDevice, b->st);
+    if (e == hipSuccess) {
+        pf::launch_faces_check_direct(b->st, g, bc.phi, bc.a, bad);
+        e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, b->st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->st);
+    (void)hipFree(bad);
+    PFB_TRY(b, e);
+    bc.fast = (hbad == 0);
+    // initial_conditions + boundary into the buffers the next step reads as "old"
+    const int q = 1 - b->parity;
+    double u0, v0;
+    pf::initial_values(*d, u0, v0);
+    std::vector<double> buf;
+    pf::fill_interior(g, u0, false, buf);
+    PFB_TRY(b, hipMemcpy(bc.U[q], buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+    pf::fill_interior(g, v0, false, buf);
+    PFB_TRY(b, hipMemcpy(bc.V[q], buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+    if (g.dim == 3) {
+        pf::fill_interior(g, 0.0, false, buf);
+        PFB_TRY(b, hipMemcpy(bc.W[q], buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+    }
+    pf::fill_interior(g, d->outlet_pressure, true, buf);
+    PFB_TRY(b, hipMemcpy(bc.P, buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+    pf::launch_boundary(b->st, g, bc.a, bc.phi, bc.U[q], bc.V[q], bc.W[q], bc.P);
+    PFB_TRY(b, hipGetLastError());
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    S.used = true;
+    b->dirty = true;
+    *slot_out = slot;
+    return PF_OK;
+}
+
+extern "C" int pf_batch_remove(pf_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return PF_EINVAL;
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    pf_batch::Slot &S = b->slots[(size_t)slot];
+    arena_free(b, S.off, S.bytes);
+    S.used = false;
+    b->dirty = true;
+    return PF_OK;
+}
+
+extern "C" int pf_batch_count(const pf_batch *b) {
+    if (!b) return PF_EINVAL;
+    int n = 0;
+    for (auto &s : b->slots) n += s.used ? 1 : 0;
+    return n;
+}
+
+static int grid_blocks(const pf_batch *b, int tiles) {
+    if (tiles <= 0) return 0;
+    const int P = b->cfg.persistent_blocks;
+    return (P > 0 && P < tiles) ? P : tiles;
+}
+
+static void launch_nat(pf_batch *b, int op, int parity, int aux) {
+    const pf::BatchView v = view(b);
+    const int nb = grid_blocks(b, v.nat_tiles);
+    if (nb == 0) return;
+    dim3 blk(64, 4, 1), gr((unsigned)nb);
+#define PFB_NAT(OP) hipLaunchKernelGGL(pf::k_batch_nat<OP>, gr, blk, 0, b->st, v, parity, aux)
+    switch (op) {
+        case pf::NAT_DIV: PFB_NAT(pf::NAT_DIV); break;
+        case pf::NAT_PREDICT: PFB_NAT(pf::NAT_PREDICT); break;
+        case pf::NAT_RHS: PFB_NAT(pf::NAT_RHS); break;
+        case pf::NAT_PROJECT: PFB_NAT(pf::NAT_PROJECT); break;
+        case pf::NAT_BOUNDARY_UNIFORM: PFB_NAT(pf::NAT_BOUNDARY_UNIFORM); break;
+        case pf::NAT_BOUNDARY_AIR: PFB_NAT(pf::NAT_BOUNDARY_AIR); break;
+        case pf::NAT_UPDATE: PFB_NAT(pf::NAT_UPDATE); break;
+        case pf::NAT_REFRESH_P: PFB_NAT(pf::NAT_REFRESH_P); break;
+        default: PFB_NAT(pf::NAT_REFRESH_E); break;
+    }
+#undef PFB_NAT
+}
+
+static void launch_swp(pf_batch *b, int kind, int colour, int track) {
+    const pf::BatchView v = view(b);
+    const int nb = grid_blocks(b, v.swp_tiles);
+    if (nb == 0) return;
+    dim3 blk(64, 4, 1), gr((unsigned)nb);
+    unsigned long long *slots = (kind == pf::SWP_RESIDUAL) ? b->d_res_slots : b->d_tile_slots;
+#define PFB_SWP(D, K) hipLaunchKernelGGL((pf::k_batch_swp<D, K>), gr, blk, 0, b->st, v, colour, track, slots)
+    if (b->kind.dim == 3) {
+        if (kind == pf::SWP_FP64) PFB_SWP(3, pf::SWP_FP64);
+        else if (kind == pf::SWP_FP32) PFB_SWP(3, pf::SWP_FP32);
+        else PFB_SWP(3, pf::SWP_RESIDUAL);
+    } else {
+        if (kind == pf::SWP_FP64) PFB_SWP(2, pf::SWP_FP64);
+        else if (kind == pf::SWP_FP32) PFB_SWP(2, pf::SWP_FP32);
+        else PFB_SWP(2, pf::SWP_RESIDUAL);
+    }
+#undef PFB_SWP
+}
+
+static void launch_reduce(pf_batch *b, unsigned long long *slots, unsigned long long *hist, int row) {
+    const pf::BatchView v = view(b);
+    if (v.ncases == 0) return;
+    hipLaunchKernelGGL(pf::k_batch_reduce, dim3((unsigned)v.ncases), dim3(256), 0, b->st, v, slots, hist, b->d_step,
+                       row, b->max_cases);
+}
+
+// Iterations of the pressure solve (fixed count): FP64 reference sweeps, or FP64 residual +
+// FP32 sweeps + FP64 update cycles.
+static void enqueue_solve_fixed(pf_batch *b) {
+    const int dim = b->kind.dim;
+    const int first = (dim == 3) ? 0 : 1;
+    const int iters = b->kind.iter_max;
+    auto sweeps = [&](int kind, int count) {
+        for (int it = 0; it < count; ++it)
+            for (int half = 0; half < 2; ++half) {
+                const int colour = half == 0 ? first : 1 - first;
+                launch_swp(b, kind, colour, (dim == 2) || (half == 1));
+                if (b->any_refresh) launch_nat(b, kind == pf::SWP_FP64 ? pf::NAT_REFRESH_P : pf::NAT_REFRESH_E, 0, 0);
+            }
+    };
+    if (b->kind.precision == PF_PREC_FP64) {
+        sweeps(pf::SWP_FP64, iters);
+        return;
+    }
+    const int K = b->cfg.inner_iters > 0 ? b->cfg.inner_iters : (iters > 0 ? iters : 1);
+    for (int done = 0; done < iters;) {
+        const int kk = std::min(K, iters - done);
+        launch_swp(b, pf::SWP_RESIDUAL, 0, 0);
+        launch_swp(b, pf::SWP_RESIDUAL, 1, 0);
+        sweeps(pf::SWP_FP32, kk);
+        launch_nat(b, pf::NAT_UPDATE, 0, 0);
+        done += kk;
+    }
+}
+
+static void enqueue_pre(pf_batch *b, int p) {
+    launch_nat(b, pf::NAT_DIV, p, 0);
+    launch_nat(b, pf::NAT_PREDICT, p, 0);
+    launch_nat(b, pf::NAT_RHS, p, 0);
+}
+
+static void enqueue_post(pf_batch *b, int p) {
+    launch_nat(b, pf::NAT_PROJECT, p, 0);
+    if (b->kind.bc == PF_BC_UNIFORM) launch_nat(b, pf::NAT_BOUNDARY_UNIFORM, p, 0);
+    else
+        for (int f = 0; f < 6; ++f) launch_nat(b, pf::NAT_BOUNDARY_AIR, p, f);
+}
+
+static void enqueue_step_fixed(pf_batch *b, int p) {
+    enqueue_pre(b, p);
+    enqueue_solve_fixed(b);
+    launch_reduce(b, b->d_tile_slots, b->d_err_hist, -1);
+    if (b->kind.report_residual && b->kind.precision == PF_PREC_MIXED) {
+        launch_swp(b, pf::SWP_RESIDUAL, 0, 1);
+        launch_swp(b, pf::SWP_RESIDUAL, 1, 1);
+        launch_reduce(b, b->d_res_slots, b->d_res_hist, -1);
+    }
+    enqueue_post(b, p);
+    hipLaunchKernelGGL(pf::k_inc_int, dim3(1), dim3(64), 0, b->st, b->d_step);
+}
+
+static int ensure_hist(pf_batch *b, int rows) {
+    if (b->hist_rows >= rows) return PF_OK;
+    if (b->d_err_hist) (void)hipFree(b->d_err_hist);
+    if (b->d_res_hist) (void)hipFree(b->d_res_hist);
+    b->d_err_hist = b->d_res_hist = nullptr;
+    const size_t n = (size_t)rows * (size_t)b->max_cases;
+    PFB_TRY(b, hipMalloc((void **)&b->d_err_hist, n * sizeof(unsigned long long)));
+    PFB_TRY(b, hipMalloc((void **)&b->d_res_hist, n * sizeof(unsigned long long)));
+    b->hist_rows = rows;
+    drop_graphs(b);   // graphs captured the old history addresses
+    return PF_OK;
+}
+
+static int capture(pf_batch *b, int p) {
+    PFB_TRY(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    enqueue_step_fixed(b, p);
+    hipGraph_t gph = nullptr;
+    PFB_TRY(b, hipStreamEndCapture(b->st, &gph));
+    b->graph[p] = gph;
+    PFB_TRY(b, hipGraphInstantiate(&b->gexec[p], gph, nullptr, nullptr, 0));
+    return PF_OK;
+}
+
+// Tolerance mode: cycles per step with a host check of each case's FP64 residual.
+static int step_tolerance(pf_batch *b, int p, int step_row, std::vector<int> &iters, std::vector<int> &cycles,
+                          std::vector<double> &res) {
+    const int nc = (int)b->order.size();
+    const int dim = b->kind.dim;
+    const int first = (dim == 3) ? 0 : 1;
+    const bool mixed = b->kind.precision == PF_PREC_MIXED;
+    const int K = b->cfg.inner_iters > 0 ? b->cfg.inner_iters : (b->kind.iter_max > 0 ? b->kind.iter_max : 1);
+    const int maxc = b->kind.max_cycles > 0 ? b->kind.max_cycles : 1000;
+    std::vector<int> act((size_t)nc, 1);
+    iters.assign((size_t)nc, 0);
+    cycles.assign((size_t)nc, 0);
+    res.assign((size_t)nc, -1.0);
+    PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+    enqueue_pre(b, p);
+    std::vector<unsigned long long> row((size_t)b->max_cases);
+    for (int cyc = 0; cyc <= maxc; ++cyc) {
+        launch_swp(b, pf::SWP_RESIDUAL, 0, 1);
+        launch_swp(b, pf::SWP_RESIDUAL, 1, 1);
+        launch_reduce(b, b->d_res_slots, b->d_res_hist, step_row);
+        PFB_TRY(b, hipMemcpyAsync(row.data(), b->d_res_hist + (size_t)step_row * b->max_cases,
+                                  sizeof(unsigned long long) * row.size(), hipMemcpyDeviceToHost, b->st));
+        PFB_TRY(b, hipStreamSynchronize(b->st));
+        int n_active = 0;
+        for (int ci = 0; ci < nc; ++ci) {
+            if (!act[(size_t)ci]) continue;
+            double r;
+            std::memcpy(&r, &row[(size_t)b->order[(size_t)ci]], sizeof(double));
+            res[(size_t)ci] = r;
+            if (r <= b->kind.residual_tol || cyc == maxc) act[(size_t)ci] = 0;
+            else ++n_active;
+        }
+        PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+        if (n_active == 0) break;
+        for (int it = 0; it < K; ++it)
+            for (int half = 0; half < 2; ++half) {
+                const int colour = half == 0 ? first : 1 - first;
+                launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1));
+                if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0);
+            }
+        if (mixed) launch_nat(b, pf::NAT_UPDATE, 0, 0);
+        for (int ci = 0; ci < nc; ++ci)
+            if (act[(size_t)ci]) { iters[(size_t)ci] += K; cycles[(size_t)ci] += 1; }
+    }
+    std::fill(act.begin(), act.end(), 1);
+    PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+    launch_reduce(b, b->d_tile_slots, b->d_err_hist, step_row);
+    enqueue_post(b, p);
+    return PF_OK;
+}
+
+extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
+    if (!b || nsteps < 0) return PF_EINVAL;
+    if (nsteps == 0) return PF_OK;
+    int rc;
+    if (b->dirty && (rc = rebuild_tables(b)) != PF_OK) return rc;
+    if (b->order.empty()) return PF_OK;
+    if ((rc = ensure_hist(b, nsteps)) != PF_OK) return rc;
+    const int nc = (int)b->order.size();
+    const bool tol = b->kind.residual_tol > 0;
+    std::vector<std::vector<int>> it_rows, cy_rows;
+    std::vector<std::vector<double>> res_rows;
+    hipLaunchKernelGGL(pf::k_set_int, dim3(1), dim3(64), 0, b->st, b->d_step, 0);
+    for (int s = 0; s < nsteps; ++s) {
+        const int p = b->parity;
+        if (tol) {
+            std::vector<int> its, cys;
+            std::vector<double> rs;
+            if ((rc = step_tolerance(b, p, s, its, cys, rs)) != PF_OK) return rc;
+            it_rows.push_back(its);
+            cy_rows.push_back(cys);
+            res_rows.push_back(rs);
+        } else if (b->cfg.use_graph) {
+            if (!b->gexec[p] && (rc = capture(b, p)) != PF_OK) return rc;
+            PFB_TRY(b, hipGraphLaunch(b->gexec[p], b->st));
+        } else {
+            enqueue_step_fixed(b, p);
+        }
+        PFB_TRY(b, hipGetLastError());
+        b->parity = 1 - p;
+    }
+    std::vector<unsigned long long> eh((size_t)nsteps * b->max_cases), rh((size_t)nsteps * b->max_cases);
+    PFB_TRY(b, hipMemcpyAsync(eh.data(), b->d_err_hist, eh.size() * 8, hipMemcpyDeviceToHost, b->st));
+    if (tol || b->kind.report_residual)
+        PFB_TRY(b, hipMemcpyAsync(rh.data(), b->d_res_hist, rh.size() * 8, hipMemcpyDeviceToHost, b->st));
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    bool finite = true;
+    for (int ci = 0; ci < nc; ++ci) b->slots[(size_t)b->order[(size_t)ci]].steps += nsteps;
+    if (stats) {
+        const int K = b->cfg.inner_iters > 0 ? b->cfg.inner_iters : (b->kind.iter_max > 0 ? b->kind.iter_max : 1);
+        const int fixed_cycles = (b->kind.precision == PF_PREC_MIXED) ? (b->kind.iter_max + K - 1) / K : 0;
+        for (int s = 0; s < nsteps; ++s)
+            for (int ci = 0; ci < nc; ++ci) {
+                const int slot = b->order[(size_t)ci];
+                pf_stats &o = stats[(size_t)s * b->max_cases + slot];
+                double e, r;
+                std::memcpy(&e, &eh[(size_t)s * b->max_cases + slot], 8);
+                std::memcpy(&r, &rh[(size_t)s * b->max_cases + slot], 8);
+                if (!std::isfinite(e)) finite = false;
+                o.sor_error = e;
+                if (tol) {
+                    o.residual = res_rows[(size_t)s][(size_t)ci];
+                    o.sor_iters = it_rows[(size_t)s][(size_t)ci];
+                    o.cycles = cy_rows[(size_t)s][(size_t)ci];
+                } else {
+                    o.residual = (b->kind.report_residual && b->kind.precision == PF_PREC_MIXED) ? r : -1.0;
+                    o.sor_iters = b->kind.iter_max;
+                    o.cycles = fixed_cycles;
+                }
+            }
+    }
+    if (!finite) { b->msg = "non-finite SOR error in a case: it diverged"; return PF_EDIVERGED; }
+    return PF_OK;
+}
+
+extern "C" int pf_batch_download(pf_batch *b, int slot, int field, double *host, int64_t ldi, int64_t ldj) {
+    if (!b || !host || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return PF_EINVAL;
+    const BatchCase &bc = b->slots[(size_t)slot].bc;
+    const PfGeom &g = bc.g;
+    if (ldi < g.m + 2 || ldj < g.n + 2) { b->msg = "leading dimensions smaller than m+2, n+2"; return PF_EINVAL; }
+    const int q = 1 - b->parity;   // the buffers holding the latest velocity
+    const double *src = nullptr;
+    bool split = false;
+    switch (field) {
+        case PF_FIELD_U: src = bc.U[q]; break;
+        case PF_FIELD_V: src = bc.V[q]; break;
+        case PF_FIELD_W: src = (g.dim == 3) ? bc.W[q] : nullptr; break;
+        case PF_FIELD_P: src = bc.P; split = true; break;
+        case PF_FIELD_POROSITY: src = bc.phi; break;
+        default: break;
+    }
+    if (!src) { b->msg = "unknown field"; return PF_EINVAL; }
+    std::vector<double> stage((size_t)(split ? 2 * g.HS : g.NS));
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    PFB_TRY(b, hipMemcpy(stage.data(), src, stage.size() * 8, hipMemcpyDeviceToHost));
+    pf::to_host(g, split, stage, host, ldi, ldj);
+    return PF_OK;
+}
+
+extern "C" int pf_batch_divergence(pf_batch *b, int slot, double *out) {
+    if (!b || !out || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return PF_EINVAL;
+    const BatchCase &bc = b->slots[(size_t)slot].bc;
+    const int q = 1 - b->parity;
+    pf::launch_divergence_out(b->st, bc.g, bc.a, bc.phi, bc.U[q], bc.V[q], bc.W[q], bc.dout);
+    PFB_TRY(b, hipGetLastError());
+    const size_t n = (size_t)bc.g.m * bc.g.n * bc.g.l;
+    PFB_TRY(b, hipMemcpyAsync(out, bc.dout, n * 8, hipMemcpyDeviceToHost, b->st));
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    return PF_OK;
+}
+
+extern "C" int pf_batch_set_config(pf_batch *b, const pf_batch_config *cfg) {
+    if (!b || !cfg || cfg->persistent_blocks < 0 || cfg->inner_iters < 0) return PF_EINVAL;
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    b->cfg = *cfg;
+    drop_graphs(b);
+    return PF_OK;
+}
+
+extern "C" int pf_batch_get_config(const pf_batch *b, pf_batch_config *cfg) {
+    if (!b || !cfg) return PF_EINVAL;
+    *cfg = b->cfg;
+    return PF_OK;
+}
+
+extern "C" int pf_batch_sync(pf_batch *b) {
+    if (!b) return PF_EINVAL;
+    PFB_TRY(b, hipStreamSynchronize(b->st));
+    return PF_OK;
+}
+
+extern "C" int64_t pf_batch_case_steps(const pf_batch *b, int slot) {
+    if (!b || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return -1;
+    return b->slots[(size_t)slot].steps;
+}
+
+extern "C" int pf_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
+    size_t f = 0, t = 0;
+    if (!free_bytes || !total_bytes) return PF_EINVAL;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return PF_EHIP;
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return PF_OK;
+}

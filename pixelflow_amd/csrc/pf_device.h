@@ -114,4 +114,72 @@ __device__ __forceinline__ double face_any(double pa, double pb, double dt, doub
     return FAST ? face_coef_fast(pa, pb, dt, d, rd) : face_coef(pa, pb, dt, d);
 }
 
+// ------------------------------------------------------------------------------------
+// Thread geometry of the colour sweeps.  Block (64, 4): x runs over the colour's slots of
+// a row.  3D: y = (row offset 0..1, plane parity 0..1); a block covers 2 rows and 2*KB
+// planes and each thread updates KB cells of its colour two planes apart (same slot, same
+// x position; independent updates whose loads overlap in flight).  2D: y = 4 rows, KB = 1.
+// Blocks form a 1D grid renumbered XCD-contiguously (xcd_logical_block).
+// ------------------------------------------------------------------------------------
+struct SweepShape {
+    int gx, gy, gz;
+    PF_HD int blocks() const { return gx * gy * gz; }
+};
+
+PF_HD SweepShape sweep_shape(const PfGeom &g, int KB) {
+    SweepShape t;
+    t.gx = ((g.m + 1) / 2 + 63) / 64;
+    if (g.dim == 3) {
+        t.gy = (g.n + 1) / 2;
+        t.gz = (g.l + 2 * KB - 1) / (2 * KB);
+    } else {
+        t.gy = (g.n + 3) / 4;
+        t.gz = 1;
+    }
+    return t;
+}
+
+struct SweepThread {
+    int j, kbase, sh, h, i;
+};
+
+template <int DIM, int KB>
+__device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const SweepArgs &s) {
+    const int nb = (int)gridDim.x;
+    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
+    const int bx = L % s.gx, byz = L / s.gx;
+    const int by = byz % s.gy, bz = byz / s.gy;
+    SweepThread t;
+    if (DIM == 3) {
+        t.j = by * 2 + (int)(threadIdx.y >> 1) + 1;
+        t.kbase = bz * 2 * KB + 1 + (int)(threadIdx.y & 1);
+    } else {
+        t.j = by * 4 + (int)threadIdx.y + 1;
+        t.kbase = 0;
+    }
+    t.sh = (s.colour + t.j + t.kbase) & 1;   // the same for kbase + 2q
+    t.h = (int)(bx * 64 + threadIdx.x) + 1 - t.sh;
+    t.i = 2 * t.h + t.sh;
+    return t;
+}
+
+__device__ __forceinline__ void block_max_to_slot(double v, unsigned long long *slots) {
+    unsigned long long e = __double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o2 = __shfl_xor(e, off, 64);
+        e = (o2 > e) ? o2 : e;
+    }
+    __shared__ unsigned long long red[4];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) red[lane >> 6] = e;
+    __syncthreads();
+    if (lane == 0) {
+        unsigned long long b = red[0];
+        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+        for (int q = 1; q < nw; ++q) b = (red[q] > b) ? red[q] : b;
+        unsigned long long *slot = slots + blockIdx.x;
+        if (b > *slot) *slot = b;
+    }
+}
+
 }  // namespace pf

@@ -29,8 +29,9 @@ struct SweepArgs {
     int colour;         // cells with (i+j+k) & 1 == colour are updated
     int fuse_y, fuse_z; // write the periodic ghost copies in the sweep (even extents)
     int track_err;      // reduce max |dp| into the step's error slot
-    int xcd_remap;      // renumber blocks XCD-contiguously (k_sor_phi)
+    int xcd_remap;      // renumber blocks XCD-contiguously
     int gx, gy;         // block grid extents of the sweep (set by the launcher)
+    int kb;             // 3D: cells per thread, two planes apart (1, 2 or 4)
 };
 
 void launch_faces(hipStream_t st, const PfGeom &g, const double *phi, double *AW, double *AS, double *AB,
@@ -40,14 +41,14 @@ void launch_div(hipStream_t st, const PfGeom &g, const StepArgs &a, const double
 void launch_predict(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *uo, const double *vo,
                     const double *wo, const double *div, const double *phi, double *u, double *v, double *w);
 void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                const double *w, const double *phi, const double *AW, const double *AS, const double *AB,
-                const double *P, double *BB);
+                const double *w, const double *phi, const double *P, double *BB);
 void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                 const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
                 unsigned long long *err_slot);
 void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                     const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
                     bool fast);
+void launch_faces_check_direct(hipStream_t st, const PfGeom &g, const double *phi, const StepArgs &a, int *bad);
 void launch_faces_check(hipStream_t st, const PfGeom &g, const double *phi, const double *AW, const double *AS,
                         const double *AB, const StepArgs &a, int *bad);
 void launch_split(hipStream_t st, const PfGeom &g, const double *src, double *dst);

@@ -6,6 +6,7 @@
 // HBM-bandwidth bound stencils: one thread per cell, x-contiguous wave rows, no MFMA.
 #include <hip/hip_runtime.h>
 
+#include "pf_cells.h"
 #include "pf_device.h"
 #include "pf_kernels.h"
 
@@ -37,42 +38,15 @@ __global__ void k_faces(PfGeom g, const double *__restrict__ phi, double *__rest
 // Ghost values are recomputed from the wrapped interior cell rather than copied, so one
 // launch produces interior and ghosts with no intra-kernel dependency.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ double div3_at(const PfGeom &g, const StepArgs &a, const double *uo,
-                                          const double *vo, const double *wo, int i, int j, int k) {
-    const int64_t o = pf_off(g, i, j, k);
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    return ((((uo[o + 1] - uo[o - 1]) / a.dx) * 0.5) + (((vo[o + sy] - vo[o - sy]) / a.dy) * 0.5)) +
-           (((wo[o + sz] - wo[o - sz]) / a.dz) * 0.5);
-}
-__device__ __forceinline__ double div2_at(const PfGeom &g, const StepArgs &a, const double *uo,
-                                          const double *vo, int i, int j) {
-    const int64_t o = pf_off(g, i, j, 0);
-    // the 2D code divides the v-difference by dx (ibm_2d_uniform_omp_cpu.f90:175-176)
-    return (((uo[o + 1] - uo[o - 1]) / a.dx) * .5) + (((vo[o + g.PX] - vo[o - g.PX]) / a.dx) * .5);
-}
+
 
 __global__ void k_div(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
                       const double *__restrict__ wo, double *__restrict__ div) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = (g.dim == 3) ? (int)blockIdx.z : 0;
-    const int m = g.m, n = g.n, l = g.l;
-    if (i > m + 1 || j > n + 1) return;
-    const bool gi = (i == 0 || i == m + 1), gj = (j == 0 || j == n + 1);
-    const bool gk = (g.dim == 3) && (k == 0 || k == l + 1);
-    const int ng = (int)gi + (int)gj + (int)gk;
-    if (ng == 0) {
-        div[pf_off(g, i, j, k)] = (g.dim == 3) ? div3_at(g, a, uo, vo, wo, i, j, k) : div2_at(g, a, uo, vo, i, j);
-        return;
-    }
-    if (ng > 1) return;  // edges and corners are never written nor read
-    double val = 0.;
-    if (a.bc == PF_BC_UNIFORM && !gi) {  // y / z periodic ghosts; x ghosts are 0
-        const int jj = (j == 0) ? n : (j == n + 1 ? 1 : j);
-        const int kk = (g.dim == 3) ? ((k == 0) ? l : (k == l + 1 ? 1 : k)) : 0;
-        val = (g.dim == 3) ? div3_at(g, a, uo, vo, wo, i, jj, kk) : div2_at(g, a, uo, vo, i, jj);
-    }
-    div[pf_off(g, i, j, k)] = val;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    div_cell(g, a, uo, vo, wo, div, i, j, k);
 }
 
 // ------------------------------------------------------------------------------------
@@ -80,6 +54,8 @@ __global__ void k_div(PfGeom g, StepArgs a, const double *__restrict__ uo, const
 // new velocity: interior cells predicted, ghost cells copied from the old velocity (the
 // reference predicts the interior of u after u_old = u, so the ghosts keep u_old's).
 // ------------------------------------------------------------------------------------
+
+
 __global__ void k_predict3(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
                            const double *__restrict__ wo, const double *__restrict__ div,
                            const double *__restrict__ po, double *__restrict__ u, double *__restrict__ v,
@@ -88,79 +64,10 @@ __global__ void k_predict3(PfGeom g, StepArgs a, const double *__restrict__ uo, 
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i > g.m + 1 || j > g.n + 1) return;
-    const int64_t o = pf_off(g, i, j, k);
-    if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1 || k == 0 || k == g.l + 1) {
-        u[o] = uo[o]; v[o] = vo[o]; w[o] = wo[o];
-        return;
-    }
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    const double dt = a.dt, dx = a.dx, dy = a.dy, dz = a.dz, xnue = a.xnue, xlambda = a.xlambda;
-    const double pc = po[o];
-    const double pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy], pT = po[o + sz], pB = po[o - sz];
-    const double dc = div[o];
-    const double uc = uo[o], vc = vo[o], wc = wo[o];
-    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy], uT = uo[o + sz], uB = uo[o - sz];
-    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy], vT = vo[o + sz], vB = vo[o - sz];
-    const double wE = wo[o + 1], wW = wo[o - 1], wN = wo[o + sy], wS = wo[o - sy], wT = wo[o + sz], wB = wo[o - sz];
-    const double wall = PF_ALPHA;
-    {   // u (:233-271)
-        double r = uc - ((((dt * uc) * (uE - uW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (uN - uS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (uT - uB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((uT - (2. * uc)) + uB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * 0.5);
-        const double t1 = (((((((uE - uW) / dx) * 0.5) + (((uE - uW) / dx) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((uN - uS) / dy) * 0.5) + (((vE - vW) / dx) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((uT - uB) / dz) * 0.5) + (((wE - wW) / dx) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * uc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        u[o] = r;
-    }
-    {   // v (:286-323)
-        double r = vc - ((((dt * uc) * (vE - vW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (vN - vS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (vT - vB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((vT - (2. * vc)) + vB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * 0.5);
-        const double t1 = (((((((vE - vW) / dx) * 0.5) + (((uN - uS) / dy) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((vT - vB) / dz) * .5) + (((wN - wS) / dy) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dy;
-            r = r - (((((((dt * xnue) * vc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        v[o] = r;
-    }
-    {   // w (:339-376)
-        double r = wc - ((((dt * uc) * (wE - wW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (wN - wS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (wT - wB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((wE - (2. * wc)) + wW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((wN - (2. * wc)) + wS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((wT - (2. * wc)) + wB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sz] - div[o - sz])) / dz) * 0.5);
-        const double t1 = (((((((wE - wW) / dx) * 0.5) + (((uT - uB) / dz) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((wN - wS) / dy) * 0.5) + (((vT - vB) / dz) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((wT - wB) / dz) * 0.5) + (((wT - wB) / dz) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pT - pB)) / dz) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dz;
-            r = r - (((((((dt * xnue) * wc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        w[o] = r;
-    }
+    predict3_cell(g, a, uo, vo, wo, div, po, u, v, w, i, j, k);
 }
+
+
 
 __global__ void k_predict2(PfGeom g, StepArgs a, const double *__restrict__ uo, const double *__restrict__ vo,
                            const double *__restrict__ div, const double *__restrict__ po,
@@ -168,97 +75,23 @@ __global__ void k_predict2(PfGeom g, StepArgs a, const double *__restrict__ uo, 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i > g.m + 1 || j > g.n + 1) return;
-    const int64_t o = pf_off(g, i, j, 0);
-    if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1) {
-        u[o] = uo[o]; v[o] = vo[o];
-        return;
-    }
-    const int64_t sy = g.PX;
-    const double dt = a.dt, dx = a.dx, dy = a.dy, xnue = a.xnue, xlambda = a.xlambda;
-    const double pc = po[o], pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy];
-    const double dc = div[o];
-    const double uc = uo[o], vc = vo[o];
-    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy];
-    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy];
-    {   // u (ibm_2d_uniform_omp_cpu.f90:204-224)
-        double r = uc - (dt * (((uc * (uE - uW)) / dx) / 2.));
-        r = r - (dt * (((vc * (uN - uS)) / dy) / 2.));
-        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * .5);
-        const double t1 = (((((((uE - uW) / dx) * .5) + (((uE - uW) / dx) * .5)) * xnue) * (pE - pW)) / dx) * .5;
-        const double t2 = (((((((uN - uS) / dy) * .5) + (((vE - vW) / dx) * .5)) * xnue) * (pN - pS)) / dy) * .5;
-        const double t3 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
-        r = r + ((dt * ((t1 + t2) + t3)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * uc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
-        }
-        u[o] = r;
-    }
-    {   // v (:235-255); the wall force uses dx here as well (:254)
-        double r = vc - (dt * (((uc * (vE - vW)) / dx) / 2.));
-        r = r - (dt * (((vc * (vN - vS)) / dy) / 2.));
-        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * .5);
-        const double t1 = (((((((vE - vW) / dx) * .5) + (((uN - uS) / dy) * .5)) * xnue) * (pE - pW)) / dx) * .5;
-        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * .5)) * xnue) * (pN - pS)) / dy) * .5;
-        const double t3 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
-        r = r + ((dt * ((t1 + t2) + t3)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * vc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
-        }
-        v[o] = r;
-    }
+    predict2_cell(g, a, uo, vo, div, po, u, v, i, j);
 }
 
 // ------------------------------------------------------------------------------------
 // RHS bb of the predicted velocity (3D :404-409, 2D :271-274) with the outlet terms of
 // boundary_matrix, written in the colour-split layout.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                                         const double *w, const double *po, int i, int j, int k) {
-    const int64_t o = pf_off(g, i, j, k);
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    const double pc = po[o], density = a.density;
-    return (((((((po[o + 1] * u[o]) + (pc * u[o + 1])) * 0.5) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) * 0.5)) * density) / a.dx) +
-            ((((((po[o + sy] * v[o]) + (pc * v[o + sy])) * 0.5) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) * 0.5)) * density) / a.dy)) +
-           ((((((po[o + sz] * w[o]) + (pc * w[o + sz])) * 0.5) - (((po[o - sz] * w[o]) + (pc * w[o - sz])) * 0.5)) * density) / a.dz);
-}
-__device__ __forceinline__ double bb2_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                                         const double *po, int i, int j) {
-    const int64_t o = pf_off(g, i, j, 0);
-    const int64_t sy = g.PX;
-    const double pc = po[o], density = a.density;
-    return ((((((po[o + 1] * u[o]) + (pc * u[o + 1])) * 0.5) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) * 0.5)) * density) / a.dx) +
-           ((((((po[o + sy] * v[o]) + (pc * v[o + sy])) * 0.5) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) * 0.5)) * density) / a.dy);
-}
+
 
 __global__ void k_rhs(PfGeom g, StepArgs a, const double *__restrict__ u, const double *__restrict__ v,
-                      const double *__restrict__ w, const double *__restrict__ po, const double *__restrict__ AW,
-                      const double *__restrict__ AS, const double *__restrict__ AB, const double *__restrict__ P,
+                      const double *__restrict__ w, const double *__restrict__ po, const double *__restrict__ P,
                       double *__restrict__ BB) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
     if (i > g.m || j > g.n) return;
-    double bb = (g.dim == 3) ? bb3_at(g, a, u, v, w, po, i, j, k) : bb2_at(g, a, u, v, po, i, j);
-    if (is_edge_row(g, a, i, j, k)) {
-        Coef c;
-        const int64_t so = pf_soff(g, i, j, k);
-        c.aw = AW[so];
-        c.ae = AW[pf_soff(g, i + 1, j, k)];
-        c.as = AS[so];
-        c.an = AS[pf_soff(g, i, j + 1, k)];
-        c.ab = (g.dim == 3) ? AB[so] : 0.;
-        c.at = (g.dim == 3) ? AB[pf_soff(g, i, j, k + 1)] : 0.;
-        double bbk1 = 0.;
-        if (a.bc == PF_BC_AIR && k == g.l && a.face[0] == 2) bbk1 = bb3_at(g, a, u, v, w, po, i, j, 1);
-        row_rules<true>(g, a, i, j, k, c, &bb, P, po, bbk1);
-    }
-    BB[pf_soff(g, i, j, k)] = bb;
+    rhs_cell(g, a, u, v, w, po, P, BB, i, j, k);
 }
 
 // ------------------------------------------------------------------------------------
@@ -377,88 +210,26 @@ __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned lon
 
 // Variant of the sweep that rebuilds the six face coefficients of each updated cell from
 // porosity (helpers in pf_device.h).
-template <int DIM, bool FAST>
+// One cell of the sweep (FP64, faces rebuilt from porosity); returns |p_new - p_old|.
+
+template <int DIM, bool FAST, int KB>
 __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
                                                  double *__restrict__ Pown, const double *__restrict__ PHIo,
                                                  const double *__restrict__ PHIx, const double *__restrict__ BBo,
                                                  const double *__restrict__ phi, unsigned long long *err_slot) {
-    // 1D grid of gx*gy*gz blocks (s.gx, s.gy), renumbered XCD-contiguously when s.xcd_remap
-    const int nb = (int)gridDim.x;
-    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
-    const int bx = L % s.gx, byz = L / s.gx;
-    const int by = byz % s.gy, bz = byz / s.gy;
-    const int j = by * blockDim.y + threadIdx.y + 1;
-    const int k = (DIM == 3) ? bz + 1 : 0;
-    const int sh = (s.colour + j + k) & 1;
-    const int h = (int)(bx * blockDim.x + threadIdx.x) + 1 - sh;
-    const int i = 2 * h + sh;
-    const bool active = (j <= g.n) && (i <= g.m);
+    const SweepThread t = sweep_thread<DIM, KB>(g, s);
     double err = 0.;
-    if (active) {
-        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-        const int64_t o = (int64_t)h + HP * ((int64_t)j + (int64_t)g.Y * k);
-        const double pc = Pown[o];
-        const double pW = Poth[o - 1 + sh], pE = Poth[o + sh];
-        const double pS = Poth[o - HP], pN = Poth[o + HP];
-        const double fc = PHIo[o];
-        const double fW = PHIx[o - 1 + sh], fE = PHIx[o + sh];
-        const double fS = PHIx[o - HP], fN = PHIx[o + HP];
-        const double dt = a.dt;
-        Coef c;
-        c.ae = face_any<FAST>(fE, fc, dt, a.dx, a.rdx);
-        c.aw = face_any<FAST>(fc, fW, dt, a.dx, a.rdx);
-        c.an = face_any<FAST>(fN, fc, dt, a.dy, a.rdy);
-        c.as = face_any<FAST>(fc, fS, dt, a.dy, a.rdy);
-        double pB = 0., pT = 0.;
-        if (DIM == 3) {
-            pB = Poth[o - HPY];
-            pT = Poth[o + HPY];
-            const double fB = PHIx[o - HPY], fT = PHIx[o + HPY];
-            c.at = face_any<FAST>(fT, fc, dt, a.dz, a.rdz);
-            c.ab = face_any<FAST>(fc, fB, dt, a.dz, a.rdz);
-        } else {
-            c.at = 0.;
-            c.ab = 0.;
+    if (t.j <= g.n && t.i <= g.m) {
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const int k = t.kbase + 2 * q;
+            if (DIM == 3 && k > g.l) break;
+            const int64_t o = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
+            const double e = sor_phi_cell<DIM, FAST>(g, a, s, t.i, t.j, k, t.sh, o, Poth, Pown, PHIo, PHIx, BBo, phi);
+            err = (e > err) ? e : err;
         }
-        double ap;
-        if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
-        else ap = (((-c.ae) - c.aw) - c.an) - c.as;
-        if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
-        const double bb = BBo[o];
-        double num;
-        if (DIM == 3)
-            num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
-        else
-            num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
-        const double pn = ((num / ap) * a.omega) + (pc * a.omega_c);
-        Pown[o] = pn;
-        if (s.fuse_y) {
-            if (j == g.n) Pown[o - HP * (int64_t)g.n] = pn;
-            if (j == 1) Pown[o + HP * (int64_t)g.n] = pn;
-        }
-        if (DIM == 3 && s.fuse_z) {
-            if (k == g.l) Pown[o - HPY * (int64_t)g.l] = pn;
-            if (k == 1) Pown[o + HPY * (int64_t)g.l] = pn;
-        }
-        err = fabs(pn - pc);
     }
-    if (!s.track_err) return;
-    unsigned long long e = __double_as_longlong(err);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o2 = __shfl_xor(e, off, 64);
-        e = (o2 > e) ? o2 : e;
-    }
-    __shared__ unsigned long long red[4];
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((lane & 63) == 0) red[lane >> 6] = e;
-    __syncthreads();
-    if (lane == 0) {
-        unsigned long long b = red[0];
-        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-        for (int q = 1; q < nw; ++q) b = (red[q] > b) ? red[q] : b;
-        unsigned long long *slot = err_slot + blockIdx.x;
-        if (b > *slot) *slot = b;
-    }
+    if (s.track_err) block_max_to_slot(err, err_slot);
 }
 
 // Porosity (natural layout, ghosts included) -> colour-split copy.
@@ -490,6 +261,26 @@ __global__ void k_faces_check(PfGeom g, const double *__restrict__ phi, const do
     if (!ok) atomicOr(bad, 1);
 }
 
+// The same check without stored faces: every face of the case evaluated both ways.
+__global__ void k_faces_check_direct(PfGeom g, const double *__restrict__ phi, StepArgs a, int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m + 1 || j > g.n + 1) return;
+    const int64_t o = pf_off(g, i, j, k);
+    const double pc = phi[o];
+    bool ok = __double_as_longlong(face_coef_fast(pc, phi[o - 1], a.dt, a.dx, a.rdx)) ==
+              __double_as_longlong(face_coef(pc, phi[o - 1], a.dt, a.dx));
+    ok = ok && __double_as_longlong(face_coef_fast(pc, phi[o - g.PX], a.dt, a.dy, a.rdy)) ==
+                   __double_as_longlong(face_coef(pc, phi[o - g.PX], a.dt, a.dy));
+    if (g.dim == 3) {
+        const double pb = phi[o - (int64_t)g.PX * g.Y];
+        ok = ok && __double_as_longlong(face_coef_fast(pc, pb, a.dt, a.dz, a.rdz)) ==
+                       __double_as_longlong(face_coef(pc, pb, a.dt, a.dz));
+    }
+    if (!ok) atomicOr(bad, 1);
+}
+
 // Periodic ghost refresh of p for odd extents (3D :463-480 / :520-537 / :588-605,
 // 2D :322-327 / :356-361 / :392-397): y ghosts for i in 1..m, k in 1..l; z ghosts for
 // i in 1..m, j in 1..n.
@@ -511,18 +302,14 @@ __global__ void k_ghost_refresh(PfGeom g, int do_y, int do_z, double *__restrict
 // ------------------------------------------------------------------------------------
 // Projection (3D main :115-123, 2D main :108-113).
 // ------------------------------------------------------------------------------------
+
 __global__ void k_project(PfGeom g, StepArgs a, const double *__restrict__ P, double *__restrict__ u,
                           double *__restrict__ v, double *__restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
     if (i > g.m || j > g.n) return;
-    const int64_t o = pf_off(g, i, j, k);
-    const double f = a.dt / a.density;
-    u[o] = u[o] - (((f * (P[pf_soff(g, i + 1, j, k)] - P[pf_soff(g, i - 1, j, k)])) / a.dx) * 0.5);
-    v[o] = v[o] - (((f * (P[pf_soff(g, i, j + 1, k)] - P[pf_soff(g, i, j - 1, k)])) / a.dy) * 0.5);
-    if (g.dim == 3)
-        w[o] = w[o] - (((f * (P[pf_soff(g, i, j, k + 1)] - P[pf_soff(g, i, j, k - 1)])) / a.dz) * 0.5);
+    project_cell(g, a, P, u, v, w, i, j, k);
 }
 
 // ------------------------------------------------------------------------------------
@@ -533,6 +320,10 @@ __global__ void k_project(PfGeom g, StepArgs a, const double *__restrict__ P, do
 // each thread evaluates that directly, so one launch has no intra-kernel dependency.
 // The launch covers the x planes i in {0,1,m+1}, the y planes and the z planes.
 // ------------------------------------------------------------------------------------
+// Final value of one boundary cell of the uniform-inflow solvers (i in {0,1,m+1}, or j, k
+// a ghost index).
+
+
 __global__ void k_boundary_uniform(PfGeom g, StepArgs a, double *__restrict__ u, double *__restrict__ v,
                                    double *__restrict__ w, double *__restrict__ P) {
     const int m = g.m, n = g.n, l = g.l;
@@ -564,186 +355,31 @@ __global__ void k_boundary_uniform(PfGeom g, StepArgs a, double *__restrict__ u,
         return;
     }
     if (g.dim == 2) k = 0;
-    // an i = 1 cell with interior (j, k) only takes the inlet velocity (its p is untouched)
-    const int jj = (j == 0) ? n : (j == n + 1 ? 1 : j);
-    const int kk = (g.dim == 3) ? ((k == 0) ? l : (k == l + 1 ? 1 : k)) : 0;
-    double uu, vv, ww, pp;
-    if (i == 0 || i == 1) {
-        uu = a.u_in; vv = a.v_in; ww = 0.;
-        pp = (i == 0) ? P[pf_soff(g, 2, jj, kk)] : P[pf_soff(g, 1, jj, kk)];
-    } else if (i == m + 1) {
-        if (m - 1 == 1) { uu = a.u_in; vv = a.v_in; ww = 0.; }
-        else {
-            const int64_t s = pf_off(g, m - 1, jj, kk);
-            uu = u[s]; vv = v[s]; ww = (g.dim == 3) ? w[s] : 0.;
-        }
-        pp = a.outlet_pressure;
-    } else {
-        const int64_t s = pf_off(g, i, jj, kk);
-        uu = u[s]; vv = v[s]; ww = (g.dim == 3) ? w[s] : 0.;
-        pp = P[pf_soff(g, i, jj, kk)];
-    }
-    const int64_t o = pf_off(g, i, j, k);
-    u[o] = uu;
-    v[o] = vv;
-    if (g.dim == 3) w[o] = ww;
-    P[pf_soff(g, i, j, k)] = pp;
+    boundary_uniform_cell(g, a, u, v, w, P, i, j, k);
 }
 
 // boundary of the air-conditioning solver, one face per launch in the reference's order
 // top, bottom, west, east, north, south (ibm_3d_air_condition_omp_cpu.f90:890-1167);
 // each launch covers the face's full index range, ghost rows included.
+
 __global__ void k_boundary_air(PfGeom g, StepArgs a, int face, const double *__restrict__ po, double *__restrict__ u,
                                double *__restrict__ v, double *__restrict__ w, double *__restrict__ P) {
-    const int m = g.m, n = g.n, l = g.l;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y * blockDim.y + threadIdx.y;
-    const double U = a.inlet_velocity, pout = a.outlet_pressure;
-#define OFF(ii, jj, kk) pf_off(g, ii, jj, kk)
-#define SOF(ii, jj, kk) pf_soff(g, ii, jj, kk)
-    if (face == 0 || face == 1) {          // top (k = l) / bottom (k = 1): r = i, q = j
-        const int i = r, j = q;
-        if (i > m + 1 || j > n + 1) return;
-        if (face == 0) {
-            const int code = a.face[0];
-            const bool wall = (code == 0) || (po[OFF(i, j, l)] < 0.9);
-            if (wall) {
-                u[OFF(i, j, l)] = 0.; v[OFF(i, j, l)] = 0.; w[OFF(i, j, l)] = 0.;
-                w[OFF(i, j, l + 1)] = -w[OFF(i, j, l - 1)];
-                P[SOF(i, j, l + 1)] = P[SOF(i, j, l - 1)];
-            } else if (code == 1) {
-                u[OFF(i, j, l)] = 0.; v[OFF(i, j, l)] = 0.; w[OFF(i, j, l)] = -U;
-                u[OFF(i, j, l + 1)] = 0.; v[OFF(i, j, l + 1)] = 0.; w[OFF(i, j, l + 1)] = -U;
-                P[SOF(i, j, l + 1)] = P[SOF(i, j, l - 1)];
-            } else {
-                u[OFF(i, j, l + 1)] = u[OFF(i, j, l - 1)];
-                v[OFF(i, j, l + 1)] = v[OFF(i, j, l - 1)];
-                w[OFF(i, j, l + 1)] = w[OFF(i, j, l - 1)];
-                P[SOF(i, j, l + 1)] = pout;
-            }
-        } else {
-            const int code = a.face[1];
-            // the inlet branch tests porosity(i,j,l) as written at :948
-            const bool wall = (code == 0) || (code == 1 && po[OFF(i, j, l)] < 0.9) ||
-                              (code == 2 && po[OFF(i, j, 1)] < 0.9);
-            if (wall) {
-                u[OFF(i, j, 1)] = 0.; v[OFF(i, j, 1)] = 0.; w[OFF(i, j, 1)] = 0.;
-                w[OFF(i, j, 0)] = -w[OFF(i, j, 2)];
-                P[SOF(i, j, 0)] = P[SOF(i, j, 2)];
-            } else if (code == 1) {
-                u[OFF(i, j, 1)] = 0.; v[OFF(i, j, 1)] = 0.; w[OFF(i, j, 1)] = U;
-                u[OFF(i, j, 0)] = 0.; v[OFF(i, j, 0)] = 0.; w[OFF(i, j, 0)] = U;
-                P[SOF(i, j, 0)] = P[SOF(i, j, 2)];
-            } else {
-                u[OFF(i, j, 0)] = u[OFF(i, j, 1)];
-                v[OFF(i, j, 0)] = v[OFF(i, j, 1)];
-                w[OFF(i, j, 0)] = w[OFF(i, j, 1)];
-                P[SOF(i, j, 0)] = pout;
-            }
-        }
-        return;
-    }
-    if (face == 2 || face == 3) {          // west (i = 1) / east (i = m): r = j, q = k
-        const int j = r, k = q;
-        if (j > n + 1 || k > l + 1) return;
-        if (face == 2) {
-            const int code = a.face[3];
-            const bool wall = (code == 0) || (po[OFF(1, j, k)] < 0.9);
-            if (wall) {
-                u[OFF(1, j, k)] = 0.; v[OFF(1, j, k)] = 0.; w[OFF(1, j, k)] = 0.;
-                u[OFF(0, j, k)] = -u[OFF(2, j, k)];
-                P[SOF(0, j, k)] = P[SOF(2, j, k)];
-            } else if (code == 1) {
-                u[OFF(1, j, k)] = U; v[OFF(1, j, k)] = 0.; w[OFF(1, j, k)] = 0.;
-                u[OFF(0, j, k)] = U; v[OFF(0, j, k)] = 0.; w[OFF(0, j, k)] = 0.;
-                P[SOF(0, j, k)] = P[SOF(2, j, k)];
-            } else {
-                u[OFF(0, j, k)] = u[OFF(1, j, k)];
-                v[OFF(0, j, k)] = v[OFF(1, j, k)];
-                w[OFF(0, j, k)] = w[OFF(1, j, k)];
-                P[SOF(0, j, k)] = pout;
-            }
-        } else {
-            const int code = a.face[2];
-            const bool wall = (code == 0) || (po[OFF(m, j, k)] < 0.9);
-            if (wall) {
-                u[OFF(m, j, k)] = 0.; v[OFF(m, j, k)] = 0.; w[OFF(m, j, k)] = 0.;
-                u[OFF(m + 1, j, k)] = -u[OFF(m - 1, j, k)];
-                P[SOF(m + 1, j, k)] = P[SOF(m - 1, j, k)];
-            } else if (code == 1) {
-                u[OFF(m, j, k)] = -U; v[OFF(m, j, k)] = 0.; w[OFF(m, j, k)] = 0.;
-                u[OFF(m + 1, j, k)] = -U; v[OFF(m + 1, j, k)] = 0.; w[OFF(m + 1, j, k)] = 0.;
-                P[SOF(m + 1, j, k)] = P[SOF(m - 1, j, k)];
-            } else {
-                u[OFF(m + 1, j, k)] = u[OFF(m, j, k)];
-                v[OFF(m + 1, j, k)] = v[OFF(m, j, k)];
-                w[OFF(m + 1, j, k)] = w[OFF(m, j, k)];
-                P[SOF(m + 1, j, k)] = pout;
-            }
-        }
-        return;
-    }
-    {                                      // north (j = n) / south (j = 1): r = i, q = k
-        const int i = r, k = q;
-        if (i > m + 1 || k > l + 1) return;
-        if (face == 4) {
-            const int code = a.face[4];
-            const bool wall = (code == 0) || (po[OFF(i, n, k)] < 0.9);
-            if (wall) {
-                u[OFF(i, n, k)] = 0.; v[OFF(i, n, k)] = 0.; w[OFF(i, n, k)] = 0.;
-                u[OFF(i, n + 1, k)] = -u[OFF(i, n - 1, k)];   // u, as written at :1087
-                P[SOF(i, n + 1, k)] = P[SOF(i, n - 1, k)];
-            } else if (code == 1) {
-                u[OFF(i, n, k)] = -U; v[OFF(i, n, k)] = 0.; w[OFF(i, n, k)] = 0.;
-                u[OFF(i, n + 1, k)] = -U; v[OFF(i, n + 1, k)] = 0.; w[OFF(i, n + 1, k)] = 0.;
-                P[SOF(i, n + 1, k)] = P[SOF(i, n - 1, k)];
-            } else {
-                u[OFF(i, n + 1, k)] = u[OFF(i, n, k)];
-                v[OFF(i, n + 1, k)] = v[OFF(i, n, k)];
-                w[OFF(i, n + 1, k)] = w[OFF(i, n, k)];
-                P[SOF(i, n + 1, k)] = pout;
-            }
-        } else {
-            const int code = a.face[5];
-            const bool wall = (code == 0) || (po[OFF(i, 1, k)] < 0.9);
-            if (wall) {
-                u[OFF(i, 1, k)] = 0.; v[OFF(i, 1, k)] = 0.; w[OFF(i, 1, k)] = 0.;
-                v[OFF(i, 0, k)] = -v[OFF(i, 2, k)];
-                P[SOF(i, 0, k)] = P[SOF(i, 2, k)];
-            } else if (code == 1) {
-                u[OFF(i, 1, k)] = U; v[OFF(i, 1, k)] = 0.; w[OFF(i, 1, k)] = 0.;
-                u[OFF(i, 0, k)] = U; v[OFF(i, 0, k)] = 0.; w[OFF(i, 0, k)] = 0.;
-                P[SOF(i, 0, k)] = P[SOF(i, 2, k)];
-            } else {
-                u[OFF(i, 0, k)] = u[OFF(i, 2, k)];    // (i,2,k), as written at :1153
-                v[OFF(i, 0, k)] = v[OFF(i, 2, k)];
-                w[OFF(i, 0, k)] = w[OFF(i, 2, k)];
-                P[SOF(i, 0, k)] = pout;
-            }
-        }
-    }
-#undef OFF
-#undef SOF
+    boundary_air_cell(g, a, face, po, u, v, w, P, r, q);
 }
 
 // ------------------------------------------------------------------------------------
 // output_divergent_2d/3d (output.f90:217-224, 929-940) into a dense interior array.
 // ------------------------------------------------------------------------------------
+
 __global__ void k_divergence_out(PfGeom g, StepArgs a, const double *__restrict__ po, const double *__restrict__ u,
                                  const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
     if (i > g.m || j > g.n) return;
-    const int64_t o = pf_off(g, i, j, k);
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    const double pc = po[o];
-    double r = ((((po[o + 1] * u[o]) + (pc * u[o + 1])) / 2.) - (((po[o - 1] * u[o]) + (pc * u[o - 1])) / 2.)) / a.dx;
-    r = r + (((((po[o + sy] * v[o]) + (pc * v[o + sy])) / 2.) - (((po[o - sy] * v[o]) + (pc * v[o - sy])) / 2.)) / a.dy);
-    if (g.dim == 3)
-        r = r + (((((po[o + sz] * w[o]) + (pc * w[o + sz])) / 2.) - (((po[o - sz] * w[o]) + (pc * w[o - sz])) / 2.)) / a.dz);
-    const int64_t d = (int64_t)(i - 1) + (int64_t)g.m * ((int64_t)(j - 1) + (int64_t)g.n * (g.dim == 3 ? k - 1 : 0));
-    out[d] = r;
+    divout_cell(g, a, po, u, v, w, out, i, j, k);
 }
 
 // Clear the per-step diagnostics with a kernel (graph-capture safe, unlike a memset node).
@@ -782,10 +418,9 @@ void launch_predict(hipStream_t st, const PfGeom &g, const StepArgs &a, const do
 }
 
 void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                const double *w, const double *phi, const double *AW, const double *AS, const double *AB,
-                const double *P, double *BB) {
+                const double *w, const double *phi, const double *P, double *BB) {
     dim3 b(64, 4, 1);
-    hipLaunchKernelGGL(k_rhs, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, u, v, w, phi, AW, AS, AB, P, BB);
+    hipLaunchKernelGGL(k_rhs, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, u, v, w, phi, P, BB);
 }
 
 void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
@@ -808,20 +443,30 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
                     bool fast) {
     const int c = s.colour;
     const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
-    dim3 b(64, 4, 1);
-    dim3 g3 = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
+    const int KB = (g.dim == 3) ? s.kb : 1;
+    const SweepShape sp = sweep_shape(g, KB);
     SweepArgs s1 = s;
-    s1.gx = (int)g3.x;
-    s1.gy = (int)g3.y;
-    dim3 gr((unsigned)(g3.x * g3.y * g3.z));
-#define PF_SOR_PHI(D, F) hipLaunchKernelGGL((k_sor_phi<D, F>), gr, b, 0, st, g, a, s1, P + oth, P + own, PHIs + own, \
-                                            PHIs + oth, BB + own, phi, err_slot)
+    s1.gx = sp.gx;
+    s1.gy = sp.gy;
+    dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
+#define PF_SOR_PHI(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
+                                               PHIs + own, PHIs + oth, BB + own, phi, err_slot)
     if (g.dim == 3) {
-        if (fast) PF_SOR_PHI(3, true); else PF_SOR_PHI(3, false);
+        if (fast) {
+            if (KB == 1) PF_SOR_PHI(3, true, 1); else if (KB == 2) PF_SOR_PHI(3, true, 2); else PF_SOR_PHI(3, true, 4);
+        } else {
+            if (KB == 1) PF_SOR_PHI(3, false, 1); else if (KB == 2) PF_SOR_PHI(3, false, 2); else PF_SOR_PHI(3, false, 4);
+        }
     } else {
-        if (fast) PF_SOR_PHI(2, true); else PF_SOR_PHI(2, false);
+        if (fast) PF_SOR_PHI(2, true, 1); else PF_SOR_PHI(2, false, 1);
     }
 #undef PF_SOR_PHI
+}
+
+void launch_faces_check_direct(hipStream_t st, const PfGeom &g, const double *phi, const StepArgs &a, int *bad) {
+    dim3 b(64, 4, 1);
+    hipLaunchKernelGGL(k_faces_check_direct, grid2(g.m + 1, g.n + 1, g.dim == 3 ? g.l + 1 : 1, b), b, 0, st, g, phi, a,
+                       bad);
 }
 
 void launch_faces_check(hipStream_t st, const PfGeom &g, const double *phi, const double *AW, const double *AS,
@@ -875,9 +520,16 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
 }
 
 int sor_blocks(const PfGeom &g) {
+    // slots for every kernel that reduces into them: stored-face sweep, KB = 1, 2, 4 sweeps
+    // and the one-cell-per-thread residual
     dim3 b(64, 4, 1);
     dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
-    return (int)(gr.x * gr.y * gr.z);
+    int n = (int)(gr.x * gr.y * gr.z);
+    for (int kb : {1, 2, 4}) {
+        const int v = sweep_shape(g, kb).blocks();
+        n = v > n ? v : n;
+    }
+    return n;
 }
 
 void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err) {

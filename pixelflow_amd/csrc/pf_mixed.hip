@@ -13,6 +13,7 @@
 // and the residual/update passes stay FP64.
 #include <hip/hip_runtime.h>
 
+#include "pf_cells.h"
 #include "pf_device.h"
 #include "pf_kernels.h"
 
@@ -20,56 +21,16 @@ namespace pf {
 
 // FP32 face coefficient: dt/d^2 * max(small, (pa+pb)/2) (the FP64 path's expression,
 // regrouped; exact order does not matter for a correction that the FP64 residual checks).
-__device__ __forceinline__ float face32(float pa, float pb, float cdt) {
-    return fmaxf(1.e-6f, (pa + pb) * 0.5f) * cdt;
-}
 
-__device__ __forceinline__ void block_max_to_slot(double v, unsigned long long *slots) {
-    unsigned long long e = __double_as_longlong(v);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o2 = __shfl_xor(e, off, 64);
-        e = (o2 > e) ? o2 : e;
-    }
-    __shared__ unsigned long long red[4];
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((lane & 63) == 0) red[lane >> 6] = e;
-    __syncthreads();
-    if (lane == 0) {
-        unsigned long long b = red[0];
-        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-        for (int q = 1; q < nw; ++q) b = (red[q] > b) ? red[q] : b;
-        unsigned long long *slot = slots + blockIdx.x;
-        if (b > *slot) *slot = b;
-    }
-}
 
-struct CellIdx {
-    int i, j, k, sh, h;
-    int64_t o;
-    bool active;
-};
-
-template <int DIM>
-__device__ __forceinline__ CellIdx sweep_cell(const PfGeom &g, const SweepArgs &s) {
-    CellIdx c;
-    const int nb = (int)gridDim.x;
-    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
-    const int bx = L % s.gx, byz = L / s.gx;
-    const int by = byz % s.gy, bz = byz / s.gy;
-    c.j = by * blockDim.y + threadIdx.y + 1;
-    c.k = (DIM == 3) ? bz + 1 : 0;
-    c.sh = (s.colour + c.j + c.k) & 1;
-    c.h = (int)(bx * blockDim.x + threadIdx.x) + 1 - c.sh;
-    c.i = 2 * c.h + c.sh;
-    c.active = (c.j <= g.n) && (c.i <= g.m);
-    c.o = (int64_t)c.h + (int64_t)g.HP * ((int64_t)c.j + (int64_t)g.Y * c.k);
-    return c;
-}
 
 // ------------------------------------------------------------------------------------
 // FP64 residual of the cells of one colour (launched for both colours) -> FP32 r; clears
 // e at the cell and at the periodic ghost slot the cell feeds; block max of |r / ap|.
 // ------------------------------------------------------------------------------------
+// One cell: FP64 residual r = bb - A p -> R (FP32), e cleared at the cell and at the
+// periodic ghost slot it feeds; returns |r / ap|.
+
 template <int DIM, bool FAST>
 __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ P,
                                                   const double *__restrict__ PHIs, const double *__restrict__ BB,
@@ -77,102 +38,31 @@ __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArg
                                                   float *__restrict__ E, unsigned long long *slots) {
     const CellIdx q = sweep_cell<DIM>(g, s);
     double res = 0.;
-    if (q.active) {
-        const int64_t own = (int64_t)s.colour * g.HS, oth = (int64_t)(1 - s.colour) * g.HS;
-        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y, o = q.o;
-        const double *Po = P + own, *Px = P + oth, *Fo = PHIs + own, *Fx = PHIs + oth;
-        const double pc = Po[o], pW = Px[o - 1 + q.sh], pE = Px[o + q.sh], pS = Px[o - HP], pN = Px[o + HP];
-        const double fc = Fo[o], fW = Fx[o - 1 + q.sh], fE = Fx[o + q.sh], fS = Fx[o - HP], fN = Fx[o + HP];
-        Coef c;
-        c.ae = face_any<FAST>(fE, fc, a.dt, a.dx, a.rdx);
-        c.aw = face_any<FAST>(fc, fW, a.dt, a.dx, a.rdx);
-        c.an = face_any<FAST>(fN, fc, a.dt, a.dy, a.rdy);
-        c.as = face_any<FAST>(fc, fS, a.dt, a.dy, a.rdy);
-        double pB = 0., pT = 0.;
-        c.at = 0.;
-        c.ab = 0.;
-        if (DIM == 3) {
-            pB = Px[o - HPY];
-            pT = Px[o + HPY];
-            c.at = face_any<FAST>(Fx[o + HPY], fc, a.dt, a.dz, a.rdz);
-            c.ab = face_any<FAST>(fc, Fx[o - HPY], a.dt, a.dz, a.rdz);
-        }
-        const double ap = (DIM == 3) ? (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab
-                                     : (((-c.ae) - c.aw) - c.an) - c.as;
-        if (is_edge_row(g, a, q.i, q.j, q.k)) row_rules<false>(g, a, q.i, q.j, q.k, c, nullptr, nullptr, phi, 0.);
-        double r = BB[own + o];
-        r = r - c.ae * pE;
-        r = r - c.aw * pW;
-        r = r - c.an * pN;
-        r = r - c.as * pS;
-        if (DIM == 3) {
-            r = r - c.at * pT;
-            r = r - c.ab * pB;
-        }
-        r = r - ap * pc;
-        if (R) R[own + o] = (float)r;
-        if (E) E[own + o] = 0.f;
-        if (E && a.bc == PF_BC_UNIFORM) {  // periodic ghost slots of e that a sweep or refresh writes
-            if (q.j == g.n) E[pf_soff(g, q.i, 0, q.k)] = 0.f;
-            if (q.j == 1) E[pf_soff(g, q.i, g.n + 1, q.k)] = 0.f;
-            if (DIM == 3) {
-                if (q.k == g.l) E[pf_soff(g, q.i, q.j, 0)] = 0.f;
-                if (q.k == 1) E[pf_soff(g, q.i, q.j, g.l + 1)] = 0.f;
-            }
-        }
-        res = fabs(r / ap);
-    }
+    if (q.active) res = residual_cell<DIM, FAST>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E);
     if (s.track_err) block_max_to_slot(res, slots);
 }
 
 // ------------------------------------------------------------------------------------
 // FP32 red/black SOR half-sweep on A e = r (same structure as the FP64 sweep).
 // ------------------------------------------------------------------------------------
-template <int DIM>
+
+template <int DIM, int KB>
 __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s, const float *__restrict__ Eoth,
                                                float *__restrict__ Eown, const float *__restrict__ F32o,
                                                const float *__restrict__ F32x, const float *__restrict__ Ro,
                                                const double *__restrict__ phi, float cdx, float cdy, float cdz,
                                                unsigned long long *err_slot) {
-    const CellIdx q = sweep_cell<DIM>(g, s);
+    const SweepThread t = sweep_thread<DIM, KB>(g, s);
     double err = 0.;
-    if (q.active) {
-        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y, o = q.o;
-        const float ec = Eown[o];
-        const float eW = Eoth[o - 1 + q.sh], eE = Eoth[o + q.sh], eS = Eoth[o - HP], eN = Eoth[o + HP];
-        const float fc = F32o[o];
-        CoefT<float> c;
-        c.ae = face32(F32x[o + q.sh], fc, cdx);
-        c.aw = face32(fc, F32x[o - 1 + q.sh], cdx);
-        c.an = face32(F32x[o + HP], fc, cdy);
-        c.as = face32(fc, F32x[o - HP], cdy);
-        float eB = 0.f, eT = 0.f;
-        c.at = 0.f;
-        c.ab = 0.f;
-        if (DIM == 3) {
-            eB = Eoth[o - HPY];
-            eT = Eoth[o + HPY];
-            c.at = face32(F32x[o + HPY], fc, cdz);
-            c.ab = face32(fc, F32x[o - HPY], cdz);
+    if (t.j <= g.n && t.i <= g.m) {
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const int k = t.kbase + 2 * q;
+            if (DIM == 3 && k > g.l) break;
+            const int64_t o = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
+            const double e = sor32_cell<DIM>(g, a, s, t.i, t.j, k, t.sh, o, Eoth, Eown, F32o, F32x, Ro, phi, cdx, cdy, cdz);
+            err = (e > err) ? e : err;
         }
-        const float ap = -(c.ae + c.aw + c.an + c.as + c.at + c.ab);
-        if (is_edge_row(g, a, q.i, q.j, q.k)) row_rules<false, float>(g, a, q.i, q.j, q.k, c, nullptr, nullptr, phi, 0.);
-        float num = Ro[o] - c.ae * eE - c.aw * eW - c.an * eN - c.as * eS;
-        if (DIM == 3) num = num - c.at * eT - c.ab * eB;
-        const float om = (float)a.omega;
-        // the approximate reciprocal measured more accurate end-to-end than IEEE division
-        // here: the error of a fixed-count FP32 iterate is set by rounding in the iteration
-        const float en = num * __builtin_amdgcn_rcpf(ap) * om + ec * (1.f - om);
-        Eown[o] = en;
-        if (s.fuse_y) {
-            if (q.j == g.n) Eown[o - HP * (int64_t)g.n] = en;
-            if (q.j == 1) Eown[o + HP * (int64_t)g.n] = en;
-        }
-        if (DIM == 3 && s.fuse_z) {
-            if (q.k == g.l) Eown[o - HPY * (int64_t)g.l] = en;
-            if (q.k == 1) Eown[o + HPY * (int64_t)g.l] = en;
-        }
-        err = fabs((double)en - (double)ec);
     }
     if (s.track_err) block_max_to_slot(err, err_slot);
 }
@@ -230,17 +120,22 @@ void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepAr
 
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
                   const float *R, const double *phi, unsigned long long *err_slot) {
-    dim3 b(64, 4, 1);
-    dim3 gr = sweep_grid(g, s, b);
+    const int KB = (g.dim == 3) ? s.kb : 1;
+    const SweepShape sp = sweep_shape(g, KB);
+    s.gx = sp.gx;
+    s.gy = sp.gy;
+    dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
     const int64_t own = (int64_t)s.colour * g.HS, oth = (int64_t)(1 - s.colour) * g.HS;
     const float cdx = (float)(a.dt / (a.dx * a.dx)), cdy = (float)(a.dt / (a.dy * a.dy)),
                 cdz = (float)(a.dt / (a.dz * a.dz));
-    if (g.dim == 3)
-        hipLaunchKernelGGL(k_sor32<3>, gr, b, 0, st, g, a, s, E + oth, E + own, F32 + own, F32 + oth, R + own, phi,
-                           cdx, cdy, cdz, err_slot);
-    else
-        hipLaunchKernelGGL(k_sor32<2>, gr, b, 0, st, g, a, s, E + oth, E + own, F32 + own, F32 + oth, R + own, phi,
-                           cdx, cdy, cdz, err_slot);
+#define PF_S32(D, K) hipLaunchKernelGGL((k_sor32<D, K>), gr, b, 0, st, g, a, s, E + oth, E + own, F32 + own, F32 + oth, \
+                                        R + own, phi, cdx, cdy, cdz, err_slot)
+    if (g.dim == 3) {
+        if (KB == 1) PF_S32(3, 1); else if (KB == 2) PF_S32(3, 2); else PF_S32(3, 4);
+    } else {
+        PF_S32(2, 1);
+    }
+#undef PF_S32
 }
 
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P) {

@@ -45,6 +45,11 @@ class PfStats(ctypes.Structure):
                 ("sor_iters", ctypes.c_int32), ("cycles", ctypes.c_int32)]
 
 
+class PfBatchConfig(ctypes.Structure):
+    _fields_ = [("persistent_blocks", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("inner_iters", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class PixelFlowError(RuntimeError):
     pass
 
@@ -54,7 +59,11 @@ _lib: Optional[ctypes.CDLL] = None
 # Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
 EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
            "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_footprint_bytes", "pf_sync",
-           "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_probe_copy")
+           "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_probe_copy",
+           "pf_batch_create", "pf_batch_destroy", "pf_batch_last_error", "pf_batch_case_bytes", "pf_batch_free_bytes",
+           "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
+           "pf_batch_divergence", "pf_batch_set_config", "pf_batch_get_config", "pf_batch_sync", "pf_batch_case_steps",
+           "pf_device_memory")
 
 
 def lib() -> ctypes.CDLL:
@@ -84,6 +93,26 @@ def lib() -> ctypes.CDLL:
     L.pf_last_sweep_time_us.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.pf_sor_kernel.argtypes = [vp]
     L.pf_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int, dp]
+    L.pf_batch_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.c_int, ctypes.c_int64, ctypes.POINTER(vp)]
+    L.pf_batch_destroy.argtypes = [vp]
+    L.pf_batch_last_error.restype = ctypes.c_char_p
+    L.pf_batch_last_error.argtypes = [vp]
+    L.pf_batch_case_bytes.restype = ctypes.c_int64
+    L.pf_batch_case_bytes.argtypes = [ctypes.POINTER(PfDesc)]
+    L.pf_batch_free_bytes.restype = ctypes.c_int64
+    L.pf_batch_free_bytes.argtypes = [vp]
+    L.pf_batch_add.argtypes = [vp, ctypes.POINTER(PfDesc), dp, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.pf_batch_remove.argtypes = [vp, ctypes.c_int]
+    L.pf_batch_count.argtypes = [vp]
+    L.pf_batch_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PfStats)]
+    L.pf_batch_download.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, i64, i64]
+    L.pf_batch_divergence.argtypes = [vp, ctypes.c_int, dp]
+    L.pf_batch_set_config.argtypes = [vp, ctypes.POINTER(PfBatchConfig)]
+    L.pf_batch_get_config.argtypes = [vp, ctypes.POINTER(PfBatchConfig)]
+    L.pf_batch_sync.argtypes = [vp]
+    L.pf_batch_case_steps.restype = ctypes.c_int64
+    L.pf_batch_case_steps.argtypes = [vp, ctypes.c_int]
+    L.pf_device_memory.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -212,3 +241,118 @@ def probe_copy_gbps(nbytes: int, iters: int = 20) -> float:
     if rc != 0:
         raise PixelFlowError(f"pf_probe_copy: {ERRORS.get(rc, rc)}")
     return out.value
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    rc = lib().pf_device_memory(ctypes.byref(f), ctypes.byref(t))
+    if rc != 0:
+        raise PixelFlowError(f"pf_device_memory: {ERRORS.get(rc, rc)}")
+    return f.value, t.value
+
+
+def batch_case_bytes(p: CaseParams, precision: int = PF_PREC_FP64, **kw) -> int:
+    d = make_desc(p, precision=precision, **kw)
+    return int(lib().pf_batch_case_bytes(ctypes.byref(d)))
+
+
+class Batch:
+    """Many cases of one kind on the GPU, advanced together (see pf_batch_* in
+    include/pixelflow.h).  Cases are added with their porosity and leave when removed."""
+
+    def __init__(self, kind: CaseParams, max_cases: int, arena_bytes: int, precision: int = PF_PREC_FP64, **kw):
+        self.kw = dict(precision=precision, **kw)
+        self.kind_desc = make_desc(kind, **self.kw)
+        self.max_cases = int(max_cases)
+        h = ctypes.c_void_p()
+        rc = lib().pf_batch_create(ctypes.byref(self.kind_desc), self.max_cases, int(arena_bytes), ctypes.byref(h))
+        if rc != 0:
+            raise PixelFlowError(f"pf_batch_create: {ERRORS.get(rc, rc)}: {lib().pf_batch_last_error(None).decode()}")
+        self._h = h
+        self.params: Dict[int, CaseParams] = {}
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {lib().pf_batch_last_error(self._h).decode()}")
+
+    def add(self, p: CaseParams, phi_raw_interior: np.ndarray) -> int:
+        d = make_desc(p, **self.kw)
+        raw = np.ascontiguousarray(phi_raw_interior, dtype=np.float64).reshape(-1)
+        if raw.size != p.cells:
+            raise ValueError("porosity size does not match the grid")
+        slot = ctypes.c_int(-1)
+        self._check(lib().pf_batch_add(self._h, ctypes.byref(d), _dp(raw), float(p.threshold), ctypes.byref(slot)),
+                    "pf_batch_add")
+        self.params[slot.value] = p
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        self._check(lib().pf_batch_remove(self._h, int(slot)), "pf_batch_remove")
+        self.params.pop(slot, None)
+
+    def count(self) -> int:
+        return int(lib().pf_batch_count(self._h))
+
+    def free_bytes(self) -> int:
+        return int(lib().pf_batch_free_bytes(self._h))
+
+    def step(self, nsteps: int = 1) -> Dict[str, np.ndarray]:
+        st = (PfStats * (max(nsteps, 1) * self.max_cases))()
+        self._check(lib().pf_batch_step(self._h, int(nsteps), st), "pf_batch_step")
+        shape = (nsteps, self.max_cases)
+        return {
+            "sor_error": np.array([x.sor_error for x in st[:nsteps * self.max_cases]]).reshape(shape),
+            "residual": np.array([x.residual for x in st[:nsteps * self.max_cases]]).reshape(shape),
+            "sor_iters": np.array([x.sor_iters for x in st[:nsteps * self.max_cases]]).reshape(shape),
+            "cycles": np.array([x.cycles for x in st[:nsteps * self.max_cases]]).reshape(shape),
+        }
+
+    def field(self, slot: int, name: str) -> np.ndarray:
+        p = self.params[slot]
+        out = np.zeros(p.full_shape(), dtype=np.float64)
+        self._check(lib().pf_batch_download(self._h, int(slot), FIELDS[name], _dp(out), p.m + 2, p.n + 2),
+                    "pf_batch_download")
+        return out
+
+    def interior(self, slot: int, name: str) -> np.ndarray:
+        a = self.field(slot, name)
+        return a[1:-1, 1:-1, 1:-1].copy() if self.params[slot].dim == 3 else a[1:-1, 1:-1].copy()
+
+    def divergence(self, slot: int) -> np.ndarray:
+        p = self.params[slot]
+        out = np.zeros(p.interior_shape(), dtype=np.float64)
+        self._check(lib().pf_batch_divergence(self._h, int(slot), _dp(out)), "pf_batch_divergence")
+        return out
+
+    def case_steps(self, slot: int) -> int:
+        return int(lib().pf_batch_case_steps(self._h, int(slot)))
+
+    def set_config(self, persistent_blocks: int = 0, use_graph: bool = True, inner_iters: int = 0) -> None:
+        c = PfBatchConfig(int(persistent_blocks), 1 if use_graph else 0, int(inner_iters), 0)
+        self._check(lib().pf_batch_set_config(self._h, ctypes.byref(c)), "pf_batch_set_config")
+
+    def get_config(self) -> Dict[str, int]:
+        c = PfBatchConfig()
+        self._check(lib().pf_batch_get_config(self._h, ctypes.byref(c)), "pf_batch_get_config")
+        return {"persistent_blocks": c.persistent_blocks, "use_graph": c.use_graph, "inner_iters": c.inner_iters}
+
+    def sync(self) -> None:
+        self._check(lib().pf_batch_sync(self._h), "pf_batch_sync")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().pf_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

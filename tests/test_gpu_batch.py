@@ -1,0 +1,118 @@
+"""Batched execution: every case of a batch gives the bits it gives alone.
+
+A batch advances cases of uneven size with one launch per stage (persistent tile loops,
+hipGraph-captured steps).  The per-cell arithmetic is the single-case path's, so each case
+must match a single-case run of it exactly: FP64 (itself bit-identical to the reference),
+mixed precision, graph and eager launch, persistent and one-block-per-tile grids, and
+cases that join or leave between steps.
+"""
+import numpy as np
+import pytest
+
+from pixelflow_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(p):
+    return ("u", "v", "w", "p") if p.dim == 3 else ("u", "v", "p")
+
+
+def _single(p, phi, steps, **kw):
+    from pixelflow_amd.solver import Solver
+    with Solver(p, phi, **kw) as s:
+        st = s.step(steps)
+        return {f: s.interior(f) for f in _fields(p)}, st
+
+
+def _mixed_2d_cases():
+    out = []
+    for idx, (m, n) in enumerate([(64, 64), (96, 48), (130, 66), (64, 33), (128, 64)]):
+        width = 2.0
+        dx = width / (m - 1)
+        p = cases.CaseParams(dim=2, m=m, n=n, width=width, height=dx * (n - 1), xnue=2e-3 + 1e-3 * idx,
+                             time=6 * 0.2 * dx, istep_max=6, iter_max=30)
+        out.append((p, cases.random_circles_2d(m, n, 1 + idx % 3, (3.0, 8.0), seed=100 + idx)))
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("persistent", [0, 97])
+def test_batch_2d_fp64_matches_single(graph, persistent):
+    from pixelflow_amd.solver import Batch
+    cs = _mixed_2d_cases()
+    kind = cs[0][0]
+    with Batch(kind, max_cases=8, arena_bytes=1 << 28) as b:
+        b.set_config(persistent_blocks=persistent, use_graph=graph)
+        slots = [b.add(p, phi) for p, phi in cs]
+        st = b.step(6)
+        for (p, phi), s in zip(cs, slots):
+            ref, rst = _single(p, phi, 6)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"slot {s} {p.m}x{p.n} {f}")
+            np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])
+
+
+def test_batch_3d_fp64_matches_single_odd_and_even():
+    from pixelflow_amd.solver import Batch
+    specs = [(32, 16, 16), (33, 17, 15), (24, 20, 18), (40, 24, 16)]
+    cs = []
+    for (m, n, l) in specs:
+        p = cases.CaseParams(dim=3, m=m, n=n, l=l, width=1.0, height=(n - 1) / (m - 1), depth=(l - 1) / (m - 1),
+                             time=4 * 1e-3, istep_max=4, iter_max=25, xlambda=1e-4)
+        cs.append((p, cases.sphere_3d(m, n, l, 1.0, p.height, p.depth, (0.3, p.height / 2, p.depth / 2), 0.15)))
+    with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 28) as b:
+        slots = [b.add(p, phi) for p, phi in cs]
+        b.step(4)
+        for (p, phi), s in zip(cs, slots):
+            ref, _ = _single(p, phi, 4)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"{p.m}x{p.n}x{p.l} {f}")
+
+
+def test_batch_2d_mixed_matches_single_mixed():
+    from pixelflow_amd.solver import PF_PREC_MIXED, Batch
+    cs = _mixed_2d_cases()
+    with Batch(cs[0][0], max_cases=8, arena_bytes=1 << 28, precision=PF_PREC_MIXED, inner_iters=10) as b:
+        b.set_config(persistent_blocks=128, use_graph=True, inner_iters=10)
+        slots = [b.add(p, phi) for p, phi in cs]
+        b.step(6)
+        for (p, phi), s in zip(cs, slots):
+            ref, _ = _single(p, phi, 6, precision=PF_PREC_MIXED, inner_iters=10)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"{p.m}x{p.n} {f}")
+
+
+def test_batch_join_and_leave_between_steps():
+    """Continuous batching: a case that joins later and one that leaves early give the
+    same results as run alone."""
+    from pixelflow_amd.solver import Batch
+    cs = _mixed_2d_cases()
+    with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 27) as b:
+        s0 = b.add(*cs[0])
+        s1 = b.add(*cs[1])
+        b.step(2)
+        b.remove(s1)
+        s2 = b.add(*cs[2])           # reuses the freed slot and arena range
+        b.step(4)
+        ref0, _ = _single(*cs[0], 6)
+        ref2, _ = _single(*cs[2], 4)
+        assert b.case_steps(s0) == 6 and b.case_steps(s2) == 4
+        for f in ("u", "v", "p"):
+            np.testing.assert_array_equal(b.interior(s0, f), ref0[f])
+            np.testing.assert_array_equal(b.interior(s2, f), ref2[f])
+
+
+def test_batch_air_tolerance_mode_matches_single():
+    from pixelflow_amd.solver import PF_PREC_MIXED, Batch
+    specs = [(24, 24, 20, 1), (32, 24, 24, 2)]
+    cs = [cases.room_case(seed, steps=3, iter_max=20, size=(m, n, l)) for (m, n, l, seed) in specs]
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=20, residual_tol=1e-6, max_cycles=50)
+    with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 27, **kw) as b:
+        slots = [b.add(p, phi) for p, phi in cs]
+        st = b.step(3)
+        for (p, phi), s in zip(cs, slots):
+            ref, rst = _single(p, phi, 3, **kw)
+            np.testing.assert_array_equal(st["sor_iters"][:, s], rst["sor_iters"])
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"{p.m}x{p.n}x{p.l} {f}")
