@@ -116,6 +116,13 @@ def main(argv=None):
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the mixed-precision comparison on the same workload (0 = skip)")
     ap.add_argument("--mixed-inner", type=int, default=0, help="FP32 iterations per defect-correction cycle")
+    ap.add_argument("--batch-cases", type=int, default=512,
+                    help="cases of the batched 2D workload (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the served 3D-room workload (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-steps", type=int, default=300)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain device-copy calibration kernel and print its GB/s")
     args = ap.parse_args(argv)
@@ -189,6 +196,17 @@ def main(argv=None):
     }
     if args.mixed_steps > 0:
         res["mixed"] = mixed_comparison(p, phi, args.mixed_steps, args.warmup, args.mixed_inner, value)
+    if args.batch_cases > 0:
+        from pixelflow_amd import workloads
+        res["batch"] = workloads.batched_2d(count=args.batch_cases, steps=args.batch_steps, log=log)
+        log(f"[bench] batch: {res['batch']['cases_per_s']:.2f} cases/s, "
+            f"{res['batch']['speedup_vs_plain_gpu']:.2f}x the plain FP64 path")
+    if args.serve_requests > 0:
+        from pixelflow_amd import workloads
+        res["serve"] = workloads.served_rooms(n_requests=args.serve_requests, rate_per_s=args.serve_rate,
+                                              steps=args.serve_steps, log=log)
+        log(f"[bench] serve: p50 {res['serve']['p50_time_to_solution_s']:.2f} s, "
+            f"p99 {res['serve']['p99_time_to_solution_s']:.2f} s")
     if args.cpu_steps > 0:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")

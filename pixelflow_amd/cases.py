@@ -292,11 +292,19 @@ def random_boxes_3d(m: int, n: int, l: int, nboxes: int, side_range: Tuple[int, 
 
 
 def random_circles_2d(m: int, n: int, count: int, radius_range: Tuple[float, float], seed: int,
-                      interface_cells: float = 2.0, max_tries: int = 2000) -> np.ndarray:
+                      interface_cells: float = 2.0, max_tries: int = 2000, downstream_clear: float = 0.0,
+                      max_blockage: float = 1.0) -> np.ndarray:
     """1..8 style obstacle field: ``count`` non-overlapping circles (radii in cells), tanh
-    interface; numpy (n, m).  Circles keep a 2-cell gap and stay clear of the x faces."""
+    interface; numpy (n, m).  Circles keep a 2-cell gap and stay clear of the x faces;
+    ``downstream_clear`` (a fraction of m) keeps that much of the domain before the outlet
+    free of solid, and ``max_blockage`` bounds the fraction of any grid column inside
+    circles (the reference's outlet condition and central scheme do not survive a wake
+    that reaches the outlet, or a nearly closed channel)."""
     rng = np.random.default_rng(seed)
     circles: List[Tuple[float, float, float]] = []
+    blocked = np.zeros(m)
+    xs = np.arange(m, dtype=np.float64)
+    x_hi = m - max(4.0, downstream_clear * m)
     tries = 0
     while len(circles) < count and tries < max_tries:
         tries += 1
@@ -304,10 +312,15 @@ def random_circles_2d(m: int, n: int, count: int, radius_range: Tuple[float, flo
         r = min(r, 0.45 * min(m, n) - 2.0)
         if r < 1.0:
             break
-        cx = float(rng.uniform(r + 4.0, m - r - 4.0)) if m - 2 * r - 8.0 > 0 else m / 2.0
+        cx = float(rng.uniform(r + 4.0, x_hi - r)) if x_hi - 2 * r - 4.0 > 0 else min(m / 2.0, x_hi - r)
         cy = float(rng.uniform(r + 2.0, n - r - 2.0)) if n - 2 * r - 4.0 > 0 else n / 2.0
-        if all(math.hypot(cx - a, cy - b) > r + c + 2.0 for a, b, c in circles):
-            circles.append((cx, cy, r))
+        if not all(math.hypot(cx - a, cy - b) > r + c + 2.0 for a, b, c in circles):
+            continue
+        chord = 2.0 * np.sqrt(np.maximum(r * r - (xs - cx) ** 2, 0.0))
+        if max_blockage < 1.0 and np.max(blocked + chord) > max_blockage * n:
+            continue
+        blocked += chord
+        circles.append((cx, cy, r))
     I, J = np.meshgrid(np.arange(m, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="xy")
     d = np.full((n, m), np.inf)
     for cx, cy, r in circles:
@@ -388,11 +401,28 @@ def boxes_case(m: int = 512, n: int = 256, l: int = 256, steps: int = 200, seed:
 BATCH_SIZES_2D: Sequence[Tuple[int, int]] = ((64, 64), (128, 64), (256, 128), (384, 192), (512, 256))
 
 
+def stable_dt(dx: float, xnue: float, velocity: float = 1.0, dim: int = 2, cfl: float = 0.2) -> float:
+    """Largest dt the reference's explicit step tolerates with margin: its convection is
+    2nd-order central and its update forward Euler (ibm_2d_uniform_omp_cpu.f90:202-208), so
+    besides CFL = U*dt/dx <= cfl it needs U^2*dt/xnue <= 1 (central advection under forward
+    Euler) and a diffusion number xnue*dt/dx^2 <= 0.4/dim."""
+    return min(cfl * dx / velocity, xnue / velocity ** 2, 0.4 / dim * dx * dx / xnue)
+
+
+BATCH_RE_LENGTH_CELLS = 400.0
+
+
 def batch_cases_2d(count: int = 512, steps: int = 1000, seed: int = 11, iter_max: int = 60,
                    cfl: float = 0.2) -> List[Tuple[CaseParams, np.ndarray]]:
     """Independent 2D cases: sizes uniform from BATCH_SIZES_2D, 1-8 circles with radii 3-20
-    cells, Re uniform in 50-400 via xnue (Re = U*D/xnue with D the mean obstacle diameter),
-    dt from CFL = U*dt/dx, SOR 60 iterations, omega = 1.7."""
+    cells (in the upstream 65% of the domain, at most half of any column blocked), Re uniform in 50-400 via xnue on a length of BATCH_RE_LENGTH_CELLS cells
+    (xnue = U * 400 dx / Re, a cell Reynolds number U dx / xnue of 0.125-1), dt from
+    :func:`stable_dt` (CFL 0.2 capped by the central-scheme limits), SOR 60 iterations,
+    omega = 1.7.
+
+    The reference's explicit central scheme is stable on porosity interfaces only at cell
+    Reynolds numbers of about 1 (its cylinder-2d test case runs at 1); with Re on the
+    obstacle diameter (3-23 cells at 50-400) the cases diverge within about 100 steps."""
     rng = np.random.default_rng(seed)
     out = []
     for c in range(count):
@@ -401,11 +431,11 @@ def batch_cases_2d(count: int = 512, steps: int = 1000, seed: int = 11, iter_max
         dx = width / (m - 1)
         height = dx * (n - 1)
         nobs = int(rng.integers(1, 9))
-        phi = random_circles_2d(m, n, nobs, (3.0, 20.0), seed=int(rng.integers(0, 2**31 - 1)))
+        phi = random_circles_2d(m, n, nobs, (3.0, 20.0), seed=int(rng.integers(0, 2**31 - 1)),
+                                downstream_clear=0.35, max_blockage=0.5)
         re = float(rng.uniform(50.0, 400.0))
-        diameter = 2.0 * 11.5 * dx
-        xnue = 1.0 * diameter / re
-        dt = cfl * dx / 1.0
+        xnue = 1.0 * BATCH_RE_LENGTH_CELLS * dx / re
+        dt = stable_dt(dx, xnue, 1.0, 2, cfl)
         p = CaseParams(dim=2, m=m, n=n, width=width, height=height, depth=0.0, xnue=xnue, time=dt * steps,
                        istep_max=steps, inlet_velocity=1.0, iter_max=iter_max, relux_factor=1.7)
         out.append((p, phi))

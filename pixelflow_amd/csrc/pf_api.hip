@@ -357,7 +357,8 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
 }
 
 // Mixed precision pressure solve of one step (see pf_mixed.hip).  Returns cycles run;
-// *res receives the last evaluated residual (or -1).
+// *res receives the last evaluated residual (or -1): in tolerance mode the residual of the
+// final pressure, in fixed-count mode (report_residual) the residual after the step's solve.
 static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *iters) {
     const PfGeom &g = c->g;
     const bool periodic = (c->d.bc == PF_BC_UNIFORM);
@@ -391,11 +392,13 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
         std::memcpy(res, &bits, sizeof(double));
         return PF_OK;
     };
-    for (int cyc = 0; cyc < maxc; ++cyc) {
+    // tolerance mode: a residual before every cycle and one after the last allowed cycle
+    // (so *res is the residual of the returned pressure)
+    for (int cyc = 0; tol_mode ? cyc <= maxc : cyc < maxc; ++cyc) {
         int rc;
         if (tol_mode) {
             if ((rc = residual(true)) != PF_OK) return rc;
-            if (*res <= c->d.residual_tol) break;
+            if (*res <= c->d.residual_tol || cyc == maxc) break;
         } else {
             if (remaining <= 0) break;
             if ((rc = residual(false)) != PF_OK) return rc;

@@ -72,20 +72,18 @@ enum NatOp {
 };
 
 template <int OP>
-__global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int aux) {
-    const int G = (int)gridDim.x;
-    const int L = xcd_logical_block((int)blockIdx.x, G);
-    for (int t = L; t < v.nat_tiles; t += G) {
+__device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, int aux) {
+    {
         const int ci = find_case(v.nat_prefix, v.ncases, t);
-        if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) continue;
+        if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) return;
         const BatchCase &C = v.cases[ci];
         const PfGeom &g = C.g;
-        if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) continue;
+        if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) return;
         const int local = t - v.nat_prefix[ci];
         const int tx = local % C.nat_gx, r = local / C.nat_gx;
         const int ty = r % C.nat_gy, tz = r / C.nat_gy;
         const int i = tx * 64 + (int)threadIdx.x, j = ty * 4 + (int)threadIdx.y, k = tz;
-        if (i > g.m + 1 || j > g.n + 1) continue;
+        if (i > g.m + 1 || j > g.n + 1) return;
         const int p = parity, q = 1 - parity;
         if (OP == NAT_DIV) {
             div_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, i, j, k);
@@ -94,7 +92,7 @@ __global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int 
             else predict2_cell(g, C.a, C.U[q], C.V[q], C.div, C.phi, C.U[p], C.V[p], i, j);
         } else if (OP == NAT_RHS || OP == NAT_PROJECT) {
             const bool interior = i >= 1 && i <= g.m && j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
-            if (!interior) continue;
+            if (!interior) return;
             if (OP == NAT_RHS) rhs_cell(g, C.a, C.U[p], C.V[p], C.W[p], C.phi, C.P, C.BB, i, j, k);
             else project_cell(g, C.a, C.P, C.U[p], C.V[p], C.W[p], i, j, k);
         } else if (OP == NAT_BOUNDARY_UNIFORM) {
@@ -123,73 +121,97 @@ __global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int 
                 sj = j;
                 sk = (k == 0) ? g.l : 1;
             }
-            if (sj < 0) continue;
+            if (sj < 0) return;
             if (OP == NAT_REFRESH_P) C.P[pf_soff(g, i, j, k)] = C.P[pf_soff(g, i, sj, sk)];
             else C.E[pf_soff(g, i, j, k)] = C.E[pf_soff(g, i, sj, sk)];
         }
     }
 }
 
+template <int OP, bool PERSIST>
+__global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int aux) {
+    const int G = (int)gridDim.x;
+    const int L = xcd_logical_block((int)blockIdx.x, G);
+    if (PERSIST) {
+        for (int t = L; t < v.nat_tiles; t += G) nat_tile<OP>(v, t, parity, aux);
+    } else if (L < v.nat_tiles) {
+        nat_tile<OP>(v, L, parity, aux);
+    }
+}
+
 enum SwpKind { SWP_FP64 = 0, SWP_FP32 = 1, SWP_RESIDUAL = 2 };
 
-template <int DIM, int KIND>
+// Sweep tiles are TX slots wide and 256/TX rows (3D: half the rows, each thread pair
+// covering two planes of opposite parity); block = 256 threads, flat.
+//
+// SEL picks the cases a launch serves: 1 = cases whose face coefficients passed the
+// fast-division check (fast path), 0 = the others (IEEE division); FP32 sweeps serve every
+// case.  Splitting them keeps the slow path's registers out of the fast kernel.
+template <int DIM, int KIND, int TX, int SEL>
+__device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, int track,
+                                         unsigned long long *slots) {
+    constexpr int RY = 256 / TX;
+    const int txi = (int)threadIdx.x % TX, tyi = (int)threadIdx.x / TX;
+    const int ci = find_case(v.swp_prefix, v.ncases, t);
+    if (!v.active[ci]) return;   // block-uniform: every thread skips the tile together
+    const BatchCase &C = v.cases[ci];
+    if (KIND != SWP_FP32 && (int)C.fast != SEL) return;
+    const PfGeom &g = C.g;
+    const int local = t - v.swp_prefix[ci];
+    const int bx = local % C.swp_gx, r = local / C.swp_gx;
+    const int by = r % C.swp_gy, bz = r / C.swp_gy;
+    int j, k;
+    if (DIM == 3) {
+        j = by * (RY / 2) + (tyi >> 1) + 1;
+        k = bz * 2 + 1 + (tyi & 1);
+    } else {
+        j = by * RY + tyi + 1;
+        k = 0;
+    }
+    const int sh = (colour + j + k) & 1;
+    const int h = bx * TX + txi + 1 - sh;
+    const int i = 2 * h + sh;
+    double e = 0.;
+    if (j <= g.n && i <= g.m && (DIM == 2 || k <= g.l)) {
+        const int64_t o = (int64_t)h + (int64_t)g.HP * ((int64_t)j + (int64_t)g.Y * k);
+        const int64_t own = (int64_t)colour * g.HS, oth = (int64_t)(1 - colour) * g.HS;
+        SweepArgs s{};
+        s.colour = colour;
+        s.fuse_y = C.fuse_y;
+        s.fuse_z = C.fuse_z;
+        if (KIND == SWP_FP64) {
+            e = sor_phi_cell<DIM, SEL == 1>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
+                                            C.PHIs + oth, C.BB + own, C.phi);
+        } else if (KIND == SWP_FP32) {
+            const float cdx = (float)(C.a.dt / (C.a.dx * C.a.dx)), cdy = (float)(C.a.dt / (C.a.dy * C.a.dy)),
+                        cdz = (float)(C.a.dt / (C.a.dz * C.a.dz));
+            e = sor32_cell<DIM>(g, C.a, s, i, j, k, sh, o, C.E + oth, C.E + own, C.PHI32 + own, C.PHI32 + oth,
+                                C.R + own, C.phi, cdx, cdy, cdz);
+        } else {
+            e = residual_cell<DIM, SEL == 1>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
+        }
+    }
+    if (track) tile_max_to_slot(e, slots, t);
+}
+
+// PERSIST = false: one block per tile, so every load of the case table precedes the
+// tile's stores and the compiler keeps the table in scalar registers; true: a fixed grid
+// walks the tile list.
+template <int DIM, int KIND, int TX, int SEL, bool PERSIST>
 __global__ __launch_bounds__(256) void k_batch_swp(BatchView v, int colour, int track, unsigned long long *slots) {
     const int G = (int)gridDim.x;
     const int L = xcd_logical_block((int)blockIdx.x, G);
-    for (int t = L; t < v.swp_tiles; t += G) {
-        const int ci = find_case(v.swp_prefix, v.ncases, t);
-        if (!v.active[ci]) continue;   // block-uniform: every thread skips the tile together
-        const BatchCase &C = v.cases[ci];
-        const PfGeom &g = C.g;
-        const int local = t - v.swp_prefix[ci];
-        const int bx = local % C.swp_gx, r = local / C.swp_gx;
-        const int by = r % C.swp_gy, bz = r / C.swp_gy;
-        int j, k;
-        if (DIM == 3) {
-            j = by * 2 + (int)(threadIdx.y >> 1) + 1;
-            k = bz * 2 + 1 + (int)(threadIdx.y & 1);
-        } else {
-            j = by * 4 + (int)threadIdx.y + 1;
-            k = 0;
-        }
-        const int sh = (colour + j + k) & 1;
-        const int h = bx * 64 + (int)threadIdx.x + 1 - sh;
-        const int i = 2 * h + sh;
-        double e = 0.;
-        if (j <= g.n && i <= g.m && (DIM == 2 || k <= g.l)) {
-            const int64_t o = (int64_t)h + (int64_t)g.HP * ((int64_t)j + (int64_t)g.Y * k);
-            const int64_t own = (int64_t)colour * g.HS, oth = (int64_t)(1 - colour) * g.HS;
-            SweepArgs s{};
-            s.colour = colour;
-            s.fuse_y = C.fuse_y;
-            s.fuse_z = C.fuse_z;
-            if (KIND == SWP_FP64) {
-                if (C.fast)
-                    e = sor_phi_cell<DIM, true>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
-                                                C.PHIs + oth, C.BB + own, C.phi);
-                else
-                    e = sor_phi_cell<DIM, false>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
-                                                 C.PHIs + oth, C.BB + own, C.phi);
-            } else if (KIND == SWP_FP32) {
-                const float cdx = (float)(C.a.dt / (C.a.dx * C.a.dx)), cdy = (float)(C.a.dt / (C.a.dy * C.a.dy)),
-                            cdz = (float)(C.a.dt / (C.a.dz * C.a.dz));
-                e = sor32_cell<DIM>(g, C.a, s, i, j, k, sh, o, C.E + oth, C.E + own, C.PHI32 + own, C.PHI32 + oth,
-                                    C.R + own, C.phi, cdx, cdy, cdz);
-            } else {
-                if (C.fast)
-                    e = residual_cell<DIM, true>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
-                else
-                    e = residual_cell<DIM, false>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
-            }
-        }
-        if (track) tile_max_to_slot(e, slots, t);
+    if (PERSIST) {
+        for (int t = L; t < v.swp_tiles; t += G) swp_tile<DIM, KIND, TX, SEL>(v, t, colour, track, slots);
+    } else if (L < v.swp_tiles) {
+        swp_tile<DIM, KIND, TX, SEL>(v, L, colour, track, slots);
     }
 }
 
 // Per case: max over its tile slots -> hist[step * stride + slot] (step from *d_step, or
 // the fixed row `row` when row >= 0); the slots are cleared.  One block per case.
 __global__ void k_batch_reduce(BatchView v, unsigned long long *slots, unsigned long long *hist, const int *d_step,
-                               int row, int stride) {
+                               int row, int stride, int only_active) {
     const int ci = (int)blockIdx.x;
     if (ci >= v.ncases) return;
     const int t0 = v.swp_prefix[ci], t1 = v.swp_prefix[ci + 1];
@@ -206,12 +228,53 @@ __global__ void k_batch_reduce(BatchView v, unsigned long long *slots, unsigned 
     __shared__ unsigned long long red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && (!only_active || v.active[ci])) {
         unsigned long long b = red[0];
         for (int q = 1; q < 4; ++q) b = (red[q] > b) ? red[q] : b;
         const int r = (row >= 0) ? row : *d_step;
         hist[(int64_t)r * stride + v.cases[ci].slot] = b;
     }
+}
+
+// Tolerance mode, after residual evaluation `cyc` of a step (one block): a case stays
+// active while its residual (the max over its cells, as double bits in res_hist) exceeds
+// tol and fewer than maxc cycles ran; an active case is charged K more iterations and
+// one cycle.  Writes the number of still-active cases to *nactive.  Cycle 0 also zeroes
+// the step's iteration and cycle counters.
+__global__ void k_batch_tol_decide(BatchView v, int *active, const unsigned long long *res_hist, int *it_hist,
+                                   int *cy_hist, const int *d_step, int stride, double tol, int cyc, int maxc,
+                                   int K, int *nactive) {
+    const int64_t row = (int64_t)(*d_step) * stride;
+    int count = 0;
+    for (int ci = (int)threadIdx.x; ci < v.ncases; ci += (int)blockDim.x) {
+        const int slot = v.cases[ci].slot;
+        if (cyc == 0) {
+            it_hist[row + slot] = 0;
+            cy_hist[row + slot] = 0;
+        }
+        if (!active[ci]) continue;
+        const double r = __longlong_as_double((long long)res_hist[row + slot]);
+        if (r <= tol || cyc == maxc) {
+            active[ci] = 0;
+        } else {
+            it_hist[row + slot] += K;
+            cy_hist[row + slot] += 1;
+            ++count;
+        }
+    }
+    __shared__ int red[256];
+    red[threadIdx.x] = count;
+    __syncthreads();
+    for (int off = (int)blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nactive = red[0];
+}
+
+__global__ void k_batch_fill_active(int *active, int ncases, int val) {
+    for (int ci = (int)(blockIdx.x * blockDim.x + threadIdx.x); ci < ncases; ci += (int)(gridDim.x * blockDim.x))
+        active[ci] = val;
 }
 
 __global__ void k_set_int(int *p, int val) {
@@ -250,9 +313,13 @@ struct pf_batch {
     unsigned long long *d_tile_slots = nullptr;   // per-tile SOR error maxima
     unsigned long long *d_res_slots = nullptr;    // per-tile residual maxima
     unsigned long long *d_err_hist = nullptr, *d_res_hist = nullptr;
+    int *d_it_hist = nullptr, *d_cy_hist = nullptr;   // tolerance mode: iterations, cycles
+    int *d_nactive = nullptr;
     int hist_rows = 0;
     bool dirty = true;
     bool any_refresh = false;
+    bool any_fast = false, any_slow = false;   // cases on the fast / IEEE division path
+    int tile_x = 64;                // sweep tile width in slots (32, 64 or 128)
     int parity = 0;
     pf_batch_config cfg{};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -320,16 +387,19 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
     b->cfg.persistent_blocks = 0;
     b->cfg.use_graph = 1;
     b->cfg.inner_iters = kind->inner_iters;
+    b->cfg.tile_x = 64;
     b->arena_bytes = align256(arena_bytes);
     hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&b->arena, (size_t)b->arena_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_step, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_active, sizeof(int) * (size_t)max_cases);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_nactive, sizeof(int));
     if (e != hipSuccess) {
         g_batch_create_error = std::string("batch allocation: ") + hipGetErrorString(e);
         if (b->arena) (void)hipFree(b->arena);
         if (b->d_step) (void)hipFree(b->d_step);
         if (b->d_active) (void)hipFree(b->d_active);
+        if (b->d_nactive) (void)hipFree(b->d_nactive);
         if (b->st) (void)hipStreamDestroy(b->st);
         delete b;
         return PF_ENOMEM;
@@ -344,7 +414,8 @@ extern "C" int pf_batch_destroy(pf_batch *b) {
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graphs(b);
     void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_active, b->d_step,
-                    b->d_tile_slots, b->d_res_slots, b->d_err_hist, b->d_res_hist};
+                    b->d_nactive, b->d_tile_slots, b->d_res_slots, b->d_err_hist, b->d_res_hist,
+                    b->d_it_hist, b->d_cy_hist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->st) (void)hipStreamDestroy(b->st);
@@ -397,17 +468,21 @@ static int rebuild_tables(pf_batch *b) {
     std::vector<BatchCase> cs;
     std::vector<int> np(1, 0), sp(1, 0);
     b->any_refresh = false;
+    b->any_fast = b->any_slow = false;
     for (int s : b->order) {
         BatchCase bc = b->slots[(size_t)s].bc;
         const PfGeom &g = bc.g;
         bc.nat_gx = (g.m + 2 + 63) / 64;
         bc.nat_gy = (g.n + 2 + 3) / 4;
-        const pf::SweepShape sh = pf::sweep_shape(g, 1);
-        bc.swp_gx = sh.gx;
-        bc.swp_gy = sh.gy;
+        const int TX = b->tile_x, RY = 256 / TX;
+        bc.swp_gx = ((g.m + 1) / 2 + TX - 1) / TX;
+        bc.swp_gy = (g.dim == 3) ? (g.n + RY / 2 - 1) / (RY / 2) : (g.n + RY - 1) / RY;
+        const int swp_gz = (g.dim == 3) ? (g.l + 1) / 2 : 1;
         np.push_back(np.back() + bc.nat_gx * bc.nat_gy * g.Z);
-        sp.push_back(sp.back() + sh.blocks());
+        sp.push_back(sp.back() + bc.swp_gx * bc.swp_gy * swp_gz);
         b->any_refresh = b->any_refresh || bc.ref_y || bc.ref_z;
+        b->any_fast = b->any_fast || bc.fast;
+        b->any_slow = b->any_slow || !bc.fast;
         b->slots[(size_t)s].bc = bc;
         cs.push_back(bc);
     }
@@ -579,7 +654,13 @@ static void launch_nat(pf_batch *b, int op, int parity, int aux) {
     const int nb = grid_blocks(b, v.nat_tiles);
     if (nb == 0) return;
     dim3 blk(64, 4, 1), gr((unsigned)nb);
-#define PFB_NAT(OP) hipLaunchKernelGGL(pf::k_batch_nat<OP>, gr, blk, 0, b->st, v, parity, aux)
+#define PFB_NAT(OP)                                                                              \
+    do {                                                                                         \
+        if (b->cfg.persistent_blocks > 0)                                                        \
+            hipLaunchKernelGGL((pf::k_batch_nat<OP, true>), gr, blk, 0, b->st, v, parity, aux);  \
+        else                                                                                     \
+            hipLaunchKernelGGL((pf::k_batch_nat<OP, false>), gr, blk, 0, b->st, v, parity, aux); \
+    } while (0)
     switch (op) {
         case pf::NAT_DIV: PFB_NAT(pf::NAT_DIV); break;
         case pf::NAT_PREDICT: PFB_NAT(pf::NAT_PREDICT); break;
@@ -594,30 +675,63 @@ static void launch_nat(pf_batch *b, int op, int parity, int aux) {
 #undef PFB_NAT
 }
 
+template <int D, int K, int T, int SEL>
+static void launch_swp_t(pf_batch *b, dim3 gr, const pf::BatchView &v, int colour, int track,
+                         unsigned long long *slots) {
+    if (b->cfg.persistent_blocks > 0)
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+    else
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+}
+
+template <int D, int K, int SEL>
+static void launch_swp_tx(pf_batch *b, dim3 gr, const pf::BatchView &v, int colour, int track,
+                          unsigned long long *slots) {
+    if (b->tile_x == 32) launch_swp_t<D, K, 32, SEL>(b, gr, v, colour, track, slots);
+    else if (b->tile_x == 128) launch_swp_t<D, K, 128, SEL>(b, gr, v, colour, track, slots);
+    else launch_swp_t<D, K, 64, SEL>(b, gr, v, colour, track, slots);
+}
+
+template <int D>
+static void launch_swp_dim(pf_batch *b, int kind, dim3 gr, const pf::BatchView &v, int colour, int track,
+                           unsigned long long *slots) {
+    if (kind == pf::SWP_FP32) {
+        launch_swp_tx<D, pf::SWP_FP32, 1>(b, gr, v, colour, track, slots);
+        return;
+    }
+    // fast-division cases and the others in separate launches (usually only the first)
+    if (kind == pf::SWP_FP64) {
+        if (b->any_fast) launch_swp_tx<D, pf::SWP_FP64, 1>(b, gr, v, colour, track, slots);
+        if (b->any_slow) launch_swp_tx<D, pf::SWP_FP64, 0>(b, gr, v, colour, track, slots);
+    } else {
+        if (b->any_fast) launch_swp_tx<D, pf::SWP_RESIDUAL, 1>(b, gr, v, colour, track, slots);
+        if (b->any_slow) launch_swp_tx<D, pf::SWP_RESIDUAL, 0>(b, gr, v, colour, track, slots);
+    }
+}
+
 static void launch_swp(pf_batch *b, int kind, int colour, int track) {
     const pf::BatchView v = view(b);
     const int nb = grid_blocks(b, v.swp_tiles);
     if (nb == 0) return;
-    dim3 blk(64, 4, 1), gr((unsigned)nb);
+    const dim3 gr((unsigned)nb);
     unsigned long long *slots = (kind == pf::SWP_RESIDUAL) ? b->d_res_slots : b->d_tile_slots;
-#define PFB_SWP(D, K) hipLaunchKernelGGL((pf::k_batch_swp<D, K>), gr, blk, 0, b->st, v, colour, track, slots)
-    if (b->kind.dim == 3) {
-        if (kind == pf::SWP_FP64) PFB_SWP(3, pf::SWP_FP64);
-        else if (kind == pf::SWP_FP32) PFB_SWP(3, pf::SWP_FP32);
-        else PFB_SWP(3, pf::SWP_RESIDUAL);
-    } else {
-        if (kind == pf::SWP_FP64) PFB_SWP(2, pf::SWP_FP64);
-        else if (kind == pf::SWP_FP32) PFB_SWP(2, pf::SWP_FP32);
-        else PFB_SWP(2, pf::SWP_RESIDUAL);
-    }
-#undef PFB_SWP
+    if (b->kind.dim == 3) launch_swp_dim<3>(b, kind, gr, v, colour, track, slots);
+    else launch_swp_dim<2>(b, kind, gr, v, colour, track, slots);
 }
 
-static void launch_reduce(pf_batch *b, unsigned long long *slots, unsigned long long *hist, int row) {
+static void launch_reduce(pf_batch *b, unsigned long long *slots, unsigned long long *hist, int row,
+                          int only_active = 0) {
     const pf::BatchView v = view(b);
     if (v.ncases == 0) return;
     hipLaunchKernelGGL(pf::k_batch_reduce, dim3((unsigned)v.ncases), dim3(256), 0, b->st, v, slots, hist, b->d_step,
-                       row, b->max_cases);
+                       row, b->max_cases, only_active);
+}
+
+static void launch_fill_active(pf_batch *b, int val) {
+    const pf::BatchView v = view(b);
+    if (v.ncases == 0) return;
+    hipLaunchKernelGGL(pf::k_batch_fill_active, dim3((unsigned)((v.ncases + 255) / 256)), dim3(256), 0, b->st,
+                       b->d_active, v.ncases, val);
 }
 
 // Iterations of the pressure solve (fixed count): FP64 reference sweeps, or FP64 residual +
@@ -677,20 +791,83 @@ static void enqueue_step_fixed(pf_batch *b, int p) {
 
 static int ensure_hist(pf_batch *b, int rows) {
     if (b->hist_rows >= rows) return PF_OK;
+    PFB_TRY(b, hipStreamSynchronize(b->st));
     if (b->d_err_hist) (void)hipFree(b->d_err_hist);
     if (b->d_res_hist) (void)hipFree(b->d_res_hist);
+    if (b->d_it_hist) (void)hipFree(b->d_it_hist);
+    if (b->d_cy_hist) (void)hipFree(b->d_cy_hist);
     b->d_err_hist = b->d_res_hist = nullptr;
+    b->d_it_hist = b->d_cy_hist = nullptr;
+    b->hist_rows = 0;
     const size_t n = (size_t)rows * (size_t)b->max_cases;
     PFB_TRY(b, hipMalloc((void **)&b->d_err_hist, n * sizeof(unsigned long long)));
     PFB_TRY(b, hipMalloc((void **)&b->d_res_hist, n * sizeof(unsigned long long)));
+    PFB_TRY(b, hipMalloc((void **)&b->d_it_hist, n * sizeof(int)));
+    PFB_TRY(b, hipMalloc((void **)&b->d_cy_hist, n * sizeof(int)));
     b->hist_rows = rows;
     drop_graphs(b);   // graphs captured the old history addresses
     return PF_OK;
 }
 
+// Tolerance mode: whole steps are captured only while the cycle cap is small (a captured
+// step always launches every cycle; the blocks of converged cases exit at once).
+static const int kTolGraphMaxCycles = 16;
+
+static int tol_max_cycles(const pf_batch *b) { return b->kind.max_cycles > 0 ? b->kind.max_cycles : 1000; }
+
+static int tol_inner(const pf_batch *b) {
+    return b->cfg.inner_iters > 0 ? b->cfg.inner_iters : (b->kind.iter_max > 0 ? b->kind.iter_max : 1);
+}
+
+// One residual evaluation + decision (`cyc`), then -- unless it was the last allowed --
+// one cycle of K sweeps (FP32 on the error equation + FP64 update, or FP64 sweeps) for the
+// cases still active.
+static void enqueue_tol_cycle(pf_batch *b, int cyc) {
+    const int dim = b->kind.dim;
+    const int first = (dim == 3) ? 0 : 1;
+    const bool mixed = b->kind.precision == PF_PREC_MIXED;
+    const int K = tol_inner(b), maxc = tol_max_cycles(b);
+    launch_swp(b, pf::SWP_RESIDUAL, 0, 1);
+    launch_swp(b, pf::SWP_RESIDUAL, 1, 1);
+    launch_reduce(b, b->d_res_slots, b->d_res_hist, -1, 1);
+    const pf::BatchView v = view(b);
+    hipLaunchKernelGGL(pf::k_batch_tol_decide, dim3(1), dim3(256), 0, b->st, v, b->d_active, b->d_res_hist,
+                       b->d_it_hist, b->d_cy_hist, b->d_step, b->max_cases, b->kind.residual_tol, cyc, maxc, K,
+                       b->d_nactive);
+    if (cyc == maxc) return;
+    for (int it = 0; it < K; ++it)
+        for (int half = 0; half < 2; ++half) {
+            const int colour = half == 0 ? first : 1 - first;
+            launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1));
+            if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0);
+        }
+    if (mixed) launch_nat(b, pf::NAT_UPDATE, 0, 0);
+}
+
+static void enqueue_tol_begin(pf_batch *b, int p) {
+    enqueue_pre(b, p);
+    launch_fill_active(b, 1);
+}
+
+static void enqueue_tol_end(pf_batch *b, int p) {
+    launch_fill_active(b, 1);
+    launch_reduce(b, b->d_tile_slots, b->d_err_hist, -1);
+    enqueue_post(b, p);
+    hipLaunchKernelGGL(pf::k_inc_int, dim3(1), dim3(64), 0, b->st, b->d_step);
+}
+
+static void enqueue_step_tol_full(pf_batch *b, int p) {
+    enqueue_tol_begin(b, p);
+    for (int cyc = 0; cyc <= tol_max_cycles(b); ++cyc) enqueue_tol_cycle(b, cyc);
+    enqueue_tol_end(b, p);
+}
+
+static bool tol_mode(const pf_batch *b) { return b->kind.residual_tol > 0; }
+
 static int capture(pf_batch *b, int p) {
     PFB_TRY(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    enqueue_step_fixed(b, p);
+    if (tol_mode(b)) enqueue_step_tol_full(b, p);
+    else enqueue_step_fixed(b, p);
     hipGraph_t gph = nullptr;
     PFB_TRY(b, hipStreamEndCapture(b->st, &gph));
     b->graph[p] = gph;
@@ -698,54 +875,22 @@ static int capture(pf_batch *b, int p) {
     return PF_OK;
 }
 
-// Tolerance mode: cycles per step with a host check of each case's FP64 residual.
-static int step_tolerance(pf_batch *b, int p, int step_row, std::vector<int> &iters, std::vector<int> &cycles,
-                          std::vector<double> &res) {
-    const int nc = (int)b->order.size();
-    const int dim = b->kind.dim;
-    const int first = (dim == 3) ? 0 : 1;
-    const bool mixed = b->kind.precision == PF_PREC_MIXED;
-    const int K = b->cfg.inner_iters > 0 ? b->cfg.inner_iters : (b->kind.iter_max > 0 ? b->kind.iter_max : 1);
-    const int maxc = b->kind.max_cycles > 0 ? b->kind.max_cycles : 1000;
-    std::vector<int> act((size_t)nc, 1);
-    iters.assign((size_t)nc, 0);
-    cycles.assign((size_t)nc, 0);
-    res.assign((size_t)nc, -1.0);
-    PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
-    enqueue_pre(b, p);
-    std::vector<unsigned long long> row((size_t)b->max_cases);
+// Tolerance mode without a graph: the same launches, with the number of active cases read
+// back every few cycles so a step ends once every case has converged.
+static int step_tolerance_eager(pf_batch *b, int p) {
+    enqueue_tol_begin(b, p);
+    const int maxc = tol_max_cycles(b);
+    const int check_every = 4;
     for (int cyc = 0; cyc <= maxc; ++cyc) {
-        launch_swp(b, pf::SWP_RESIDUAL, 0, 1);
-        launch_swp(b, pf::SWP_RESIDUAL, 1, 1);
-        launch_reduce(b, b->d_res_slots, b->d_res_hist, step_row);
-        PFB_TRY(b, hipMemcpyAsync(row.data(), b->d_res_hist + (size_t)step_row * b->max_cases,
-                                  sizeof(unsigned long long) * row.size(), hipMemcpyDeviceToHost, b->st));
-        PFB_TRY(b, hipStreamSynchronize(b->st));
-        int n_active = 0;
-        for (int ci = 0; ci < nc; ++ci) {
-            if (!act[(size_t)ci]) continue;
-            double r;
-            std::memcpy(&r, &row[(size_t)b->order[(size_t)ci]], sizeof(double));
-            res[(size_t)ci] = r;
-            if (r <= b->kind.residual_tol || cyc == maxc) act[(size_t)ci] = 0;
-            else ++n_active;
+        enqueue_tol_cycle(b, cyc);
+        if (cyc < maxc && (cyc + 1) % check_every == 0) {
+            int nact = 0;
+            PFB_TRY(b, hipMemcpyAsync(&nact, b->d_nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
+            PFB_TRY(b, hipStreamSynchronize(b->st));
+            if (nact == 0) break;
         }
-        PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
-        if (n_active == 0) break;
-        for (int it = 0; it < K; ++it)
-            for (int half = 0; half < 2; ++half) {
-                const int colour = half == 0 ? first : 1 - first;
-                launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1));
-                if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0);
-            }
-        if (mixed) launch_nat(b, pf::NAT_UPDATE, 0, 0);
-        for (int ci = 0; ci < nc; ++ci)
-            if (act[(size_t)ci]) { iters[(size_t)ci] += K; cycles[(size_t)ci] += 1; }
     }
-    std::fill(act.begin(), act.end(), 1);
-    PFB_TRY(b, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
-    launch_reduce(b, b->d_tile_slots, b->d_err_hist, step_row);
-    enqueue_post(b, p);
+    enqueue_tol_end(b, p);
     return PF_OK;
 }
 
@@ -757,22 +902,16 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
     if (b->order.empty()) return PF_OK;
     if ((rc = ensure_hist(b, nsteps)) != PF_OK) return rc;
     const int nc = (int)b->order.size();
-    const bool tol = b->kind.residual_tol > 0;
-    std::vector<std::vector<int>> it_rows, cy_rows;
-    std::vector<std::vector<double>> res_rows;
+    const bool tol = tol_mode(b);
+    const bool graph = b->cfg.use_graph && (!tol || tol_max_cycles(b) <= kTolGraphMaxCycles);
     hipLaunchKernelGGL(pf::k_set_int, dim3(1), dim3(64), 0, b->st, b->d_step, 0);
     for (int s = 0; s < nsteps; ++s) {
         const int p = b->parity;
-        if (tol) {
-            std::vector<int> its, cys;
-            std::vector<double> rs;
-            if ((rc = step_tolerance(b, p, s, its, cys, rs)) != PF_OK) return rc;
-            it_rows.push_back(its);
-            cy_rows.push_back(cys);
-            res_rows.push_back(rs);
-        } else if (b->cfg.use_graph) {
+        if (graph) {
             if (!b->gexec[p] && (rc = capture(b, p)) != PF_OK) return rc;
             PFB_TRY(b, hipGraphLaunch(b->gexec[p], b->st));
+        } else if (tol) {
+            if ((rc = step_tolerance_eager(b, p)) != PF_OK) return rc;
         } else {
             enqueue_step_fixed(b, p);
         }
@@ -780,9 +919,16 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
         b->parity = 1 - p;
     }
     std::vector<unsigned long long> eh((size_t)nsteps * b->max_cases), rh((size_t)nsteps * b->max_cases);
+    std::vector<int> ih, ch;
     PFB_TRY(b, hipMemcpyAsync(eh.data(), b->d_err_hist, eh.size() * 8, hipMemcpyDeviceToHost, b->st));
     if (tol || b->kind.report_residual)
         PFB_TRY(b, hipMemcpyAsync(rh.data(), b->d_res_hist, rh.size() * 8, hipMemcpyDeviceToHost, b->st));
+    if (tol) {
+        ih.resize(eh.size());
+        ch.resize(eh.size());
+        PFB_TRY(b, hipMemcpyAsync(ih.data(), b->d_it_hist, ih.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+        PFB_TRY(b, hipMemcpyAsync(ch.data(), b->d_cy_hist, ch.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+    }
     PFB_TRY(b, hipStreamSynchronize(b->st));
     bool finite = true;
     for (int ci = 0; ci < nc; ++ci) b->slots[(size_t)b->order[(size_t)ci]].steps += nsteps;
@@ -799,9 +945,9 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
                 if (!std::isfinite(e)) finite = false;
                 o.sor_error = e;
                 if (tol) {
-                    o.residual = res_rows[(size_t)s][(size_t)ci];
-                    o.sor_iters = it_rows[(size_t)s][(size_t)ci];
-                    o.cycles = cy_rows[(size_t)s][(size_t)ci];
+                    o.residual = r;
+                    o.sor_iters = ih[(size_t)s * b->max_cases + slot];
+                    o.cycles = ch[(size_t)s * b->max_cases + slot];
                 } else {
                     o.residual = (b->kind.report_residual && b->kind.precision == PF_PREC_MIXED) ? r : -1.0;
                     o.sor_iters = b->kind.iter_max;
@@ -851,8 +997,15 @@ extern "C" int pf_batch_divergence(pf_batch *b, int slot, double *out) {
 
 extern "C" int pf_batch_set_config(pf_batch *b, const pf_batch_config *cfg) {
     if (!b || !cfg || cfg->persistent_blocks < 0 || cfg->inner_iters < 0) return PF_EINVAL;
+    const int tx = cfg->tile_x == 0 ? 64 : cfg->tile_x;
+    if (tx != 32 && tx != 64 && tx != 128) { b->msg = "tile_x must be 32, 64 or 128"; return PF_EINVAL; }
     PFB_TRY(b, hipStreamSynchronize(b->st));
     b->cfg = *cfg;
+    b->cfg.tile_x = tx;
+    if (tx != b->tile_x) {
+        b->tile_x = tx;
+        b->dirty = true;
+    }
     drop_graphs(b);
     return PF_OK;
 }

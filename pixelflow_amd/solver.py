@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG, "_build", "libpixelflow.so")
 PF_BC_UNIFORM, PF_BC_AIR = 0, 1
 PF_PREC_FP64, PF_PREC_MIXED = 0, 1
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "porosity": 4}
+PF_EDIVERGED = -5
 ERRORS = {-1: "PF_EINVAL", -2: "PF_EHIP", -3: "PF_ENOMEM", -4: "PF_ESTATE", -5: "PF_EDIVERGED"}
 
 
@@ -47,7 +48,7 @@ class PfStats(ctypes.Structure):
 
 class PfBatchConfig(ctypes.Structure):
     _fields_ = [("persistent_blocks", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("inner_iters", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("inner_iters", ctypes.c_int32), ("tile_x", ctypes.c_int32)]
 
 
 class PixelFlowError(RuntimeError):
@@ -142,6 +143,20 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+# pf_stats as a numpy record (same layout), so per-step statistics of large batches are
+# filled in place instead of converted struct by struct.
+STATS_DTYPE = np.dtype([("sor_error", "<f8"), ("residual", "<f8"), ("sor_iters", "<i4"), ("cycles", "<i4")])
+assert STATS_DTYPE.itemsize == ctypes.sizeof(PfStats)
+
+
+def _stats_ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(PfStats))
+
+
+def _stats_dict(a: np.ndarray) -> Dict[str, np.ndarray]:
+    return {k: np.ascontiguousarray(a[k]) for k in STATS_DTYPE.names}
+
+
 class Solver:
     """One case on the GPU (FP64 path: reproduces the reference bit for bit)."""
 
@@ -167,15 +182,10 @@ class Solver:
             raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
 
     def step(self, nsteps: int = 1) -> Dict[str, np.ndarray]:
-        st = (PfStats * max(nsteps, 1))()
-        self._check(lib().pf_step(self._h, int(nsteps), st), "pf_step")
+        st = np.zeros(max(nsteps, 1), dtype=STATS_DTYPE)
+        self._check(lib().pf_step(self._h, int(nsteps), _stats_ptr(st)), "pf_step")
         self.steps_done += nsteps
-        return {
-            "sor_error": np.array([s.sor_error for s in st[:nsteps]]),
-            "residual": np.array([s.residual for s in st[:nsteps]]),
-            "sor_iters": np.array([s.sor_iters for s in st[:nsteps]]),
-            "cycles": np.array([s.cycles for s in st[:nsteps]]),
-        }
+        return _stats_dict(st[:nsteps])
 
     def field(self, name: str) -> np.ndarray:
         """Full field with ghosts, numpy shape (l+2, n+2, m+2) or (n+2, m+2)."""
@@ -297,16 +307,20 @@ class Batch:
     def free_bytes(self) -> int:
         return int(lib().pf_batch_free_bytes(self._h))
 
-    def step(self, nsteps: int = 1) -> Dict[str, np.ndarray]:
-        st = (PfStats * (max(nsteps, 1) * self.max_cases))()
-        self._check(lib().pf_batch_step(self._h, int(nsteps), st), "pf_batch_step")
-        shape = (nsteps, self.max_cases)
-        return {
-            "sor_error": np.array([x.sor_error for x in st[:nsteps * self.max_cases]]).reshape(shape),
-            "residual": np.array([x.residual for x in st[:nsteps * self.max_cases]]).reshape(shape),
-            "sor_iters": np.array([x.sor_iters for x in st[:nsteps * self.max_cases]]).reshape(shape),
-            "cycles": np.array([x.cycles for x in st[:nsteps * self.max_cases]]).reshape(shape),
-        }
+    def case_bytes(self, p: CaseParams) -> int:
+        """Arena bytes a case of this batch's settings takes (pf_batch_case_bytes)."""
+        return batch_case_bytes(p, **self.kw)
+
+    def step(self, nsteps: int = 1, raise_diverged: bool = True) -> Dict[str, np.ndarray]:
+        """Advance every case `nsteps` steps; per-step statistics shaped (nsteps, max_cases).
+        A case whose SOR error became non-finite raises, unless `raise_diverged` is False:
+        then the statistics come back (the other cases advanced normally) and the caller
+        finds the diverged slots by their non-finite ``sor_error``."""
+        st = np.zeros((max(nsteps, 1), self.max_cases), dtype=STATS_DTYPE)
+        rc = lib().pf_batch_step(self._h, int(nsteps), _stats_ptr(st))
+        if not (rc == PF_EDIVERGED and not raise_diverged):
+            self._check(rc, "pf_batch_step")
+        return _stats_dict(st[:nsteps])
 
     def field(self, slot: int, name: str) -> np.ndarray:
         p = self.params[slot]
@@ -328,14 +342,16 @@ class Batch:
     def case_steps(self, slot: int) -> int:
         return int(lib().pf_batch_case_steps(self._h, int(slot)))
 
-    def set_config(self, persistent_blocks: int = 0, use_graph: bool = True, inner_iters: int = 0) -> None:
-        c = PfBatchConfig(int(persistent_blocks), 1 if use_graph else 0, int(inner_iters), 0)
+    def set_config(self, persistent_blocks: int = 0, use_graph: bool = True, inner_iters: int = 0,
+                   tile_x: int = 64) -> None:
+        c = PfBatchConfig(int(persistent_blocks), 1 if use_graph else 0, int(inner_iters), int(tile_x))
         self._check(lib().pf_batch_set_config(self._h, ctypes.byref(c)), "pf_batch_set_config")
 
     def get_config(self) -> Dict[str, int]:
         c = PfBatchConfig()
         self._check(lib().pf_batch_get_config(self._h, ctypes.byref(c)), "pf_batch_get_config")
-        return {"persistent_blocks": c.persistent_blocks, "use_graph": c.use_graph, "inner_iters": c.inner_iters}
+        return {"persistent_blocks": c.persistent_blocks, "use_graph": c.use_graph, "inner_iters": c.inner_iters,
+                "tile_x": c.tile_x}
 
     def sync(self) -> None:
         self._check(lib().pf_batch_sync(self._h), "pf_batch_sync")

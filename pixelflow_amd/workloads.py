@@ -1,0 +1,141 @@
+"""Drivers of the batched and served workloads (used by bench.py).
+
+* :func:`batched_2d` -- many independent 2D cylinder-array cases in one batch, FP32 SOR with
+  FP64 defect correction, tuned by the autotuner, against the plain one-case-at-a-time FP64
+  path on a sample of the same cases;
+* :func:`served_rooms` -- Poisson-arriving 3D room-ventilation requests through the
+  continuous-batching server, mixed precision, reporting p50/p99 time-to-solution.
+"""
+from __future__ import annotations
+
+import time
+from typing import Callable, Dict
+
+import numpy as np
+
+from . import autotune, cases
+from .serve import Request, Server, latency_stats
+from .solver import PF_PREC_MIXED, Batch, Solver, batch_case_bytes
+
+
+def _noop(*a):
+    pass
+
+
+def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: int = 11, tune_steps: int = 2,
+               plain_sample: int = 5, plain_steps: int = 20,
+               log: Callable = _noop) -> Dict[str, object]:
+    cl = cases.batch_cases_2d(count=count, steps=steps, seed=seed, iter_max=iter_max)
+    kind = cl[0][0]
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=iter_max // 2)
+    arena = sum(batch_case_bytes(p, **kw) for p, _ in cl) + (64 << 20)
+    total_cells = sum(p.cells for p, _ in cl)
+    sample_idx = list(range(0, len(cl), max(1, len(cl) // plain_sample)))[:plain_sample]
+    key = autotune.size_class_key(2, "uniform", True, iter_max, [p.cells for p, _ in cl])
+    log(f"[batch] {count} cases, {total_cells} cells, arena {arena / 2**30:.2f} GiB; autotuning")
+    with Batch(kind, max_cases=count, arena_bytes=arena, **kw) as tb:
+        for p, phi in cl:
+            tb.add(p, phi)
+        tuned = autotune.tune(tb, key, iter_max, True, steps=tune_steps, use_cache=False, log=log)
+        tb.sync()
+    cfg = tuned["best"]
+    log(f"[batch] tuned config {cfg}")
+    with Batch(kind, max_cases=count, arena_bytes=arena, **kw) as b:
+        b.set_config(**cfg)
+        slots = [b.add(p, phi) for p, phi in cl]
+        b.step(1)                                  # captures the graphs
+        b.sync()
+        t0 = time.perf_counter()
+        st = b.step(steps, raise_diverged=False)
+        b.sync()
+        wall = time.perf_counter() - t0
+        diverged = int(np.sum(~np.all(np.isfinite(st["sor_error"][:, slots]), axis=0)))
+        # accuracy on a sample: the batched mixed run vs each case alone in FP64
+        accuracy = []
+        for i in sample_idx:
+            s = slots[i]
+            p, phi = cl[i]
+            with Solver(p, phi) as ref:
+                ref.step(steps + 1)
+                accuracy.append({"case": f"{p.m}x{p.n}", **{f: float(np.linalg.norm(b.interior(s, f) - ref.interior(f))
+                                                                       / max(np.linalg.norm(ref.interior(f)), 1e-300))
+                                                              for f in ("u", "v", "p")}})
+    cell_steps = total_cells * steps / wall
+    # plain GPU path: the same cases, one at a time, FP64 (the reference's arithmetic)
+    plain_cells, plain_time = 0, 0.0
+    for i in sample_idx:
+        p, phi = cl[i]
+        with Solver(p, phi) as s:
+            s.step(2)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(plain_steps)
+            s.sync()
+            plain_time += time.perf_counter() - t0
+            plain_cells += p.cells * plain_steps
+    plain = plain_cells / plain_time
+    return {
+        "workload": f"batch2d_{count}_cases_cylinders",
+        "cases": count, "total_cells": total_cells, "steps": steps, "iter_max": iter_max,
+        "precision": "fp32 SOR + fp64 defect correction (fixed count)",
+        "wall_s": wall, "cases_per_s": count / wall,
+        "cell_steps_per_s": cell_steps, "sor_cell_updates_per_s": cell_steps * iter_max,
+        "plain_fp64_cell_steps_per_s": plain, "speedup_vs_plain_gpu": cell_steps / plain,
+        "plain_sample": f"{plain_sample} of the cases, one at a time, {plain_steps} steps each",
+        "autotune": {"best": cfg, "candidates": len(tuned.get("candidates", []))},
+        "accuracy_vs_fp64_alone": accuracy, "diverged_cases": diverged,
+    }
+
+
+def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
+                 inner: int = 25, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
+                 plain_sample: int = 3, plain_steps: int = 10, log: Callable = _noop) -> Dict[str, object]:
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, size=n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for r in range(n_requests):
+        size = None if sizes is None else sizes[r % len(sizes)]
+        p, phi = cases.room_case(int(rng.integers(0, 2**31 - 1)), steps=steps, iter_max=iter_max, size=size)
+        reqs.append(Request(r, p, phi, steps, float(arrivals[r])))
+    kind = reqs[0].params
+    kw = dict(inner_iters=inner, residual_tol=tol, max_cycles=max(1, iter_max // inner))
+    peak = max(batch_case_bytes(r.params, precision=PF_PREC_MIXED, **kw) for r in reqs)
+    log(f"[serve] {n_requests} requests, rate {rate_per_s}/s, {steps} steps each; largest case {peak / 2**20:.0f} MiB")
+    srv = Server(kind, PF_PREC_MIXED, max_cases=min(n_requests, 64), memory_fraction=0.3, chunk=chunk,
+                 keep_fields=(), **kw)
+    for r in reqs:
+        srv.submit(r)
+    t0 = time.perf_counter()
+    done = srv.run()
+    wall = time.perf_counter() - t0
+    srv.close()
+    stats = latency_stats(done)
+    # plain GPU path on a sample of the same rooms: one case at a time, FP64, iter_max
+    # iterations every step (the reference's schedule)
+    plain_upd, plain_time = 0, 0.0
+    for r in reqs[:plain_sample]:
+        with Solver(r.params, r.phi) as s:
+            s.step(2)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(plain_steps)
+            s.sync()
+            plain_time += time.perf_counter() - t0
+            plain_upd += r.params.cells * r.params.iter_max * plain_steps
+    plain = plain_upd / plain_time
+    served = srv.sor_cell_updates / wall
+    res = [r.last_stats["residual"] for r in done if r.last_stats]
+    its = [r.last_stats["sor_iters"] for r in done if r.last_stats]
+    return {
+        "workload": f"served_rooms_{n_requests}_requests",
+        "rate_per_s": rate_per_s, "steps_per_request": steps, "iter_cap_per_step": iter_max,
+        "residual_tol": tol, "precision": "fp32 SOR + fp64 defect correction (tolerance mode, capped)",
+        **stats, "wall_s": wall, "cell_steps_per_s": srv.cell_steps / wall,
+        "sor_cell_updates_per_s": served, "plain_fp64_sor_cell_updates_per_s": plain,
+        "speedup_vs_plain_gpu": served / plain,
+        "plain_sample": f"{plain_sample} of the rooms, one at a time, {plain_steps} steps each",
+        "final_residual_median": float(np.median(res)) if res else None,
+        "final_step_iterations_median": float(np.median(its)) if its else None,
+        "sizes": sorted({f"{r.params.m}x{r.params.n}x{r.params.l}" for r in reqs}),
+    }

@@ -36,14 +36,15 @@ def _mixed_2d_cases():
     return out
 
 
+@pytest.mark.parametrize("tile_x", [32, 64, 128])
 @pytest.mark.parametrize("graph", [True, False])
 @pytest.mark.parametrize("persistent", [0, 97])
-def test_batch_2d_fp64_matches_single(graph, persistent):
+def test_batch_2d_fp64_matches_single(graph, persistent, tile_x):
     from pixelflow_amd.solver import Batch
     cs = _mixed_2d_cases()
     kind = cs[0][0]
     with Batch(kind, max_cases=8, arena_bytes=1 << 28) as b:
-        b.set_config(persistent_blocks=persistent, use_graph=graph)
+        b.set_config(persistent_blocks=persistent, use_graph=graph, tile_x=tile_x)
         slots = [b.add(p, phi) for p, phi in cs]
         st = b.step(6)
         for (p, phi), s in zip(cs, slots):
@@ -62,6 +63,7 @@ def test_batch_3d_fp64_matches_single_odd_and_even():
                              time=4 * 1e-3, istep_max=4, iter_max=25, xlambda=1e-4)
         cs.append((p, cases.sphere_3d(m, n, l, 1.0, p.height, p.depth, (0.3, p.height / 2, p.depth / 2), 0.15)))
     with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 28) as b:
+        b.set_config(tile_x=32)
         slots = [b.add(p, phi) for p, phi in cs]
         b.step(4)
         for (p, phi), s in zip(cs, slots):
@@ -103,16 +105,23 @@ def test_batch_join_and_leave_between_steps():
             np.testing.assert_array_equal(b.interior(s2, f), ref2[f])
 
 
-def test_batch_air_tolerance_mode_matches_single():
+@pytest.mark.parametrize("max_cycles,graph", [(50, True), (50, False), (8, True), (8, False)])
+def test_batch_air_tolerance_mode_matches_single(max_cycles, graph):
+    """Tolerance mode decides convergence on the device (captured whole when the cycle cap
+    is small, else launched with a periodic check); iteration counts and fields must equal
+    the single-case run's."""
     from pixelflow_amd.solver import PF_PREC_MIXED, Batch
     specs = [(24, 24, 20, 1), (32, 24, 24, 2)]
     cs = [cases.room_case(seed, steps=3, iter_max=20, size=(m, n, l)) for (m, n, l, seed) in specs]
-    kw = dict(precision=PF_PREC_MIXED, inner_iters=20, residual_tol=1e-6, max_cycles=50)
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=20, residual_tol=1e-6, max_cycles=max_cycles)
     with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 27, **kw) as b:
+        b.set_config(use_graph=graph, inner_iters=20)
         slots = [b.add(p, phi) for p, phi in cs]
         st = b.step(3)
         for (p, phi), s in zip(cs, slots):
             ref, rst = _single(p, phi, 3, **kw)
             np.testing.assert_array_equal(st["sor_iters"][:, s], rst["sor_iters"])
+            np.testing.assert_array_equal(st["cycles"][:, s], rst["cycles"])
+            np.testing.assert_array_equal(st["residual"][:, s], rst["residual"])
             for f in _fields(p):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"{p.m}x{p.n}x{p.l} {f}")
