@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 1
+#define PF_ABI_VERSION 2
 
 enum pf_status {
     PF_OK = 0,
@@ -52,6 +52,9 @@ typedef struct pf_desc {
     int32_t inner_iters;      /* mixed: FP32 SOR iterations per defect-correction cycle */
     int32_t max_cycles;       /* mixed, tolerance mode: cap on cycles per step */
     int32_t report_residual;  /* mixed, fixed-count mode: also evaluate the final FP64 residual */
+    int32_t inlet_porosity;   /* 2D uniform: inlet and initial velocity scaled by porosity, the
+                                 backward-facing-step executable (ibm_2d_backstep_omp_cpu.f90:
+                                 533-534, 615-618); 0 for the others */
     double dx, dy, dz, dt;
     double xnue, xlambda, density, thickness;
     double inlet_velocity, outlet_pressure, aoa;
@@ -104,6 +107,13 @@ int pf_step(pf_ctx *ctx, int nsteps, pf_stats *stats);
 /* output_divergent_2d/3d (src/omp_parallel/lib/output.f90:217-224, 929-940): the
  * porosity-weighted flux divergence of the current velocity, interior only, i fastest. */
 int pf_divergence(pf_ctx *ctx, double *out_interior);
+
+/* output_force_log_2d/3d (src/omp_parallel/lib/output.f90:244-305, 1090-1165), the drag
+ * executable's per-step log (ibm_2d_drag_omp_cpu.f90:121): out[0..5] = pressure force x,
+ * y, z and wall-force (viscous) force x, y, z on the immersed body for the current fields
+ * (2D: z entries 0).  Each cell's terms are the reference's; the sum is a fixed-order
+ * device tree (deterministic, equal to the reference's serial sum up to rounding). */
+int pf_force(pf_ctx *ctx, double *out);
 
 /* Device bytes the context holds (admission control sizes cases with this). */
 int64_t pf_footprint_bytes(const pf_desc *desc);

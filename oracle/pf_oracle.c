@@ -132,12 +132,16 @@ void pfo_setup_porosity(const pfo_case *c, const double *raw, double threshold, 
 /* boundary: 2D src/omp_parallel/ibm_2d_uniform_omp_cpu.f90:460-538;                     */
 /* 3D ibm_3d_uniform_omp_cpu.f90:669-752; air ibm_3d_air_condition_omp_cpu.f90:873-1170  */
 /* ------------------------------------------------------------------------------------ */
-static void boundary_2d(const pfo_case *c, double *u, double *v, double *p) {
+static void boundary_2d(const pfo_case *c, const double *phi, double *u, double *v, double *p) {
     const int m = c->m, n = c->n, M2 = m + 2;
     const double pai = atan(1.) * 4.;
     for (int j = 1; j <= n; ++j) {
         u[I2(1, j)] = c->inlet_velocity * cos(((c->aoa / 180.) * pai));
         v[I2(1, j)] = c->inlet_velocity * sin(((c->aoa / 180.) * pai));
+        if (c->inlet_porosity) {   /* ibm_2d_backstep_omp_cpu.f90:533-534 */
+            u[I2(1, j)] = u[I2(1, j)] * phi[I2(1, j)];
+            v[I2(1, j)] = v[I2(1, j)] * phi[I2(1, j)];
+        }
         u[I2(0, j)] = u[I2(1, j)];
         v[I2(0, j)] = v[I2(1, j)];
         p[I2(0, j)] = p[I2(2, j)];
@@ -346,7 +350,7 @@ static void boundary_3d_air(const pfo_case *c, const double *phi, double *u, dou
 
 static void boundary_any(const pfo_case *c, const double *phi, double *u, double *v, double *w,
                          double *p) {
-    if (c->dim == 2) boundary_2d(c, u, v, p);
+    if (c->dim == 2) boundary_2d(c, phi, u, v, p);
     else if (c->bc == PFO_BC_AIR) boundary_3d_air(c, phi, u, v, w, p);
     else boundary_3d_uniform(c, u, v, w, p);
 }
@@ -367,6 +371,10 @@ void pfo_initial(const pfo_case *c, const double *phi, double *u, double *v, dou
             for (int i = 1; i <= m; ++i) {
                 u[I2(i, j)] = c->inlet_velocity * cos(((c->aoa / 180.) * pai));
                 v[I2(i, j)] = c->inlet_velocity * sin(((c->aoa / 180.) * pai));
+                if (c->inlet_porosity) {   /* ibm_2d_backstep_omp_cpu.f90:615-618 */
+                    u[I2(i, j)] = u[I2(i, j)] * phi[I2(i, j)];
+                    v[I2(i, j)] = v[I2(i, j)] * phi[I2(i, j)];
+                }
                 p[I2(i, j)] = c->outlet_pressure;
             }
     } else {
@@ -539,7 +547,7 @@ static double step_2d(pfo_ctx *x, const double *phi, double *u, double *v, doubl
             u[o] = u[o] - ((((dt / density) * (p[I2(i + 1, j)] - p[I2(i - 1, j)])) / dx) * 0.5);
             v[o] = v[o] - ((((dt / density) * (p[I2(i, j + 1)] - p[I2(i, j - 1)])) / dy) * 0.5);
         }
-    boundary_2d(c, u, v, p);
+    boundary_2d(c, phi, u, v, p);
     return err;
 }
 
@@ -911,6 +919,67 @@ void pfo_divergence(const pfo_case *c, const double *phi, const double *u, const
                     (((((po[I3(i, j, k + 1)] * w[o]) + (po[o] * w[I3(i, j, k + 1)])) / 2.) -
                       (((po[I3(i, j, k - 1)] * w[o]) + (po[o] * w[I3(i, j, k - 1)])) / 2.)) / dz);
             }
+}
+
+/* output_force_log_2d (src/omp_parallel/lib/output.f90:244-305, loop :264-284, i outer)
+ * and output_force_log_3d (:1090-1165, loop :1118-1145, k outer): the pressure and the
+ * wall-force (viscous) contributions of every interior cell, each term in the Fortran
+ * evaluation order, summed serially in loop order.  x**2 is x*x; unary minus applies to
+ * the whole product. */
+void pfo_force(const pfo_case *c, const double *phi, const double *u, const double *v,
+               const double *w, const double *p, double *out) {
+    const int m = c->m, n = c->n, l = c->l, M2 = m + 2, N2 = n + 2;
+    const double dx = c->dx, dy = c->dy, dz = c->dz, th = c->thickness;
+    const double small = 1.e-6, alpha = 32.0, rho = c->density, nu = c->xnue;
+    double fpx = 0., fpy = 0., fpz = 0., fvx = 0., fvy = 0., fvz = 0.;
+    if (c->dim == 2) {
+        for (int i = 1; i <= m; ++i)
+            for (int j = 1; j <= n; ++j) {
+                const size_t o = I2(i, j);
+                const double gx = (phi[I2(i + 1, j)] - phi[I2(i - 1, j)]) * 0.5;
+                const double gy = (phi[I2(i, j + 1)] - phi[I2(i, j - 1)]) * 0.5;
+                const double na = sqrt((gx * gx) + (gy * gy));
+                const double den = na > small ? na : small;
+                const double nx = gx / den, ny = gy / den;
+                const double ph = phi[o], q = 1.0 - ph;
+                const double dpx = -((((((dx * dy) * p[o]) * 2) * ph) * q) / (th * dx) * nx);
+                const double dpy = -((((((dx * dy) * p[o]) * 2) * ph) * q) / (th * dy) * ny);
+                const double tx = (ph * q) / (th * dx), ty = (ph * q) / (th * dy);
+                const double dvx = (((((dx * dy) * alpha) * rho) * nu) * (tx * tx)) * u[o];
+                const double dvy = (((((dx * dy) * alpha) * rho) * nu) * (ty * ty)) * v[o];
+                fpx = fpx + dpx;
+                fpy = fpy + dpy;
+                fvx = fvx + dvx;
+                fvy = fvy + dvy;
+            }
+    } else {
+        for (int k = 1; k <= l; ++k)
+            for (int j = 1; j <= n; ++j)
+                for (int i = 1; i <= m; ++i) {
+                    const size_t o = I3(i, j, k);
+                    const double gx = (phi[I3(i + 1, j, k)] - phi[I3(i - 1, j, k)]) * 0.5;
+                    const double gy = (phi[I3(i, j + 1, k)] - phi[I3(i, j - 1, k)]) * 0.5;
+                    const double gz = (phi[I3(i, j, k + 1)] - phi[I3(i, j, k - 1)]) * 0.5;
+                    const double na = sqrt(((gx * gx) + (gy * gy)) + (gz * gz));
+                    const double den = na > small ? na : small;
+                    const double nx = gx / den, ny = gy / den, nz = gz / den;
+                    const double ph = phi[o], q = 1.0 - ph;
+                    const double base = (((((dx * dy) * dz) * p[o]) * 2) * ph) * q;
+                    const double dpx = -(base / (th * dx) * nx);
+                    const double dpy = -(base / (th * dy) * ny);
+                    const double dpz = -(base / (th * dz) * nz);
+                    const double tx = (ph * q) / (th * dx), ty = (ph * q) / (th * dy), tz = (ph * q) / (th * dz);
+                    const double cv = ((((dx * dy) * dz) * alpha) * rho) * nu;
+                    fpx = fpx + dpx;
+                    fpy = fpy + dpy;
+                    fpz = fpz + dpz;
+                    fvx = fvx + (cv * (tx * tx)) * u[o];
+                    fvy = fvy + (cv * (ty * ty)) * v[o];
+                    fvz = fvz + (cv * (tz * tz)) * w[o];
+                }
+    }
+    out[0] = fpx; out[1] = fpy; out[2] = fpz;
+    out[3] = fvx; out[4] = fvy; out[5] = fvz;
 }
 
 /* Brute force over the linear-index decode of solve_matrix_vec_omp (3D :493-508; 2D

@@ -31,6 +31,7 @@ typedef struct pfo_case {
     double xnue, xlambda, density, thickness;
     double inlet_velocity, outlet_pressure, aoa;
     double omega;        /* relux_factor */
+    int inlet_porosity;  /* 2D backward-step variant: inlet and initial velocity times porosity */
 } pfo_case;
 
 typedef struct pfo_ctx pfo_ctx;
@@ -50,6 +51,10 @@ int pfo_step(pfo_ctx *x, const double *phi, double *u, double *v, double *w, dou
 /* output_divergent_*: porosity-weighted flux divergence, interior only, i fastest. */
 void pfo_divergence(const pfo_case *c, const double *phi, const double *u, const double *v,
                     const double *w, double *div_interior);
+/* output_force_log_2d/3d: out[0..5] = pressure force x, y, z, viscous force x, y, z
+ * (2D: z entries 0), summed serially in the reference's loop order. */
+void pfo_force(const pfo_case *c, const double *phi, const double *u, const double *v,
+               const double *w, const double *p, double *out);
 /* Colour the reference's linear-index decode assigns to interior cell (i,j[,k]) in the
  * first sweep: returns 1 if the first sweep updates it, 0 if the second does, -1 if the
  * decode visits it in neither or twice (never for valid extents). Brute force helper. */

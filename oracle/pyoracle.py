@@ -27,6 +27,7 @@ class _Case(ctypes.Structure):
         ("xnue", ctypes.c_double), ("xlambda", ctypes.c_double), ("density", ctypes.c_double),
         ("thickness", ctypes.c_double), ("inlet_velocity", ctypes.c_double),
         ("outlet_pressure", ctypes.c_double), ("aoa", ctypes.c_double), ("omega", ctypes.c_double),
+        ("inlet_porosity", ctypes.c_int),
     ]
 
 
@@ -48,6 +49,7 @@ def lib() -> ctypes.CDLL:
         L.pfo_step.restype = ctypes.c_int
         L.pfo_step.argtypes = [ctypes.c_void_p, dp, dp, dp, dp, dp, ctypes.c_int, dp]
         L.pfo_divergence.argtypes = [ctypes.POINTER(_Case), dp, dp, dp, dp, dp]
+        L.pfo_force.argtypes = [ctypes.POINTER(_Case), dp, dp, dp, dp, dp, dp]
         L.pfo_decode_first_sweep.restype = ctypes.c_int
         L.pfo_decode_first_sweep.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)]
         _lib = L
@@ -74,6 +76,7 @@ def make_case(p) -> _Case:
     c.xnue, c.xlambda, c.density, c.thickness = p.xnue, p.xlambda, p.density, p.thickness
     c.inlet_velocity, c.outlet_pressure, c.aoa = p.inlet_velocity, p.outlet_pressure, p.aoa
     c.omega = p.relux_factor
+    c.inlet_porosity = 1 if getattr(p, "inlet_porosity", False) else 0
     return c
 
 
@@ -107,6 +110,13 @@ class OracleRun:
     def divergence(self) -> np.ndarray:
         out = np.zeros(self.p.interior_shape(), dtype=np.float64)
         lib().pfo_divergence(ctypes.byref(self.case), _dp(self.phi), _dp(self.u), _dp(self.v), _dp(self.w), _dp(out))
+        return out
+
+    def force(self) -> np.ndarray:
+        """output_force_log_*: [Fp_x, Fp_y, Fp_z, Fv_x, Fv_y, Fv_z] of the current state."""
+        out = np.zeros(6, dtype=np.float64)
+        lib().pfo_force(ctypes.byref(self.case), _dp(self.phi), _dp(self.u), _dp(self.v), _dp(self.w),
+                        _dp(self.p_), _dp(out))
         return out
 
     def interior(self, name: str) -> np.ndarray:

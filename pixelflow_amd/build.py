@@ -81,9 +81,9 @@ def build_fortran() -> List[str]:
     moddir = os.path.join(OUT, "mod")
     os.makedirs(bindir, exist_ok=True)
     os.makedirs(moddir, exist_ok=True)
-    mods = ["pf_binding.f90", "pf_case_io.f90"]
+    mods = ["pf_binding.f90", "pf_case_io.f90", "pf_driver.f90"]
     mains = sorted(f for f in os.listdir(FSRC) if f.startswith("pf_main_") and f.endswith(".f90"))
-    flags = ["-O2", "-J", moddir]
+    flags = ["-O2", "-ffp-contract=off", "-J", moddir]
     objs = []
     for f in mods:
         if not os.path.exists(os.path.join(FSRC, f)):

@@ -68,6 +68,9 @@ class CaseParams:
     # 0 wall, 1 inlet where porosity >= 0.9, 2 outlet where porosity >= 0.9.
     # Default = the reference's compile-time choice (ibm_3d_air_condition_omp_cpu.f90:10-15).
     faces: Tuple[int, int, int, int, int, int] = (1, 0, 0, 0, 0, 2)
+    # 2D backward-facing-step executable: inlet and initial velocity times porosity
+    # (ibm_2d_backstep_omp_cpu.f90:533-534, 615-618)
+    inlet_porosity: bool = False
 
     @property
     def dx(self) -> float:
@@ -265,6 +268,22 @@ def cylinder_2d(m: int, n: int, width: float, height: float, center: Tuple[float
     return csv_round(tanh_profile(d, interface_cells))
 
 
+def backstep_2d(m: int, n: int, step_len: int, step_height: int, wall: int = 4,
+                interface_cells: float = 2.0) -> np.ndarray:
+    """Backward-facing step in a channel, the layout of the reference's test/backstep.zip
+    porosity: a solid band ``wall`` cells thick across the middle of y (with the periodic
+    y wrap it is the channel's top and bottom wall) that is ``step_height`` cells thicker
+    over the first ``step_len`` cells in x; tanh interface; numpy (n, m)."""
+    I, J = np.meshgrid(np.arange(m, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="xy")
+    y0 = n // 2 - wall // 2 - 0.5            # band rows [y0 + 0.5, y1 - 0.5]
+    y1 = y0 + wall
+    d_band = np.maximum(y0 - J, J - y1)      # signed distance, positive in the fluid
+    dx = I - (step_len - 0.5)
+    dy = np.maximum(y0 - J, J - (y1 + step_height))
+    d_block = np.hypot(np.maximum(dx, 0.0), np.maximum(dy, 0.0)) + np.minimum(np.maximum(dx, dy), 0.0)
+    return csv_round(tanh_profile(np.minimum(d_band, d_block), interface_cells))
+
+
 def sphere_3d(m: int, n: int, l: int, width: float, height: float, depth: float,
               center: Tuple[float, float, float], diameter: float, interface_cells: float = 2.0) -> np.ndarray:
     """Sphere (solid) in a fluid box; numpy shape (l, n, m)."""
@@ -293,9 +312,9 @@ def random_boxes_3d(m: int, n: int, l: int, nboxes: int, side_range: Tuple[int, 
 
 def random_circles_2d(m: int, n: int, count: int, radius_range: Tuple[float, float], seed: int,
                       interface_cells: float = 2.0, max_tries: int = 2000, downstream_clear: float = 0.0,
-                      max_blockage: float = 1.0) -> np.ndarray:
+                      max_blockage: float = 1.0, gap: float = 2.0) -> np.ndarray:
     """1..8 style obstacle field: ``count`` non-overlapping circles (radii in cells), tanh
-    interface; numpy (n, m).  Circles keep a 2-cell gap and stay clear of the x faces;
+    interface; numpy (n, m).  Circles keep a ``gap``-cell gap and stay clear of the x faces;
     ``downstream_clear`` (a fraction of m) keeps that much of the domain before the outlet
     free of solid, and ``max_blockage`` bounds the fraction of any grid column inside
     circles (the reference's outlet condition and central scheme do not survive a wake
@@ -314,7 +333,7 @@ def random_circles_2d(m: int, n: int, count: int, radius_range: Tuple[float, flo
             break
         cx = float(rng.uniform(r + 4.0, x_hi - r)) if x_hi - 2 * r - 4.0 > 0 else min(m / 2.0, x_hi - r)
         cy = float(rng.uniform(r + 2.0, n - r - 2.0)) if n - 2 * r - 4.0 > 0 else n / 2.0
-        if not all(math.hypot(cx - a, cy - b) > r + c + 2.0 for a, b, c in circles):
+        if not all(math.hypot(cx - a, cy - b) > r + c + gap for a, b, c in circles):
             continue
         chord = 2.0 * np.sqrt(np.maximum(r * r - (xs - cx) ** 2, 0.0))
         if max_blockage < 1.0 and np.max(blocked + chord) > max_blockage * n:
@@ -405,8 +424,9 @@ def stable_dt(dx: float, xnue: float, velocity: float = 1.0, dim: int = 2, cfl: 
     """Largest dt the reference's explicit step tolerates with margin: its convection is
     2nd-order central and its update forward Euler (ibm_2d_uniform_omp_cpu.f90:202-208), so
     besides CFL = U*dt/dx <= cfl it needs U^2*dt/xnue <= 1 (central advection under forward
-    Euler) and a diffusion number xnue*dt/dx^2 <= 0.4/dim."""
-    return min(cfl * dx / velocity, xnue / velocity ** 2, 0.4 / dim * dx * dx / xnue)
+    Euler) and a diffusion number xnue*dt/dx^2 <= 0.2/dim (at 0.4/dim, cases with obstacles
+    near each other already diverge)."""
+    return min(cfl * dx / velocity, xnue / velocity ** 2, 0.2 / dim * dx * dx / xnue)
 
 
 BATCH_RE_LENGTH_CELLS = 400.0
@@ -415,7 +435,8 @@ BATCH_RE_LENGTH_CELLS = 400.0
 def batch_cases_2d(count: int = 512, steps: int = 1000, seed: int = 11, iter_max: int = 60,
                    cfl: float = 0.2) -> List[Tuple[CaseParams, np.ndarray]]:
     """Independent 2D cases: sizes uniform from BATCH_SIZES_2D, 1-8 circles with radii 3-20
-    cells (in the upstream 65% of the domain, at most half of any column blocked), Re uniform in 50-400 via xnue on a length of BATCH_RE_LENGTH_CELLS cells
+    cells (in the upstream 65% of the domain, 4-cell gaps, at most 35% of any column
+    blocked), Re uniform in 50-400 via xnue on a length of BATCH_RE_LENGTH_CELLS cells
     (xnue = U * 400 dx / Re, a cell Reynolds number U dx / xnue of 0.125-1), dt from
     :func:`stable_dt` (CFL 0.2 capped by the central-scheme limits), SOR 60 iterations,
     omega = 1.7.
@@ -432,7 +453,7 @@ def batch_cases_2d(count: int = 512, steps: int = 1000, seed: int = 11, iter_max
         height = dx * (n - 1)
         nobs = int(rng.integers(1, 9))
         phi = random_circles_2d(m, n, nobs, (3.0, 20.0), seed=int(rng.integers(0, 2**31 - 1)),
-                                downstream_clear=0.35, max_blockage=0.5)
+                                downstream_clear=0.35, max_blockage=0.35, gap=4.0)
         re = float(rng.uniform(50.0, 400.0))
         xnue = 1.0 * BATCH_RE_LENGTH_CELLS * dx / re
         dt = stable_dt(dx, xnue, 1.0, 2, cfl)

@@ -26,6 +26,7 @@ struct pf_ctx {
     // natural layout
     double *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     double *phi = nullptr, *div = nullptr, *dout = nullptr;
+    double *force_buf = nullptr;   // body force: per-block partial sums, then the 6 totals
     // colour-split layout (2 * HS doubles each)
     double *P = nullptr, *BB = nullptr, *AW = nullptr, *AS = nullptr, *AB = nullptr, *PHIs = nullptr;
     // SOR sweep kernel: 0 stored face coefficients, 1 faces rebuilt from porosity with IEEE
@@ -96,7 +97,7 @@ static int alloc(pf_ctx *c, double **p, int64_t n) {
 
 static void release(pf_ctx *c) {
     double **all[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->phi, &c->div, &c->dout,
-                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs};
+                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs, &c->force_buf};
     for (double **p : all)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (c->err) { (void)hipFree(c->err); c->err = nullptr; }
@@ -305,11 +306,18 @@ extern "C" int pf_initial(pf_ctx *c) {
     const pf_desc &d = c->d;
     double u0, v0;
     pf::initial_values(d, u0, v0);
-    std::vector<double> nat, spl;
+    std::vector<double> nat, spl, phi;
+    if (d.inlet_porosity) {
+        phi.resize((size_t)g.NS);
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        PF_TRY_HIP(c, hipMemcpy(phi.data(), c->phi, phi.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
     auto fill = [&](double val, bool split) { pf::fill_interior(g, val, split, split ? spl : nat); };
     fill(u0, false);
+    if (d.inlet_porosity) pf::scale_interior(g, phi, nat);
     PF_TRY_HIP(c, hipMemcpy(c->u, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
     fill(v0, false);
+    if (d.inlet_porosity) pf::scale_interior(g, phi, nat);
     PF_TRY_HIP(c, hipMemcpy(c->v, nat.data(), nat.size() * sizeof(double), hipMemcpyHostToDevice));
     if (g.dim == 3) {
         fill(0.0, false);
@@ -554,6 +562,21 @@ extern "C" int pf_divergence(pf_ctx *c, double *out) {
     PF_TRY_HIP(c, hipGetLastError());
     const size_t n = (size_t)g.m * g.n * g.l;
     PF_TRY_HIP(c, hipMemcpyAsync(out, c->dout, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+    return PF_OK;
+}
+
+extern "C" int pf_force(pf_ctx *c, double *out) {
+    if (!c || !out) return PF_EINVAL;
+    if (!c->have_init) return fail(c, PF_ESTATE, "pf_force before pf_initial");
+    const PfGeom &g = c->g;
+    const int nb = pf::force_blocks(g);
+    if (!c->force_buf)
+        PF_TRY_HIP(c, hipMalloc((void **)&c->force_buf, sizeof(double) * ((size_t)nb * 6 + 6)));
+    double *tot = c->force_buf + (size_t)nb * 6;
+    pf::launch_force(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P, c->force_buf, tot);
+    PF_TRY_HIP(c, hipGetLastError());
+    PF_TRY_HIP(c, hipMemcpyAsync(out, tot, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     PF_TRY_HIP(c, hipStreamSynchronize(c->st));
     return PF_OK;
 }

@@ -98,7 +98,7 @@ __device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, 
         } else if (OP == NAT_BOUNDARY_UNIFORM) {
             const bool bnd = (i == 0 || i == 1 || i == g.m + 1 || j == 0 || j == g.n + 1 ||
                               (g.dim == 3 && (k == 0 || k == g.l + 1)));
-            if (bnd) boundary_uniform_cell(g, C.a, C.U[p], C.V[p], C.W[p], C.P, i, j, k);
+            if (bnd) boundary_uniform_cell(g, C.a, C.phi, C.U[p], C.V[p], C.W[p], C.P, i, j, k);
         } else if (OP == NAT_BOUNDARY_AIR) {
             int rr = -1, qq = -1;
             if (aux == 0 && k == g.l) { rr = i; qq = j; }
@@ -608,8 +608,10 @@ extern "C" int pf_batch_add(pf_batch *b, const pf_desc *d, const double *raw, do
     pf::initial_values(*d, u0, v0);
     std::vector<double> buf;
     pf::fill_interior(g, u0, false, buf);
+    if (d->inlet_porosity) pf::scale_interior(g, phi, buf);
     PFB_TRY(b, hipMemcpy(bc.U[q], buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
     pf::fill_interior(g, v0, false, buf);
+    if (d->inlet_porosity) pf::scale_interior(g, phi, buf);
     PFB_TRY(b, hipMemcpy(bc.V[q], buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
     if (g.dim == 3) {
         pf::fill_interior(g, 0.0, false, buf);

@@ -281,7 +281,8 @@ __device__ __forceinline__ void project_cell(const PfGeom &g, const StepArgs &a,
         w[o] = w[o] - (((f * (P[pf_soff(g, i, j, k + 1)] - P[pf_soff(g, i, j, k - 1)])) / a.dz) * 0.5);
 }
 
-__device__ __forceinline__ void boundary_uniform_cell(const PfGeom &g, const StepArgs &a, double *__restrict__ u,
+__device__ __forceinline__ void boundary_uniform_cell(const PfGeom &g, const StepArgs &a,
+                                                      const double *__restrict__ po, double *__restrict__ u,
                                                       double *__restrict__ v, double *__restrict__ w,
                                                       double *__restrict__ P, int i, int j, int k) {
     const int m = g.m, n = g.n, l = g.l;
@@ -291,6 +292,11 @@ __device__ __forceinline__ void boundary_uniform_cell(const PfGeom &g, const Ste
     double uu, vv, ww, pp;
     if (i == 0 || i == 1) {
         uu = a.u_in; vv = a.v_in; ww = 0.;
+        if (a.inlet_porosity) {      // u(1,j) = U cos * porosity(1,j); u(0,j) = u(1,j)
+            const double f = po[pf_off(g, 1, jj, kk)];
+            uu = uu * f;
+            vv = vv * f;
+        }
         pp = (i == 0) ? P[pf_soff(g, 2, jj, kk)] : P[pf_soff(g, 1, jj, kk)];
     } else if (i == m + 1) {
         if (m - 1 == 1) { uu = a.u_in; vv = a.v_in; ww = 0.; }

@@ -22,6 +22,10 @@ inline bool desc_valid(const pf_desc *d, std::string &why) {
     for (int f = 0; f < 6; ++f)
         if (d->face[f] < 0 || d->face[f] > 2) { why = "face codes must be 0, 1 or 2"; return false; }
     if (d->iter_max < 0) { why = "iter_max < 0"; return false; }
+    if (d->inlet_porosity && (d->dim != 2 || d->bc != PF_BC_UNIFORM)) {
+        why = "inlet_porosity is the 2D uniform-inflow (backward step) variant";
+        return false;
+    }
     if (d->precision != PF_PREC_FP64 && d->precision != PF_PREC_MIXED) { why = "precision must be FP64 or MIXED"; return false; }
     if (d->inner_iters < 0 || d->max_cycles < 0 || !(d->residual_tol >= 0)) {
         why = "inner_iters, max_cycles >= 0 and residual_tol >= 0";
@@ -42,6 +46,7 @@ inline StepArgs make_step_args(const pf_desc &d) {
     StepArgs a{};
     a.bc = d.bc;
     a.nonslip = d.nonslip ? 1 : 0;
+    a.inlet_porosity = d.inlet_porosity ? 1 : 0;
     for (int f = 0; f < 6; ++f) a.face[f] = d.face[f];
     a.dt = d.dt;
     a.dx = d.dx;
@@ -119,6 +124,19 @@ inline void fill_interior(const PfGeom &g, double val, bool split, std::vector<d
         for (int j = 1; j <= g.n; ++j)
             for (int i = 1; i <= g.m; ++i)
                 buf[(size_t)(split ? pf_soff(g, i, j, k) : pf_off(g, i, j, k))] = val;
+}
+
+// Backward-step initial velocity: U*cos(AoA/180*pi)*porosity(i,j)
+// (ibm_2d_backstep_omp_cpu.f90:615-618): the uniform value times each interior cell's
+// porosity, in that order.  buf and phi are in the natural layout.
+inline void scale_interior(const PfGeom &g, const std::vector<double> &phi, std::vector<double> &buf) {
+    const int kmin = (g.dim == 3) ? 1 : 0, kmax = (g.dim == 3) ? g.l : 0;
+    for (int k = kmin; k <= kmax; ++k)
+        for (int j = 1; j <= g.n; ++j)
+            for (int i = 1; i <= g.m; ++i) {
+                const size_t o = (size_t)pf_off(g, i, j, k);
+                buf[o] = buf[o] * phi[o];
+            }
 }
 
 // device layout <-> reference host storage (Fortran order, leading dimensions ldi, ldj)

@@ -16,6 +16,7 @@ namespace pf {
 struct StepArgs {
     int bc;             // PF_BC_UNIFORM / PF_BC_AIR
     int nonslip;
+    int inlet_porosity; // inlet velocity scaled by the inlet cell's porosity (2D backward step)
     int face[6];        // air face codes: top, bottom, east, west, north, south
     double dt, dx, dy, dz;
     double rdx, rdy, rdz;         // RN(1/dx) etc. for the checked constant-divisor path
@@ -61,6 +62,10 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
 int sor_blocks(const PfGeom &g);
+// Body force (pressure + wall-force terms): partial holds force_blocks(g)*6 doubles, out 6.
+int force_blocks(const PfGeom &g);
+void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,
+                  const double *v, const double *w, const double *P, double *partial, double *out);
 void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n);
 // mixed precision (pf_mixed.hip)
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,

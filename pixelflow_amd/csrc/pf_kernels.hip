@@ -324,8 +324,8 @@ __global__ void k_project(PfGeom g, StepArgs a, const double *__restrict__ P, do
 // a ghost index).
 
 
-__global__ void k_boundary_uniform(PfGeom g, StepArgs a, double *__restrict__ u, double *__restrict__ v,
-                                   double *__restrict__ w, double *__restrict__ P) {
+__global__ void k_boundary_uniform(PfGeom g, StepArgs a, const double *__restrict__ po, double *__restrict__ u,
+                                   double *__restrict__ v, double *__restrict__ w, double *__restrict__ P) {
     const int m = g.m, n = g.n, l = g.l;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t Y = g.Y, Z = g.Z, X = m + 2;
@@ -355,7 +355,7 @@ __global__ void k_boundary_uniform(PfGeom g, StepArgs a, double *__restrict__ u,
         return;
     }
     if (g.dim == 2) k = 0;
-    boundary_uniform_cell(g, a, u, v, w, P, i, j, k);
+    boundary_uniform_cell(g, a, po, u, v, w, P, i, j, k);
 }
 
 // boundary of the air-conditioning solver, one face per launch in the reference's order
@@ -380,6 +380,79 @@ __global__ void k_divergence_out(PfGeom g, StepArgs a, const double *__restrict_
     const int k = (g.dim == 3) ? (int)blockIdx.z + 1 : 0;
     if (i > g.m || j > g.n) return;
     divout_cell(g, a, po, u, v, w, out, i, j, k);
+}
+
+// ------------------------------------------------------------------------------------
+// Force on the immersed body (output_force_log_2d, output.f90:264-284; output_force_log_3d,
+// :1118-1145): per interior cell the pressure term -dV*p*2*phi*(1-phi)/(thickness*d)*n and
+// the wall-force term dV*32*rho*nu*(phi*(1-phi)/(thickness*d))**2*u, each in the Fortran
+// evaluation order.  The sum runs in a fixed tree (each thread strides over cells, then a
+// block tree, then k_force_final over the blocks), so it is deterministic but not the
+// reference's serial order: results agree to rounding, not bit for bit.
+// partial[b*6 + q]: q = 0..2 pressure x, y, z; 3..5 viscous x, y, z.
+// ------------------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(256) void k_force(PfGeom g, StepArgs a, const double *__restrict__ po,
+                                               const double *__restrict__ u, const double *__restrict__ v,
+                                               const double *__restrict__ w, const double *__restrict__ P,
+                                               double *__restrict__ partial) {
+    const int64_t cells = (int64_t)g.m * g.n * (DIM == 3 ? g.l : 1);
+    const double dx = a.dx, dy = a.dy, dz = a.dz, th = a.thickness;
+    const double alpha = 32.0, rho = a.density, nu = a.xnue;
+    double f[6] = {0., 0., 0., 0., 0., 0.};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % g.m) + 1;
+        const int64_t r = c / g.m;
+        const int j = (int)(r % g.n) + 1;
+        const int k = (DIM == 3) ? (int)(r / g.n) + 1 : 0;
+        const int64_t o = pf_off(g, i, j, k);
+        const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+        const double gx = (po[o + 1] - po[o - 1]) * 0.5;
+        const double gy = (po[o + sy] - po[o - sy]) * 0.5;
+        const double gz = (DIM == 3) ? (po[o + sz] - po[o - sz]) * 0.5 : 0.;
+        const double na = (DIM == 3) ? sqrt(((gx * gx) + (gy * gy)) + (gz * gz)) : sqrt((gx * gx) + (gy * gy));
+        const double den = fmax_ref(na, PF_SMALL);
+        const double ph = po[o], q = 1.0 - ph;
+        const double pc = P[pf_soff(g, i, j, k)];
+        const double dv = (DIM == 3) ? ((dx * dy) * dz) : (dx * dy);
+        const double base = (((dv * pc) * 2) * ph) * q;
+        const double cv = ((dv * alpha) * rho) * nu;
+        const double tx = (ph * q) / (th * dx), ty = (ph * q) / (th * dy);
+        f[0] += -((base / (th * dx)) * (gx / den));
+        f[1] += -((base / (th * dy)) * (gy / den));
+        f[3] += (cv * (tx * tx)) * u[o];
+        f[4] += (cv * (ty * ty)) * v[o];
+        if (DIM == 3) {
+            const double tz = (ph * q) / (th * dz);
+            f[2] += -((base / (th * dz)) * (gz / den));
+            f[5] += (cv * (tz * tz)) * w[o];
+        }
+    }
+    __shared__ double red[6][256];
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = f[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[(int64_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_force_final(const double *__restrict__ partial, int nblocks,
+                                                     double *__restrict__ out) {
+    double f[6] = {0., 0., 0., 0., 0., 0.};
+    for (int b = (int)threadIdx.x; b < nblocks; b += 256)
+        for (int q = 0; q < 6; ++q) f[q] += partial[(int64_t)b * 6 + q];
+    __shared__ double red[6][256];
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = f[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Clear the per-step diagnostics with a kernel (graph-capture safe, unlike a memset node).
@@ -499,7 +572,8 @@ void launch_boundary(hipStream_t st, const PfGeom &g, const StepArgs &a, const d
         const int64_t X = g.m + 2;
         const int64_t total = 3 * (int64_t)g.Y * g.Z + 2 * X * g.Z + (g.dim == 3 ? 2 * X * g.Y : 0);
         const int b = 256;
-        hipLaunchKernelGGL(k_boundary_uniform, dim3((unsigned)((total + b - 1) / b)), dim3(b), 0, st, g, a, u, v, w, P);
+        hipLaunchKernelGGL(k_boundary_uniform, dim3((unsigned)((total + b - 1) / b)), dim3(b), 0, st, g, a, phi, u, v,
+                           w, P);
         return;
     }
     dim3 b(64, 4, 1);
@@ -517,6 +591,22 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out) {
     dim3 b(64, 4, 1);
     hipLaunchKernelGGL(k_divergence_out, grid2(g.m, g.n, g.dim == 3 ? g.l : 1, b), b, 0, st, g, a, phi, u, v, w, out);
+}
+
+int force_blocks(const PfGeom &g) {
+    const int64_t cells = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
+    const int64_t b = (cells + 255) / 256;
+    return (int)(b < 1024 ? b : 1024);
+}
+
+void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,
+                  const double *v, const double *w, const double *P, double *partial, double *out) {
+    const int nb = force_blocks(g);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_force<3>, dim3((unsigned)nb), dim3(256), 0, st, g, a, phi, u, v, w, P, partial);
+    else
+        hipLaunchKernelGGL(k_force<2>, dim3((unsigned)nb), dim3(256), 0, st, g, a, phi, u, v, w, P, partial);
+    hipLaunchKernelGGL(k_force_final, dim3(1), dim3(256), 0, st, partial, nb, out);
 }
 
 int sor_blocks(const PfGeom &g) {

@@ -32,7 +32,7 @@ class PfDesc(ctypes.Structure):
         ("dim", ctypes.c_int32), ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("l", ctypes.c_int32),
         ("bc", ctypes.c_int32), ("nonslip", ctypes.c_int32), ("iter_max", ctypes.c_int32),
         ("face", ctypes.c_int32 * 6), ("precision", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
-        ("max_cycles", ctypes.c_int32), ("report_residual", ctypes.c_int32),
+        ("max_cycles", ctypes.c_int32), ("report_residual", ctypes.c_int32), ("inlet_porosity", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
         ("xnue", ctypes.c_double), ("xlambda", ctypes.c_double), ("density", ctypes.c_double),
         ("thickness", ctypes.c_double), ("inlet_velocity", ctypes.c_double),
@@ -59,7 +59,7 @@ _lib: Optional[ctypes.CDLL] = None
 
 # Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
 EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
-           "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_footprint_bytes", "pf_sync",
+           "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_force", "pf_footprint_bytes", "pf_sync",
            "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_probe_copy",
            "pf_batch_create", "pf_batch_destroy", "pf_batch_last_error", "pf_batch_case_bytes", "pf_batch_free_bytes",
            "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
@@ -98,6 +98,7 @@ def lib() -> ctypes.CDLL:
     L.pf_batch_destroy.argtypes = [vp]
     L.pf_batch_last_error.restype = ctypes.c_char_p
     L.pf_batch_last_error.argtypes = [vp]
+    L.pf_force.argtypes = [vp, dp]
     L.pf_batch_case_bytes.restype = ctypes.c_int64
     L.pf_batch_case_bytes.argtypes = [ctypes.POINTER(PfDesc)]
     L.pf_batch_free_bytes.restype = ctypes.c_int64
@@ -131,6 +132,7 @@ def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0
     d.inner_iters = inner_iters
     d.max_cycles = max_cycles
     d.report_residual = 1 if report_residual else 0
+    d.inlet_porosity = 1 if getattr(p, "inlet_porosity", False) else 0
     d.dx, d.dy, d.dz, d.dt = p.dx, p.dy, (p.dz if p.dim == 3 else 1.0), p.dt
     d.xnue, d.xlambda, d.density, d.thickness = p.xnue, p.xlambda, p.density, p.thickness
     d.inlet_velocity, d.outlet_pressure, d.aoa = p.inlet_velocity, p.outlet_pressure, p.aoa
@@ -204,6 +206,12 @@ class Solver:
     def divergence(self) -> np.ndarray:
         out = np.zeros(self.p.interior_shape(), dtype=np.float64)
         self._check(lib().pf_divergence(self._h, _dp(out)), "pf_divergence")
+        return out
+
+    def force(self) -> np.ndarray:
+        """[Fp_x, Fp_y, Fp_z, Fv_x, Fv_y, Fv_z] on the body for the current fields."""
+        out = np.zeros(6, dtype=np.float64)
+        self._check(lib().pf_force(self._h, _dp(out)), "pf_force")
         return out
 
     def sync(self) -> None:

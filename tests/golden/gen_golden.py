@@ -59,8 +59,11 @@ def parse_sections(path: str, dtype=np.float64):
     return {k: np.array([float(t) for t in v], dtype=dtype) for k, v in secs.items()}
 
 
-def run_reference(p: cases.CaseParams, phi: np.ndarray, flavour: str = "fp64", csv_text_path: str = None):
-    exe = os.path.join(REF_BIN, flavour, MAINS[(p.bc, p.dim)])
+def run_reference(p: cases.CaseParams, phi: np.ndarray, flavour: str = "fp64", csv_text_path: str = None,
+                  main: str = None, threads: str = None):
+    if main is None:
+        main = "ref_ibm_2d_backstep" if p.inlet_porosity else MAINS[(p.bc, p.dim)]
+    exe = os.path.join(REF_BIN, flavour, main)
     if not os.path.exists(exe):
         raise SystemExit(f"missing {exe}: run `make -C oracle ref` first")
     work = tempfile.mkdtemp(prefix="pf_golden_")
@@ -71,9 +74,12 @@ def run_reference(p: cases.CaseParams, phi: np.ndarray, flavour: str = "fp64", c
             shutil.copy(csv_text_path, os.path.join(work, p.csv_file))
         else:
             cases.write_porosity_csv(os.path.join(work, p.csv_file), phi)
-        env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"))
+        env = dict(os.environ, OMP_NUM_THREADS=threads or os.environ.get("OMP_NUM_THREADS", "8"))
         out = subprocess.run([exe], cwd=work, capture_output=True, text=True, env=env, check=True)
         errs = np.array([float(x) for x in re.findall(r"p error:\s*(\S+)", out.stdout)])
+        # output_force_log_2d lines (drag executable): "Fp = fx fy", "Fv = fx fy"
+        fp = re.findall(r"Fp =\s*(\S+)\s+(\S+)", out.stdout)
+        fv = re.findall(r"Fv =\s*(\S+)\s+(\S+)", out.stdout)
         dt = np.float32 if flavour == "fp32" else np.float64
         sol = parse_sections(os.path.join(work, "etc", "solution_uvp.dat"), dt)
         dv = parse_sections(os.path.join(work, "etc", "divergent.dat"), dt)
@@ -84,6 +90,9 @@ def run_reference(p: cases.CaseParams, phi: np.ndarray, flavour: str = "fp64", c
                "p": sol["pressure P_all"].reshape(sh), "div": dv["divergent velocity"].reshape(sh)}
         if p.dim == 3:
             res["w"] = sol["velocity w_inst"].reshape(sh)
+        if fp:
+            res["force_p"] = np.array([[float(a), float(b)] for a, b in fp])
+            res["force_v"] = np.array([[float(a), float(b)] for a, b in fv])
         return res, work
     except Exception:
         shutil.rmtree(work, ignore_errors=True)
@@ -155,6 +164,11 @@ def case_list():
     # Binary porosity (threshold clamp, 1e-6 faces) as in the boxes workload.
     p = C(dim=3, m=32, n=16, l=16, width=1.0, height=0.48, depth=0.48, time=4 * 5e-4, istep_max=4, iter_max=40, xnue=5e-4)
     L.append(("g3d_boxes", p, cases.random_boxes_3d(32, 16, 16, 6, (3, 8), seed=5)))
+    # Backward-facing step (ibm_2d_backstep): inlet and initial velocity times porosity; a
+    # channel wall band with a step at the inlet, tanh interface.
+    p = C(dim=2, m=48, n=24, width=0.94, height=0.46, time=8 * 1e-3, istep_max=8, iter_max=40, xnue=4e-2,
+          inlet_porosity=True)
+    L.append(("g2d_backstep", p, cases.backstep_2d(48, 24, 12, 4, 4)))
     # Air-conditioning solver on a synthetic room (stable dt), reference face codes.
     p, phi = stable_room(24, 22, 20, seed=3, steps=5, iter_max=40)
     L.append(("g3d_air_room24", p, phi))
@@ -194,6 +208,17 @@ def main():
         save_full(name, p, res, extra)
         shutil.rmtree(work, ignore_errors=True)
         print("wrote", name, flush=True)
+    if not args.only or args.only == "g2d_drag":
+        # Drag executable (ibm_2d_drag): the uniform step plus output_force_log_2d every step,
+        # run on one thread so its reduction is the serial loop order.
+        C = cases.CaseParams
+        p = C(dim=2, m=48, n=24, width=0.47, height=0.23, time=10 * 1e-3, istep_max=10, xnue=1e-3, radius=0.04,
+              iter_max=50)
+        phi = cases.cylinder_2d(48, 24, 0.47, 0.23, (0.15, 0.115), 0.08)
+        res, work = run_reference(p, phi, main="ref_ibm_2d_drag", threads="1")
+        save_full("g2d_drag", p, res)
+        shutil.rmtree(work, ignore_errors=True)
+        print("wrote g2d_drag", flush=True)
     if not args.only or args.only == "g3d_ref_room64":
         p, csv, tmp = reference_room_case()
         res, work = run_reference(p, None, csv_text_path=csv)

@@ -132,3 +132,54 @@ def test_gpu_default_sweep_is_checked_fast_path():
     p, phi = cases.sphere_case(m=64, n=32, l=32, steps=1)
     with _solver(p, phi) as s:
         assert s.sor_kernel() == 2
+
+
+def _force_scale(p, phi, pp, u, v, w):
+    """Sum over cells of |term| for each of the 6 force components (numpy, any order): the
+    scale that bounds the rounding difference between two summation orders."""
+    sl = (slice(1, -1),) * p.dim
+    def d(a, ax):
+        return (np.roll(a, -1, axis=ax) - np.roll(a, 1, axis=ax))[sl] * 0.5
+    axes = (1, 0) if p.dim == 2 else (2, 1, 0)
+    g = [d(phi, ax) for ax in axes]
+    na = np.sqrt(sum(x * x for x in g))
+    den = np.maximum(na, 1e-6)
+    ph = phi[sl]
+    q = 1.0 - ph
+    dv = p.dx * p.dy * (p.dz if p.dim == 3 else 1.0)
+    hs = (p.dx, p.dy, p.dz)
+    vel = (u, v, w)
+    out = np.zeros(6)
+    for c in range(p.dim):
+        out[c] = np.abs(dv * pp[sl] * 2 * ph * q / (p.thickness * hs[c]) * g[c] / den).sum()
+        t = ph * q / (p.thickness * hs[c])
+        out[3 + c] = np.abs(dv * 32.0 * p.density * p.xnue * t * t * vel[c][sl]).sum()
+    return out
+
+
+def test_gpu_force_matches_reference_drag_log():
+    """The drag executable's per-step force log (reference run on one thread): every step,
+    to the rounding bound of a different summation order."""
+    p, g = load_golden("g2d_drag")
+    with _solver(p, g["phi_raw"]) as s:
+        for st in range(p.istep_max):
+            s.step(1)
+            f = s.force()
+            scale = _force_scale(p, s.field("porosity"), s.field("p"), s.field("u"), s.field("v"), None)
+            ref = np.concatenate([g["force_p"][st], [0.0], g["force_v"][st], [0.0]])
+            tol = 1e-13 * scale + 1e-300
+            assert np.all(np.abs(f - ref) <= tol), (st, f, ref, tol)
+
+
+def test_gpu_force_3d_matches_oracle():
+    import pyoracle
+    p, phi = cases.sphere_case(m=48, n=24, l=24, steps=3)
+    o = pyoracle.OracleRun(p, phi)
+    o.step(3)
+    ref = o.force()
+    with _solver(p, phi) as s:
+        s.step(3)
+        f = s.force()
+        scale = _force_scale(p, s.field("porosity"), s.field("p"), s.field("u"), s.field("v"), s.field("w"))
+    assert np.all(np.abs(f - ref) <= 1e-13 * scale), (f, ref, scale)
+    assert abs(ref[0]) > 0 and abs(ref[3]) > 0

@@ -92,3 +92,37 @@ def test_fp32_build_is_close_but_not_equal():
         a64, a32 = g[f], g[f + "_fp32"].astype(np.float64)
         rel = np.linalg.norm(a32 - a64) / np.linalg.norm(a64)
         assert 1e-9 < rel < 1e-3, (f, rel)
+
+
+def test_oracle_force_matches_reference_drag_log():
+    """output_force_log_2d of the drag executable (run on one thread, so its reduction is
+    the serial loop order): every step's pressure and viscous force, bit for bit."""
+    p, g = load_golden("g2d_drag")
+    o = pyoracle.OracleRun(p, g["phi_raw"])
+    for s in range(p.istep_max):
+        o.step(1)
+        f = o.force()
+        np.testing.assert_array_equal(f[[0, 1]], g["force_p"][s], err_msg=f"step {s + 1} Fp")
+        np.testing.assert_array_equal(f[[3, 4]], g["force_v"][s], err_msg=f"step {s + 1} Fv")
+        assert f[2] == 0.0 and f[5] == 0.0
+
+
+def test_force_closed_forms():
+    """Pressure force of a constant pressure over a mirror-symmetric body is zero in x
+    (equal and opposite faces); the viscous force is linear in velocity; without an
+    interface (porosity 1) both vanish."""
+    from pixelflow_amd import cases
+    p = cases.CaseParams(dim=2, m=40, n=20, width=1.0, height=0.487, time=1e-3, istep_max=1, iter_max=5)
+    phi = cases.cylinder_2d(40, 20, 1.0, 0.487, (0.5, 0.2435), 0.2)
+    o = pyoracle.OracleRun(p, phi)
+    o.p_[...] = 2.0
+    o.u[...] = 1.0
+    o.v[...] = 0.0
+    f1 = o.force()
+    assert abs(f1[0]) < 1e-12 * np.abs(f1).max() + 1e-300
+    o.u[...] = 3.0
+    f3 = o.force()
+    np.testing.assert_allclose(f3[3], 3.0 * f1[3], rtol=1e-14)
+    flat = pyoracle.OracleRun(p, np.ones((20, 40)))
+    flat.p_[...] = 5.0
+    assert np.all(flat.force() == 0.0)
