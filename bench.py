@@ -153,16 +153,25 @@ def main(argv=None):
     s.sync()
     wall = time.perf_counter() - t0
     sweep_us, launches = s.last_sweep_time_us()
+    fused = s.sor_fused()
     s.close()
-    log(f"[bench] {args.steps} steps in {wall:.3f} s; SOR sweep avg {sweep_us:.2f} us over {launches} launches")
+    log(f"[bench] {args.steps} steps in {wall:.3f} s; colour sweep avg {sweep_us:.2f} us over {launches} sweeps"
+        f" ({'fused: one launch per iteration' if fused else 'one launch per colour sweep'})")
     from pixelflow_amd.solver import probe_copy_gbps
     copy_gbps = {"128MB": probe_copy_gbps(128 << 20, 20), "2GB": probe_copy_gbps(2048 << 20, 10)}
 
     cells = p.cells
     value = cells * args.steps / wall
-    sweep_bytes = 3 * 8 * cells
-    achieved = sweep_bytes / (sweep_us * 1e-6) / 1e9 if sweep_us > 0 else 0.0
-    step_bytes = 8 * ((2 * 3 + 2) + 6 * p.iter_max + (2 * 3 + 1)) * cells
+    if fused:
+        # one launch = one red/black iteration: p read and written once, bb and porosity
+        # read once (4 FP64 words per cell)
+        kernel, launch_us, launch_bytes = "k_sor_fused (one red/black SOR iteration)", 2 * sweep_us, 4 * 8 * cells
+    else:
+        # one launch = one colour: 3 FP64 words per cell
+        kernel, launch_us, launch_bytes = "k_sor_phi (red/black colour sweep)", sweep_us, 3 * 8 * cells
+    achieved = launch_bytes / (launch_us * 1e-6) / 1e9 if launch_us > 0 else 0.0
+    sor_words = 4 if fused else 6   # per SOR iteration and cell: fused launch / two colour sweeps
+    step_bytes = 8 * ((2 * 3 + 2) + sor_words * p.iter_max + (2 * 3 + 1)) * cells
     res = {
         "metric": METRIC,
         "value": value,
@@ -184,14 +193,14 @@ def main(argv=None):
         "device_copy_GBps": copy_gbps,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_sor_phi (red/black colour sweep)",
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_sor_phi", workload),
-            "bytes_per_launch_algorithmic": sweep_bytes,
-            "avg_launch_us": sweep_us,
+            "traffic": measured_traffic(kernel.split()[0], workload),
+            "bytes_per_launch_algorithmic": launch_bytes,
+            "avg_launch_us": launch_us,
         },
     }
     if args.mixed_steps > 0:

@@ -124,8 +124,9 @@ int pf_sync(pf_ctx *ctx);
 /* Measurement hooks (no reference counterpart: the reference prints wall-clock stamps
  * only, src/omp_parallel/lib/utils.f90:7-16).  With timing on, pf_step brackets each
  * step's SOR sweeps with HIP events on the context's stream; pf_last_sweep_time_us then
- * gives the average time per colour-sweep launch of the last pf_step (gaps between the
- * back-to-back launches included) and the number of launches it covers. */
+ * gives the average time per colour sweep (half an SOR iteration; a fused launch does
+ * two) of the last pf_step, gaps between the back-to-back launches included, and the
+ * number of colour sweeps it covers. */
 int pf_set_timing(pf_ctx *ctx, int on);
 /* Which colour-sweep kernel the context uses (0 stored face coefficients, 1 faces rebuilt
  * from porosity with IEEE division, 2 the same with the constant-divisor division that
@@ -134,6 +135,10 @@ int pf_sor_kernel(const pf_ctx *ctx);
 /* Bandwidth calibration (no reference counterpart): a plain device-to-device copy of
  * `bytes`, `iters` times; *gbps = (read + written bytes) / second. */
 int pf_probe_copy(int64_t bytes, int iters, double *gbps);
+/* 1 if the context runs each SOR iteration as one fused launch (both colours, ping-pong
+ * buffers; even periodic extents; opt-in with PF_FUSED=1 in the environment), 0 if as two
+ * colour-sweep launches (the default, faster on MI355X).  Same results. */
+int pf_sor_fused(const pf_ctx *ctx);
 int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
 
 /* ------------------------------------------------------------------------------------

@@ -27,6 +27,9 @@ struct pf_ctx {
     double *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     double *phi = nullptr, *div = nullptr, *dout = nullptr;
     double *force_buf = nullptr;   // body force: per-block partial sums, then the 6 totals
+    double *P2 = nullptr;          // fused sweep: second pressure buffer (ping-pong)
+    float *E2 = nullptr;           // fused sweep, mixed precision: second correction buffer
+    bool fused = false;            // one launch per SOR iteration (PF_FUSED=1; see pf_fused.h)
     // colour-split layout (2 * HS doubles each)
     double *P = nullptr, *BB = nullptr, *AW = nullptr, *AS = nullptr, *AB = nullptr, *PHIs = nullptr;
     // SOR sweep kernel: 0 stored face coefficients, 1 faces rebuilt from porosity with IEEE
@@ -97,14 +100,14 @@ static int alloc(pf_ctx *c, double **p, int64_t n) {
 
 static void release(pf_ctx *c) {
     double **all[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->phi, &c->div, &c->dout,
-                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs, &c->force_buf};
+                      &c->P, &c->BB, &c->AW, &c->AS, &c->AB, &c->PHIs, &c->force_buf, &c->P2};
     for (double **p : all)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (c->err) { (void)hipFree(c->err); c->err = nullptr; }
     if (c->err_part) { (void)hipFree(c->err_part); c->err_part = nullptr; }
     if (c->res_part) { (void)hipFree(c->res_part); c->res_part = nullptr; }
     if (c->res_val) { (void)hipFree(c->res_val); c->res_val = nullptr; }
-    float **f32[] = {&c->E32, &c->R32, &c->PHI32};
+    float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2};
     for (float **p : f32)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
@@ -160,9 +163,16 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     }
     pf::launch_clear_u64(c->st, c->err_part, c->nparts);
     pf::launch_clear_u64(c->st, c->res_part, c->nparts);
+    {
+        const char *fv = std::getenv("PF_FUSED");
+        if (fv && *fv) c->fused = std::atoi(fv) != 0;
+        c->fused = c->fused && pf::sor_fused_ok(g, desc->bc);
+    }
+    if (c->fused && desc->precision == PF_PREC_FP64 && (rc = alloc(c, &c->P2, 2 * g.HS)) != PF_OK) goto bad;
     if (desc->precision == PF_PREC_MIXED) {
-        float **f32[] = {&c->E32, &c->R32, &c->PHI32};
+        float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2};
         for (float **p : f32) {
+            if (p == &c->E2 && !c->fused) continue;
             if (hipMalloc((void **)p, sizeof(float) * (size_t)(2 * g.HS)) != hipSuccess) {
                 rc = fail(c, PF_ENOMEM, "hipMalloc of the FP32 arrays failed");
                 goto bad;
@@ -344,7 +354,19 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
     const int rz = periodic && g.dim == 3 && !fz;
     // 3D sweeps (i+j+k) even first (:493), 2D sweeps (i+j) odd first (:339)
     const int first = (g.dim == 3) ? 0 : 1;
-    for (int it = 0; it < iters; ++it) {
+    int done = 0;
+    if (c->fused && c->sor_variant != 0 && iters >= 2) {
+        // pairs of fused iterations P -> P2 -> P; P2 takes P's ghost values first (the
+        // x ghosts and the air room's faces are not written by the sweep)
+        (void)hipMemcpyAsync(c->P2, c->P, sizeof(double) * (size_t)(2 * g.HS), hipMemcpyDeviceToDevice, c->st);
+        const bool fast = (c->sor_variant == 2);
+        const int track = 1;   // 3D tracks the second colour inside the kernel, 2D both
+        for (; done + 2 <= iters; done += 2) {
+            pf::launch_sor_fused(c->st, g, c->a, c->P, c->P2, c->PHIs, c->BB, c->phi, slot, track, fast);
+            pf::launch_sor_fused(c->st, g, c->a, c->P2, c->P, c->PHIs, c->BB, c->phi, slot, track, fast);
+        }
+    }
+    for (int it = done; it < iters; ++it) {
         for (int half = 0; half < 2; ++half) {
             pf::SweepArgs s;
             s.colour = half == 0 ? first : 1 - first;
@@ -412,7 +434,15 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
             if ((rc = residual(false)) != PF_OK) return rc;
         }
         const int kk = tol_mode ? K : (remaining < K ? remaining : K);
-        for (int it = 0; it < kk; ++it) {
+        int done = 0;
+        if (c->fused && kk >= 2) {
+            // E2's non-periodic ghosts are 0 like E's (zeroed at allocation, never written)
+            for (; done + 2 <= kk; done += 2) {
+                pf::launch_sor32_fused(c->st, g, c->a, c->E32, c->E2, c->PHI32, c->R32, c->phi, slot, 1);
+                pf::launch_sor32_fused(c->st, g, c->a, c->E2, c->E32, c->PHI32, c->R32, c->phi, slot, 1);
+            }
+        }
+        for (int it = done; it < kk; ++it) {
             for (int half = 0; half < 2; ++half) {
                 pf::SweepArgs s{};
                 s.colour = half == 0 ? first : 1 - first;
@@ -579,6 +609,12 @@ extern "C" int pf_force(pf_ctx *c, double *out) {
     PF_TRY_HIP(c, hipMemcpyAsync(out, tot, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     PF_TRY_HIP(c, hipStreamSynchronize(c->st));
     return PF_OK;
+}
+
+extern "C" int pf_sor_fused(const pf_ctx *c) {
+    if (!c) return PF_EINVAL;
+    if (!c->fused) return 0;
+    return (c->d.precision == PF_PREC_MIXED || c->sor_variant != 0) ? 1 : 0;
 }
 
 extern "C" int pf_set_timing(pf_ctx *c, int on) {

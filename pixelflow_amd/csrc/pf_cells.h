@@ -217,6 +217,39 @@ __device__ __forceinline__ void rhs_cell(const PfGeom &g, const StepArgs &a, con
     BB[pf_soff(g, i, j, k)] = bb;
 }
 
+// The SOR update of one cell from the values around it: pressure of the cell and its six
+// neighbours, porosity of the same seven cells, the RHS (3D :492-516 / :549-572, 2D
+// :339-352 / :373-386).  Shared by the two-launch sweep below and the fused sweep.
+template <int DIM, bool FAST>
+__device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, int i, int j, int k, double pc,
+                                            double pW, double pE, double pS, double pN, double pB, double pT,
+                                            double fc, double fW, double fE, double fS, double fN, double fB,
+                                            double fT, double bb, const double *__restrict__ phi) {
+    const double dt = a.dt;
+    Coef c;
+    c.ae = face_any<FAST>(fE, fc, dt, a.dx, a.rdx);
+    c.aw = face_any<FAST>(fc, fW, dt, a.dx, a.rdx);
+    c.an = face_any<FAST>(fN, fc, dt, a.dy, a.rdy);
+    c.as = face_any<FAST>(fc, fS, dt, a.dy, a.rdy);
+    if (DIM == 3) {
+        c.at = face_any<FAST>(fT, fc, dt, a.dz, a.rdz);
+        c.ab = face_any<FAST>(fc, fB, dt, a.dz, a.rdz);
+    } else {
+        c.at = 0.;
+        c.ab = 0.;
+    }
+    double ap;
+    if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
+    else ap = (((-c.ae) - c.aw) - c.an) - c.as;
+    if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    double num;
+    if (DIM == 3)
+        num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
+    else
+        num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
+    return ((num / ap) * a.omega) + (pc * a.omega_c);
+}
+
 template <int DIM, bool FAST>
 __device__ __forceinline__ double sor_phi_cell(const PfGeom &g, const StepArgs &a, const SweepArgs &s, int i, int j,
                                                int k, int sh, int64_t o, const double *__restrict__ Poth,
@@ -230,34 +263,15 @@ __device__ __forceinline__ double sor_phi_cell(const PfGeom &g, const StepArgs &
     const double fc = PHIo[o];
     const double fW = PHIx[o - 1 + sh], fE = PHIx[o + sh];
     const double fS = PHIx[o - HP], fN = PHIx[o + HP];
-    const double dt = a.dt;
-    Coef c;
-    c.ae = face_any<FAST>(fE, fc, dt, a.dx, a.rdx);
-    c.aw = face_any<FAST>(fc, fW, dt, a.dx, a.rdx);
-    c.an = face_any<FAST>(fN, fc, dt, a.dy, a.rdy);
-    c.as = face_any<FAST>(fc, fS, dt, a.dy, a.rdy);
-    double pB = 0., pT = 0.;
+    double pB = 0., pT = 0., fB = 0., fT = 0.;
     if (DIM == 3) {
         pB = Poth[o - HPY];
         pT = Poth[o + HPY];
-        const double fB = PHIx[o - HPY], fT = PHIx[o + HPY];
-        c.at = face_any<FAST>(fT, fc, dt, a.dz, a.rdz);
-        c.ab = face_any<FAST>(fc, fB, dt, a.dz, a.rdz);
-    } else {
-        c.at = 0.;
-        c.ab = 0.;
+        fB = PHIx[o - HPY];
+        fT = PHIx[o + HPY];
     }
-    double ap;
-    if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
-    else ap = (((-c.ae) - c.aw) - c.an) - c.as;
-    if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
-    const double bb = BBo[o];
-    double num;
-    if (DIM == 3)
-        num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
-    else
-        num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
-    const double pn = ((num / ap) * a.omega) + (pc * a.omega_c);
+    const double pn = sor_value<DIM, FAST>(g, a, i, j, k, pc, pW, pE, pS, pN, pB, pT, fc, fW, fE, fS, fN, fB, fT,
+                                           BBo[o], phi);
     Pown[o] = pn;
     if (s.fuse_y) {
         if (j == g.n) Pown[o - HP * (int64_t)g.n] = pn;
@@ -543,6 +557,34 @@ __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs 
     return fabs(r / ap);
 }
 
+// FP32 SOR update of the correction e on A e = r (mixed precision), from the values
+// around the cell; shared by the two-launch and the fused sweep.
+template <int DIM>
+__device__ __forceinline__ float sor32_value(const PfGeom &g, const StepArgs &a, int i, int j, int k, float ec,
+                                             float eW, float eE, float eS, float eN, float eB, float eT, float fc,
+                                             float fW, float fE, float fS, float fN, float fB, float fT, float rr,
+                                             const double *__restrict__ phi, float cdx, float cdy, float cdz) {
+    CoefT<float> c;
+    c.ae = face32(fE, fc, cdx);
+    c.aw = face32(fc, fW, cdx);
+    c.an = face32(fN, fc, cdy);
+    c.as = face32(fc, fS, cdy);
+    c.at = 0.f;
+    c.ab = 0.f;
+    if (DIM == 3) {
+        c.at = face32(fT, fc, cdz);
+        c.ab = face32(fc, fB, cdz);
+    }
+    const float ap = -(c.ae + c.aw + c.an + c.as + c.at + c.ab);
+    if (is_edge_row(g, a, i, j, k)) row_rules<false, float>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    float num = rr - c.ae * eE - c.aw * eW - c.an * eN - c.as * eS;
+    if (DIM == 3) num = num - c.at * eT - c.ab * eB;
+    const float om = (float)a.omega;
+    // the approximate reciprocal measured more accurate end-to-end than IEEE division here:
+    // the error of a fixed-count FP32 iterate is set by rounding inside the iteration
+    return num * __builtin_amdgcn_rcpf(ap) * om + ec * (1.f - om);
+}
+
 template <int DIM>
 __device__ __forceinline__ double sor32_cell(const PfGeom &g, const StepArgs &a, const SweepArgs &s, int i, int j,
                                              int k, int sh, int64_t o, const float *__restrict__ Eoth,
@@ -552,29 +594,15 @@ __device__ __forceinline__ double sor32_cell(const PfGeom &g, const StepArgs &a,
     const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
     const float ec = Eown[o];
     const float eW = Eoth[o - 1 + sh], eE = Eoth[o + sh], eS = Eoth[o - HP], eN = Eoth[o + HP];
-    const float fc = F32o[o];
-    CoefT<float> c;
-    c.ae = face32(F32x[o + sh], fc, cdx);
-    c.aw = face32(fc, F32x[o - 1 + sh], cdx);
-    c.an = face32(F32x[o + HP], fc, cdy);
-    c.as = face32(fc, F32x[o - HP], cdy);
-    float eB = 0.f, eT = 0.f;
-    c.at = 0.f;
-    c.ab = 0.f;
+    float eB = 0.f, eT = 0.f, fB = 0.f, fT = 0.f;
     if (DIM == 3) {
         eB = Eoth[o - HPY];
         eT = Eoth[o + HPY];
-        c.at = face32(F32x[o + HPY], fc, cdz);
-        c.ab = face32(fc, F32x[o - HPY], cdz);
+        fB = F32x[o - HPY];
+        fT = F32x[o + HPY];
     }
-    const float ap = -(c.ae + c.aw + c.an + c.as + c.at + c.ab);
-    if (is_edge_row(g, a, i, j, k)) row_rules<false, float>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
-    float num = Ro[o] - c.ae * eE - c.aw * eW - c.an * eN - c.as * eS;
-    if (DIM == 3) num = num - c.at * eT - c.ab * eB;
-    const float om = (float)a.omega;
-    // the approximate reciprocal measured more accurate end-to-end than IEEE division here:
-    // the error of a fixed-count FP32 iterate is set by rounding inside the iteration
-    const float en = num * __builtin_amdgcn_rcpf(ap) * om + ec * (1.f - om);
+    const float en = sor32_value<DIM>(g, a, i, j, k, ec, eW, eE, eS, eN, eB, eT, F32o[o], F32x[o - 1 + sh],
+                                      F32x[o + sh], F32x[o - HP], F32x[o + HP], fB, fT, Ro[o], phi, cdx, cdy, cdz);
     Eown[o] = en;
     if (s.fuse_y) {
         if (j == g.n) Eown[o - HP * (int64_t)g.n] = en;

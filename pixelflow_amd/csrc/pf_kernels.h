@@ -62,6 +62,14 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
 int sor_blocks(const PfGeom &g);
+// One red/black iteration in one launch, ping-pong A -> B (pf_fused.h); usable when
+// sor_fused_ok (even periodic extents).  B must hold A's non-periodic ghost values.
+bool sor_fused_ok(const PfGeom &g, int bc);
+void launch_sor_fused(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *A, double *B,
+                      const double *F, const double *R, const double *phi, unsigned long long *slot, int track,
+                      bool fast);
+void launch_sor32_fused(hipStream_t st, const PfGeom &g, const StepArgs &a, const float *A, float *B, const float *F,
+                        const float *R, const double *phi, unsigned long long *slot, int track);
 // Body force (pressure + wall-force terms): partial holds force_blocks(g)*6 doubles, out 6.
 int force_blocks(const PfGeom &g);
 void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *phi, const double *u,

@@ -8,6 +8,7 @@
 
 #include "pf_cells.h"
 #include "pf_device.h"
+#include "pf_fused.h"
 #include "pf_kernels.h"
 
 namespace pf {
@@ -455,6 +456,20 @@ __global__ __launch_bounds__(256) void k_force_final(const double *__restrict__ 
     if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// One red/black SOR iteration in one launch (pf_fused.h): reads A, writes B.  One block per
+// tile, XCD-contiguous tile order; the per-block running max of |dp| goes to err_slot.
+template <int DIM, int KIND, bool FAST>
+__global__ __launch_bounds__(256) void k_sor_fused(PfGeom g, StepArgs a, FusedGeom fg,
+                                                   FusedArgs<typename FusedT<KIND>::T> p,
+                                                   unsigned long long *err_slot, int track) {
+    const int nb = (int)gridDim.x;
+    const int L = xcd_logical_block((int)blockIdx.x, nb);
+    int x0, j0, k0, k1;
+    fused_tile_coords(g, fg, L, x0, j0, k0, k1);
+    const double e = fused_tile<DIM, KIND, FAST>(g, a, fg, p, x0, j0, k0, k1);
+    if (track) fused_block_max(e, err_slot + L);
+}
+
 // Clear the per-step diagnostics with a kernel (graph-capture safe, unlike a memset node).
 __global__ void k_clear_u64(unsigned long long *p, int count) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -610,8 +625,8 @@ void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const doub
 }
 
 int sor_blocks(const PfGeom &g) {
-    // slots for every kernel that reduces into them: stored-face sweep, KB = 1, 2, 4 sweeps
-    // and the one-cell-per-thread residual
+    // slots for every kernel that reduces into them: stored-face sweep, KB = 1, 2, 4 sweeps,
+    // the one-cell-per-thread residual and the fused sweep
     dim3 b(64, 4, 1);
     dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
     int n = (int)(gr.x * gr.y * gr.z);
@@ -619,7 +634,36 @@ int sor_blocks(const PfGeom &g) {
         const int v = sweep_shape(g, kb).blocks();
         n = v > n ? v : n;
     }
-    return n;
+    const FusedGeom f = fused_geom(g, 1);
+    const int nf = f.gx * f.gy * f.gz;
+    return nf > n ? nf : n;
+}
+
+bool sor_fused_ok(const PfGeom &g, int bc) { return fused_ok(g, bc == PF_BC_UNIFORM); }
+
+void launch_sor_fused(hipStream_t st, const PfGeom &g, const StepArgs &a, const double *A, double *B,
+                      const double *F, const double *R, const double *phi, unsigned long long *slot, int track,
+                      bool fast) {
+    const FusedGeom f = fused_geom(g, a.bc == PF_BC_UNIFORM);
+    const dim3 gr((unsigned)(f.gx * f.gy * f.gz));
+    FusedArgs<double> p{A, B, F, R, phi, 0.f, 0.f, 0.f};
+    if (g.dim == 3) {
+        if (fast) hipLaunchKernelGGL((k_sor_fused<3, 0, true>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
+        else hipLaunchKernelGGL((k_sor_fused<3, 0, false>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
+    } else {
+        if (fast) hipLaunchKernelGGL((k_sor_fused<2, 0, true>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
+        else hipLaunchKernelGGL((k_sor_fused<2, 0, false>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
+    }
+}
+
+void launch_sor32_fused(hipStream_t st, const PfGeom &g, const StepArgs &a, const float *A, float *B, const float *F,
+                        const float *R, const double *phi, unsigned long long *slot, int track) {
+    const FusedGeom f = fused_geom(g, a.bc == PF_BC_UNIFORM);
+    const dim3 gr((unsigned)(f.gx * f.gy * f.gz));
+    FusedArgs<float> p{A, B, F, R, phi, (float)(a.dt / (a.dx * a.dx)), (float)(a.dt / (a.dy * a.dy)),
+                       (float)(a.dt / (a.dz * a.dz))};
+    if (g.dim == 3) hipLaunchKernelGGL((k_sor_fused<3, 1, true>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
+    else hipLaunchKernelGGL((k_sor_fused<2, 1, true>), gr, dim3(256), 0, st, g, a, f, p, slot, track);
 }
 
 void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err) {

@@ -78,6 +78,7 @@ class Server:
         self.steps_run = 0
         self.cell_steps = 0
         self.sor_cell_updates = 0
+        self.busy_s = 0.0                # wall time inside batch steps (the device is busy)
 
     def footprint(self, r: Request) -> int:
         return self.batch.case_bytes(r.params)
@@ -117,7 +118,9 @@ class Server:
             return
         todo = min(r.steps - self.batch.case_steps(s) for s, r in self.running.items())
         n = max(1, min(self.chunk, todo))
+        t0 = time.perf_counter()
         st = self.batch.step(n, raise_diverged=False)
+        self.busy_s += time.perf_counter() - t0
         for s, r in self.running.items():
             r.last_stats = {k: float(v[-1, s]) for k, v in st.items()}
             self.cell_steps += r.params.cells * n

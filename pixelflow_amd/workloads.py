@@ -125,6 +125,7 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
             plain_upd += r.params.cells * r.params.iter_max * plain_steps
     plain = plain_upd / plain_time
     served = srv.sor_cell_updates / wall
+    served_busy = srv.sor_cell_updates / max(srv.busy_s, 1e-9)
     res = [r.last_stats["residual"] for r in done if r.last_stats]
     its = [r.last_stats["sor_iters"] for r in done if r.last_stats]
     return {
@@ -134,6 +135,8 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
         **stats, "wall_s": wall, "cell_steps_per_s": srv.cell_steps / wall,
         "sor_cell_updates_per_s": served, "plain_fp64_sor_cell_updates_per_s": plain,
         "speedup_vs_plain_gpu": served / plain,
+        "busy_s": srv.busy_s, "sor_cell_updates_per_busy_s": served_busy,
+        "speedup_vs_plain_gpu_busy": served_busy / plain,
         "plain_sample": f"{plain_sample} of the rooms, one at a time, {plain_steps} steps each",
         "final_residual_median": float(np.median(res)) if res else None,
         "final_step_iterations_median": float(np.median(its)) if its else None,

@@ -183,3 +183,59 @@ def test_gpu_force_3d_matches_oracle():
         scale = _force_scale(p, s.field("porosity"), s.field("p"), s.field("u"), s.field("v"), s.field("w"))
     assert np.all(np.abs(f - ref) <= 1e-13 * scale), (f, ref, scale)
     assert abs(ref[0]) > 0 and abs(ref[3]) > 0
+
+
+def test_gpu_boxes_512x256x256_one_step_matches_oracle():
+    """The 512x256x256 boxes grid (40 seeded boxes, tanh voxel interface), one full step
+    of 100 SOR iterations: bit-identical to the oracle."""
+    import pyoracle
+    p, phi = cases.boxes_case(steps=1)
+    print("[gpu] full-size 512x256x256 boxes, 1 step", flush=True)
+    o = pyoracle.OracleRun(p, phi)
+    e_ref = o.step(1)
+    with _solver(p, phi) as s:
+        st = s.step(1)
+        for f in ("u", "v", "w", "p"):
+            np.testing.assert_array_equal(s.interior(f), o.interior(f), err_msg=f)
+        np.testing.assert_array_equal(st["sor_error"], e_ref)
+    o.close()
+
+
+FUSED_CASES = ["g2d_cyl48", "g2d_c1_10", "g2d_backstep", "g3d_sph32", "g3d_stage_i1_s1", "g3d_weird", "g3d_air_room24",
+               "g3d_boxes", "g3d_odd_32_17_15"]
+
+
+@pytest.mark.parametrize("name", FUSED_CASES)
+def test_gpu_fused_sweep_matches_reference(name, monkeypatch):
+    """One launch per SOR iteration (both colours, ping-pong buffers) against the
+    reference fixture, and against the two-launch sweep: the same bits."""
+    p, g = load_golden(name)
+    res = {}
+    for fused in (1, 0):
+        monkeypatch.setenv("PF_FUSED", str(fused))
+        with _solver(p, g["phi_raw"]) as s:
+            expect = fused == 1 and (p.bc == "air" or (p.n % 2 == 0 and (p.dim == 2 or p.l % 2 == 0)))
+            assert s.sor_fused() == expect, name
+            st = s.step(p.istep_max)
+            res[fused] = ({f: s.interior(f) for f in _fields(p)}, st["sor_error"])
+    for f in _fields(p):
+        np.testing.assert_array_equal(res[1][0][f], g[f], err_msg=f"{name}:{f} fused")
+        np.testing.assert_array_equal(res[1][0][f], res[0][0][f], err_msg=f"{name}:{f}")
+    np.testing.assert_array_equal(res[1][1], g["errs"])
+
+
+@pytest.mark.parametrize("name", ["3d_sphere_64x32x32", "3d_air_room_40x36x32", "2d_c1_256x128"])
+def test_gpu_fused_mixed_matches_two_launch(name, monkeypatch):
+    """FP32 fused iterations give the bits of the two-launch FP32 sweeps."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, phi = CASES[name]()
+    out = {}
+    for fused in (1, 0):
+        monkeypatch.setenv("PF_FUSED", str(fused))
+        with Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=10) as s:
+            assert s.sor_fused() == bool(fused)
+            st = s.step(p.istep_max)
+            out[fused] = ({f: s.interior(f) for f in _fields(p)}, st["sor_error"])
+    for f in _fields(p):
+        np.testing.assert_array_equal(out[1][0][f], out[0][0][f], err_msg=f"{name}:{f}")
+    np.testing.assert_array_equal(out[1][1], out[0][1])
