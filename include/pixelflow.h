@@ -183,6 +183,42 @@ int64_t pf_batch_case_steps(const pf_batch *b, int slot);
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int pf_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 
+/* ------------------------------------------------------------------------------------
+ * Serving: continuous batching over one resident batch, driven by a worker thread of the
+ * library (no reference counterpart).  Requests submitted from any host thread are
+ * admitted in arrival order as slots and arena memory free up, advanced in shared step
+ * chunks with the cases already resident, and retired when their step count is reached
+ * (the fields asked for are kept in host memory) or when their step became non-finite.
+ * Every result equals the case run alone.  Times are seconds since pf_serve_create.
+ * ------------------------------------------------------------------------------------ */
+enum pf_serve_state { PF_SERVE_QUEUED = 0, PF_SERVE_RUNNING = 1, PF_SERVE_DONE = 2, PF_SERVE_FAILED = 3 };
+typedef struct pf_server pf_server;
+
+/* kind and max_cases as for pf_batch_create; arena_bytes <= 0: half the free device memory;
+ * chunk: steps advanced between admissions. */
+int pf_serve_create(const pf_desc *kind, int max_cases, int64_t arena_bytes, int chunk, pf_server **out);
+int pf_serve_destroy(pf_server *s);
+/* The server's batch (read-only use: pf_batch_free_bytes, pf_batch_get_config; a
+ * pf_batch_set_config is safe only while no request is queued or running). */
+pf_batch *pf_serve_batch(pf_server *s);
+/* fields: bit PF_FIELD_* set = keep that field for pf_serve_result.  The porosity is copied. */
+int pf_serve_submit(pf_server *s, const pf_desc *desc, const double *raw_interior, double threshold, int64_t nsteps,
+                    int32_t fields, int64_t *request_id);
+/* state (pf_serve_state), steps taken so far, SOR iterations over those steps, the
+ * statistics of its latest step (pointers other than state may be NULL). */
+int pf_serve_poll(pf_server *s, int64_t id, int *state, int64_t *steps_done, int64_t *sor_iters, pf_stats *last);
+/* Block until the request is done or failed, or timeout_s (< 0: no limit) passes. */
+int pf_serve_wait(pf_server *s, int64_t id, double timeout_s, int *state);
+int pf_serve_times(pf_server *s, int64_t id, double *t_submit, double *t_admit, double *t_finish);
+/* A kept field of a done request, in the reference's host storage (see top). */
+int pf_serve_result(pf_server *s, int64_t id, int field, double *host, int64_t ldi, int64_t ldj);
+/* Forget a finished request and its fields. */
+int pf_serve_release(pf_server *s, int64_t id);
+/* Steps run, seconds spent inside batch steps, requests queued and running. */
+int pf_serve_counters(pf_server *s, int64_t *steps_run, double *busy_s, int *queued, int *running);
+/* Why a request failed (valid until pf_serve_release). */
+const char *pf_serve_error(pf_server *s, int64_t id);
+
 #ifdef __cplusplus
 }
 #endif

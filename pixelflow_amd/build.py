@@ -25,7 +25,7 @@ OUT = os.path.join(PKG, "_build")
 LIB = os.path.join(OUT, "libpixelflow.so")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pf_kernels.hip", "pf_mixed.hip", "pf_api.hip", "pf_batch.hip"]
+HIP_SOURCES = ["pf_kernels.hip", "pf_mixed.hip", "pf_api.hip", "pf_batch.hip", "pf_serve.cpp"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,7 +64,7 @@ def build_lib(jobs: int = 4) -> str:
         if p.wait() != 0:
             raise subprocess.CalledProcessError(p.returncode, "hipcc")
     tmp = LIB + ".tmp"
-    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-pthread", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

@@ -64,7 +64,9 @@ EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set
            "pf_batch_create", "pf_batch_destroy", "pf_batch_last_error", "pf_batch_case_bytes", "pf_batch_free_bytes",
            "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
            "pf_batch_divergence", "pf_batch_set_config", "pf_batch_get_config", "pf_batch_sync", "pf_batch_case_steps",
-           "pf_device_memory")
+           "pf_device_memory", "pf_serve_create", "pf_serve_destroy", "pf_serve_batch", "pf_serve_submit",
+           "pf_serve_poll", "pf_serve_wait", "pf_serve_times", "pf_serve_result", "pf_serve_release",
+           "pf_serve_counters", "pf_serve_error")
 
 
 def lib() -> ctypes.CDLL:
@@ -116,6 +118,22 @@ def lib() -> ctypes.CDLL:
     L.pf_batch_case_steps.restype = ctypes.c_int64
     L.pf_batch_case_steps.argtypes = [vp, ctypes.c_int]
     L.pf_device_memory.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.pf_serve_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.pf_serve_destroy.argtypes = [vp]
+    L.pf_serve_batch.restype = vp
+    L.pf_serve_batch.argtypes = [vp]
+    L.pf_serve_submit.argtypes = [vp, ctypes.POINTER(PfDesc), dp, ctypes.c_double, ctypes.c_int64, ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int64)]
+    L.pf_serve_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(PfStats)]
+    L.pf_serve_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.pf_serve_times.argtypes = [vp, ctypes.c_int64, dp, dp, dp]
+    L.pf_serve_result.argtypes = [vp, ctypes.c_int64, ctypes.c_int, dp, i64, i64]
+    L.pf_serve_release.argtypes = [vp, ctypes.c_int64]
+    L.pf_serve_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.pf_serve_error.restype = ctypes.c_char_p
+    L.pf_serve_error.argtypes = [vp, ctypes.c_int64]
     _lib = L
     return L
 
@@ -372,6 +390,114 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().pf_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SERVE_QUEUED, SERVE_RUNNING, SERVE_DONE, SERVE_FAILED = 0, 1, 2, 3
+
+
+class NativeServer:
+    """The library's continuous-batching server (pf_serve_*): a worker thread inside the
+    library admits submitted cases as slots and arena memory free up and advances them
+    in shared step chunks; results are kept per request until released."""
+
+    def __init__(self, kind: CaseParams, max_cases: int, arena_bytes: int = 0, chunk: int = 5,
+                 precision: int = PF_PREC_FP64, batch_config: Optional[dict] = None, **kw):
+        self.kw = dict(precision=precision, **kw)
+        d = make_desc(kind, **self.kw)
+        h = ctypes.c_void_p()
+        rc = lib().pf_serve_create(ctypes.byref(d), int(max_cases), int(arena_bytes), int(chunk), ctypes.byref(h))
+        if rc != 0:
+            raise PixelFlowError(f"pf_serve_create: {ERRORS.get(rc, rc)}: {lib().pf_batch_last_error(None).decode()}")
+        self._h = h
+        self.params: Dict[int, CaseParams] = {}
+        if batch_config:
+            c = PfBatchConfig(int(batch_config.get("persistent_blocks", 0)), 1 if batch_config.get("use_graph", True) else 0,
+                              int(batch_config.get("inner_iters", 0)), int(batch_config.get("tile_x", 64)))
+            rc = lib().pf_batch_set_config(lib().pf_serve_batch(self._h), ctypes.byref(c))
+            if rc != 0:
+                raise PixelFlowError(f"pf_batch_set_config: {ERRORS.get(rc, rc)}")
+
+    def _check(self, rc: int, what: str, rid: int = -1) -> None:
+        if rc != 0:
+            raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {lib().pf_serve_error(self._h, rid).decode()}")
+
+    def case_bytes(self, p: CaseParams) -> int:
+        return batch_case_bytes(p, **self.kw)
+
+    def submit(self, p: CaseParams, phi_raw_interior: np.ndarray, nsteps: int, fields=("u", "v", "w", "p")) -> int:
+        d = make_desc(p, **self.kw)
+        raw = np.ascontiguousarray(phi_raw_interior, dtype=np.float64).reshape(-1)
+        if raw.size != p.cells:
+            raise ValueError("porosity size does not match the grid")
+        mask = 0
+        for f in fields:
+            if not (f == "w" and p.dim == 2):
+                mask |= 1 << FIELDS[f]
+        rid = ctypes.c_int64()
+        self._check(lib().pf_serve_submit(self._h, ctypes.byref(d), _dp(raw), float(p.threshold), int(nsteps), mask,
+                                          ctypes.byref(rid)), "pf_serve_submit")
+        self.params[rid.value] = p
+        return rid.value
+
+    def poll(self, rid: int):
+        """(state, steps done, SOR iterations over them, statistics of the latest step)."""
+        st, done, its, last = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), PfStats()
+        self._check(lib().pf_serve_poll(self._h, int(rid), ctypes.byref(st), ctypes.byref(done), ctypes.byref(its),
+                                        ctypes.byref(last)), "pf_serve_poll", rid)
+        return st.value, done.value, its.value, {"sor_error": last.sor_error, "residual": last.residual,
+                                                 "sor_iters": last.sor_iters, "cycles": last.cycles}
+
+    def wait(self, rid: int, timeout: float = -1.0) -> int:
+        st = ctypes.c_int()
+        self._check(lib().pf_serve_wait(self._h, int(rid), float(timeout), ctypes.byref(st)), "pf_serve_wait", rid)
+        return st.value
+
+    def times(self, rid: int):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().pf_serve_times(self._h, int(rid), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "pf_serve_times", rid)
+        return a.value, b.value, c.value
+
+    def result(self, rid: int, name: str) -> np.ndarray:
+        p = self.params[rid]
+        out = np.zeros(p.full_shape(), dtype=np.float64)
+        self._check(lib().pf_serve_result(self._h, int(rid), FIELDS[name], _dp(out), p.m + 2, p.n + 2),
+                    "pf_serve_result", rid)
+        return out
+
+    def interior(self, rid: int, name: str) -> np.ndarray:
+        a = self.result(rid, name)
+        return a[1:-1, 1:-1, 1:-1].copy() if self.params[rid].dim == 3 else a[1:-1, 1:-1].copy()
+
+    def error(self, rid: int) -> str:
+        return lib().pf_serve_error(self._h, int(rid)).decode()
+
+    def release(self, rid: int) -> None:
+        self._check(lib().pf_serve_release(self._h, int(rid)), "pf_serve_release", rid)
+        self.params.pop(rid, None)
+
+    def counters(self) -> Dict[str, float]:
+        steps, busy, q, r = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().pf_serve_counters(self._h, ctypes.byref(steps), ctypes.byref(busy), ctypes.byref(q),
+                                            ctypes.byref(r)), "pf_serve_counters")
+        return {"steps_run": steps.value, "busy_s": busy.value, "queued": q.value, "running": r.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().pf_serve_destroy(self._h)
             self._h = None
 
     def __enter__(self):

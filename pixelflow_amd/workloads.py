@@ -14,8 +14,7 @@ from typing import Callable, Dict
 import numpy as np
 
 from . import autotune, cases
-from .serve import Request, Server, latency_stats
-from .solver import PF_PREC_MIXED, Batch, Solver, batch_case_bytes
+from .solver import PF_PREC_MIXED, Batch, Solver, batch_case_bytes, device_memory
 
 
 def _noop(*a):
@@ -88,8 +87,12 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
 
 
 def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
-                 inner: int = 25, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
+                 inner: int = 50, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
                  plain_sample: int = 3, plain_steps: int = 10, log: Callable = _noop) -> Dict[str, object]:
+    """Poisson-arriving room requests through the library's serving loop (pf_serve_*): a
+    host thread submits each request at its arrival time while the worker steps the
+    resident ones."""
+    from .solver import NativeServer, SERVE_DONE
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_per_s, size=n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -97,48 +100,63 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
     for r in range(n_requests):
         size = None if sizes is None else sizes[r % len(sizes)]
         p, phi = cases.room_case(int(rng.integers(0, 2**31 - 1)), steps=steps, iter_max=iter_max, size=size)
-        reqs.append(Request(r, p, phi, steps, float(arrivals[r])))
-    kind = reqs[0].params
+        reqs.append((p, phi))
+    kind = reqs[0][0]
     kw = dict(inner_iters=inner, residual_tol=tol, max_cycles=max(1, iter_max // inner))
-    peak = max(batch_case_bytes(r.params, precision=PF_PREC_MIXED, **kw) for r in reqs)
+    peak = max(batch_case_bytes(p, precision=PF_PREC_MIXED, **kw) for p, _ in reqs)
     log(f"[serve] {n_requests} requests, rate {rate_per_s}/s, {steps} steps each; largest case {peak / 2**20:.0f} MiB")
-    srv = Server(kind, PF_PREC_MIXED, max_cases=min(n_requests, 64), memory_fraction=0.3, chunk=chunk,
-                 keep_fields=(), **kw)
-    for r in reqs:
-        srv.submit(r)
-    t0 = time.perf_counter()
-    done = srv.run()
-    wall = time.perf_counter() - t0
-    srv.close()
-    stats = latency_stats(done)
+    free, _total = device_memory()
+    with NativeServer(kind, max_cases=min(n_requests, 64), arena_bytes=int(free * 0.3), chunk=chunk,
+                      precision=PF_PREC_MIXED, **kw) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for (p, phi), at in zip(reqs, arrivals):
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(p, phi, steps, fields=()))
+        states = [srv.wait(rid) for rid in ids]
+        wall = time.perf_counter() - t0
+        times = [srv.times(rid) for rid in ids]
+        polls = [srv.poll(rid) for rid in ids]
+        lasts = [q[3] for q in polls]
+        counters = srv.counters()
+        failed = sum(1 for s_ in states if s_ != SERVE_DONE)
+    ok = [t for t, s_ in zip(times, states) if s_ == SERVE_DONE]
+    tts = np.array([t[2] - t[0] for t in ok])
+    wait_s = np.array([t[1] - t[0] for t in ok])
+    span = max(t[2] for t in ok) - min(t[0] for t in ok) if ok else 0.0
+    upd = sum(p.cells * steps for p, _ in reqs)             # cell-steps
+    sor_upd = sum(p.cells * q[2] for (p, _), q in zip(reqs, polls))   # SOR cell-updates actually run
     # plain GPU path on a sample of the same rooms: one case at a time, FP64, iter_max
     # iterations every step (the reference's schedule)
     plain_upd, plain_time = 0, 0.0
-    for r in reqs[:plain_sample]:
-        with Solver(r.params, r.phi) as s:
+    for p, phi in reqs[:plain_sample]:
+        with Solver(p, phi) as s:
             s.step(2)
             s.sync()
-            t0 = time.perf_counter()
+            t1 = time.perf_counter()
             s.step(plain_steps)
             s.sync()
-            plain_time += time.perf_counter() - t0
-            plain_upd += r.params.cells * r.params.iter_max * plain_steps
+            plain_time += time.perf_counter() - t1
+            plain_upd += p.cells * p.iter_max * plain_steps
     plain = plain_upd / plain_time
-    served = srv.sor_cell_updates / wall
-    served_busy = srv.sor_cell_updates / max(srv.busy_s, 1e-9)
-    res = [r.last_stats["residual"] for r in done if r.last_stats]
-    its = [r.last_stats["sor_iters"] for r in done if r.last_stats]
+    served_busy = sor_upd / max(counters["busy_s"], 1e-9)
     return {
         "workload": f"served_rooms_{n_requests}_requests",
+        "server": "pf_serve (library worker thread, continuous batching)",
         "rate_per_s": rate_per_s, "steps_per_request": steps, "iter_cap_per_step": iter_max,
         "residual_tol": tol, "precision": "fp32 SOR + fp64 defect correction (tolerance mode, capped)",
-        **stats, "wall_s": wall, "cell_steps_per_s": srv.cell_steps / wall,
-        "sor_cell_updates_per_s": served, "plain_fp64_sor_cell_updates_per_s": plain,
-        "speedup_vs_plain_gpu": served / plain,
-        "busy_s": srv.busy_s, "sor_cell_updates_per_busy_s": served_busy,
+        "requests": len(ok), "failed": failed,
+        "p50_time_to_solution_s": float(np.percentile(tts, 50)) if len(tts) else None,
+        "p99_time_to_solution_s": float(np.percentile(tts, 99)) if len(tts) else None,
+        "mean_queue_wait_s": float(wait_s.mean()) if len(wait_s) else None,
+        "cases_per_s": len(ok) / span if span > 0 else 0.0, "span_s": span, "wall_s": wall,
+        "cell_steps_per_s": upd / wall, "busy_s": counters["busy_s"],
+        "sor_cell_updates_per_busy_s": served_busy, "plain_fp64_sor_cell_updates_per_s": plain,
         "speedup_vs_plain_gpu_busy": served_busy / plain,
         "plain_sample": f"{plain_sample} of the rooms, one at a time, {plain_steps} steps each",
-        "final_residual_median": float(np.median(res)) if res else None,
-        "final_step_iterations_median": float(np.median(its)) if its else None,
-        "sizes": sorted({f"{r.params.m}x{r.params.n}x{r.params.l}" for r in reqs}),
+        "final_residual_median": float(np.median([l["residual"] for l in lasts])),
+        "final_step_iterations_median": float(np.median([l["sor_iters"] for l in lasts])),
+        "sizes": sorted({f"{p.m}x{p.n}x{p.l}" for p, _ in reqs}),
     }

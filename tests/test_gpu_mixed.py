@@ -80,3 +80,15 @@ def test_mixed_tolerance_mode_matches_fp64_tolerance_mode():
             rel = np.linalg.norm(y - x) / np.linalg.norm(x)
             assert rel <= FIELD_TOL, (f, rel)
         print(f"[gpu] tolerance mode iterations fp64 {sa['sor_iters']} mixed {sb['sor_iters']}", flush=True)
+
+
+def test_mixed_boxes_512x256x256_full_size():
+    """The 512x256x256 boxes grid (40 boxes, tanh voxel interface) in mixed precision,
+    fixed count, 3 steps: the same bars as the smaller cases, at the full size."""
+    p, phi = cases.boxes_case(steps=3)
+    print("[gpu] mixed full-size 512x256x256 boxes, 3 steps", flush=True)
+    errs, ddiv, sa, sb = _pair(p, phi, 3, inner_iters=25, report_residual=True)
+    print(f"[gpu] mixed 512x256x256 rel-L2 {errs} max|ddiv| {ddiv:.3e}", flush=True)
+    assert all(e <= FIELD_TOL for f, e in errs.items() if f != "p"), errs
+    assert errs["p"] <= P_FIXED_TOL, errs
+    assert ddiv <= DIV_REL_TOL

@@ -97,3 +97,44 @@ def test_autotune_picks_a_config_and_keeps_results():
             ref = _alone(r.params, r.phi, done + 2)
             for f in _fields(r.params):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"case {r.rid} {f}")
+
+
+def test_native_server_results_equal_alone():
+    """The library's own serving loop (pf_serve_*, worker thread): requests submitted while
+    others run, an arena that holds about three cases so later ones wait for memory;
+    every result equals the case run alone, bit for bit."""
+    import time
+    from pixelflow_amd.solver import PF_PREC_MIXED, SERVE_DONE, NativeServer, batch_case_bytes
+    kw = dict(inner_iters=10)
+    reqs = _cyl_requests()
+    sizes = sorted(batch_case_bytes(r.params, precision=PF_PREC_MIXED, **kw) for r in reqs)
+    arena = sizes[-1] + sizes[-2] + sizes[-3]
+    with NativeServer(reqs[0].params, max_cases=4, arena_bytes=arena, chunk=2, precision=PF_PREC_MIXED, **kw) as srv:
+        ids = []
+        for r in reqs:
+            ids.append(srv.submit(r.params, r.phi, r.steps))
+            time.sleep(0.002)
+        for rid in ids:
+            assert srv.wait(rid, 60.0) == SERVE_DONE, srv.error(rid)
+        times = {rid: srv.times(rid) for rid in ids}
+        # at least one request was admitted only after another had finished (memory wait)
+        assert any(t[1] >= min(u[2] for q, u in times.items() if q != rid) for rid, t in times.items())
+        for r, rid in zip(reqs, ids):
+            ref = _alone(r.params, r.phi, r.steps, precision=PF_PREC_MIXED, **kw)
+            for f in _fields(r.params):
+                np.testing.assert_array_equal(srv.interior(rid, f), ref[f], err_msg=f"request {r.rid} {f}")
+            srv.release(rid)
+        c = srv.counters()
+        assert c["queued"] == 0 and c["running"] == 0 and c["steps_run"] > 0
+
+
+def test_native_server_rejects_case_larger_than_arena():
+    from pixelflow_amd.solver import SERVE_DONE, SERVE_FAILED, NativeServer, batch_case_bytes
+    reqs = _cyl_requests()
+    small = min(reqs, key=lambda r: r.params.cells)
+    big = max(reqs, key=lambda r: r.params.cells)
+    with NativeServer(small.params, max_cases=2, arena_bytes=batch_case_bytes(small.params) + 4096, chunk=2) as srv:
+        a = srv.submit(big.params, big.phi, 2)
+        b = srv.submit(small.params, small.phi, 2)
+        assert srv.wait(a, 60.0) == SERVE_FAILED and "arena" in srv.error(a)
+        assert srv.wait(b, 60.0) == SERVE_DONE
