@@ -368,7 +368,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
     }
     for (int it = done; it < iters; ++it) {
         for (int half = 0; half < 2; ++half) {
-            pf::SweepArgs s;
+            pf::SweepArgs s{};
             s.colour = half == 0 ? first : 1 - first;
             s.fuse_y = fy;
             s.fuse_z = fz;
