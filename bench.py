@@ -103,6 +103,62 @@ def mixed_comparison(p, phi, steps: int, warmup: int, inner: int, fp64_value: fl
             "mode": "fixed-count (iter_max FP32 SOR iterations in FP64 defect-correction cycles)"}
 
 
+def _timed_run(s, steps: int, warmup: int):
+    """Warm up, then time `steps` steps with the colour-sweep events on; returns
+    (wall seconds, average colour-sweep microseconds, statistics of the timed steps)."""
+    if warmup:
+        s.step(warmup)
+    s.set_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    st = s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    sweep_us, _ = s.last_sweep_time_us()
+    return wall, sweep_us, st
+
+
+def mixed_large(steps: int, warmup: int, inner: int):
+    """The 512x256x256 boxes workload (40 seeded boxes, sides 4-32 cells, tanh voxel
+    interface, U = 1), whose fields exceed the 256 MB last-level cache, in its stable form:
+    cell Reynolds number 1 (xnue = U dx, dt from the scheme's stability limits) and boxes
+    clear of the inlet, the outlet region and the periodic faces.  The explicit step
+    diverges within 10 steps at the specified xnue = 5e-4 with boxes placed anywhere, and
+    within 50 with either change alone.  FP64 and mixed precision are timed on the same
+    case, fixed count (iter_max = 100), and the fields after warmup + steps compared."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, phi = cases.boxes_case(steps=warmup + steps, cell_re=1.0, clear_faces=True)
+    log(f"[bench] mixed vs FP64 on boxes3d_{p.m}x{p.n}x{p.l}: {steps} steps, inner={inner}")
+    a = Solver(p, phi)
+    wa, ua, sa = _timed_run(a, steps, warmup)
+    b = Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=inner, report_residual=True)
+    wb, ub, sb = _timed_run(b, steps, warmup)
+    errs = {}
+    for f in ("u", "v", "w", "p"):
+        x, y = a.interior(f), b.interior(f)
+        errs[f] = float(np.linalg.norm(y - x) / max(np.linalg.norm(x), 1e-300))
+    da, db = a.divergence(), b.divergence()
+    ddiv = float(np.max(np.abs(da - db)) / max(np.max(np.abs(da)), 1e-300))
+    a.close()
+    b.close()
+    cells = p.cells
+    va, vb = cells * steps / wa, cells * steps / wb
+    return {
+        "workload": f"boxes3d_{p.m}x{p.n}x{p.l}", "xnue": p.xnue, "dt": p.dt, "cell_reynolds": p.dx / p.xnue,
+        "steps": steps, "warmup": warmup, "iter_max": p.iter_max,
+        "inner_iters": inner, "mode": "fixed-count",
+        "fp64": {"value": va, "unit": "cell-steps/s", "ms_per_step": wa * 1e3 / steps,
+                 "sor_cell_updates_per_s": va * p.iter_max, "avg_colour_sweep_us": ua,
+                 "sweep_roofline_frac": 3 * 8 * cells / (ua * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "mixed": {"value": vb, "unit": "cell-steps/s", "ms_per_step": wb * 1e3 / steps,
+                  "sor_cell_updates_per_s": vb * p.iter_max, "avg_colour_sweep_us": ub,
+                  "sweep_roofline_frac": 3 * 4 * cells / (ub * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                  "final_residual": float(sb["residual"][-1])},
+        "speedup_vs_fp64": vb / va,
+        "field_rel_l2_vs_fp64": errs, "divergence_max_diff_rel_vs_fp64": ddiv,
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +172,9 @@ def main(argv=None):
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the mixed-precision comparison on the same workload (0 = skip)")
     ap.add_argument("--mixed-inner", type=int, default=0, help="FP32 iterations per defect-correction cycle")
+    ap.add_argument("--large-steps", type=int, default=10,
+                    help="steps of the FP64/mixed comparison on the 512x256x256 boxes workload (0 = skip)")
+    ap.add_argument("--large-inner", type=int, default=25, help="FP32 iterations per cycle on that workload")
     ap.add_argument("--batch-cases", type=int, default=512,
                     help="cases of the batched 2D workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -205,6 +264,9 @@ def main(argv=None):
     }
     if args.mixed_steps > 0:
         res["mixed"] = mixed_comparison(p, phi, args.mixed_steps, args.warmup, args.mixed_inner, value)
+    if args.large_steps > 0:
+        res["mixed_large"] = mixed_large(args.large_steps, 2, args.large_inner)
+        log(f"[bench] boxes 512x256x256: mixed {res['mixed_large']['speedup_vs_fp64']:.2f}x FP64")
     if args.batch_cases > 0:
         from pixelflow_amd import workloads
         res["batch"] = workloads.batched_2d(count=args.batch_cases, steps=args.batch_steps, log=log)

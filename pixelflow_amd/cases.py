@@ -296,16 +296,29 @@ def sphere_3d(m: int, n: int, l: int, width: float, height: float, depth: float,
     return csv_round(tanh_profile(d, interface_cells))
 
 
-def random_boxes_3d(m: int, n: int, l: int, nboxes: int, side_range: Tuple[int, int], seed: int) -> np.ndarray:
+def random_boxes_3d(m: int, n: int, l: int, nboxes: int, side_range: Tuple[int, int], seed: int,
+                    clear_faces: bool = False) -> np.ndarray:
     """Binary porosity (1 fluid, 0 solid) with ``nboxes`` axis-aligned boxes, sides uniform
-    in ``side_range`` cells, placed uniformly and clipped to the domain; numpy (l, n, m)."""
+    in ``side_range`` cells, placed uniformly and clipped to the domain; numpy (l, n, m).
+
+    ``clear_faces`` places every box whole inside the region that leaves 16 cells after the
+    inlet, the last 35 % of x before the outlet and 4 cells at each periodic y / z face
+    free of solid: the reference's explicit step diverges at once where a box meets the
+    inlet, and the interface of a box clipped at a periodic face is a jump across the
+    wrap."""
     rng = np.random.default_rng(seed)
     phi = np.ones((l, n, m), dtype=np.float64)
     for _ in range(nboxes):
         sx, sy, sz = (int(v) for v in rng.integers(side_range[0], side_range[1] + 1, size=3))
-        x0 = int(rng.integers(0, m))
-        y0 = int(rng.integers(0, n))
-        z0 = int(rng.integers(0, l))
+        if clear_faces:
+            x_hi, margin = int(0.65 * m), 4
+            x0 = int(rng.integers(16, max(17, x_hi - sx + 1)))
+            y0 = int(rng.integers(margin, max(margin + 1, n - margin - sy + 1)))
+            z0 = int(rng.integers(margin, max(margin + 1, l - margin - sz + 1)))
+        else:
+            x0 = int(rng.integers(0, m))
+            y0 = int(rng.integers(0, n))
+            z0 = int(rng.integers(0, l))
         phi[z0:z0 + sz, y0:y0 + sy, x0:x0 + sx] = 0.0
     return phi
 
@@ -405,15 +418,27 @@ def sphere_case(m: int = 256, n: int = 128, l: int = 128, steps: int = 500, iter
 
 def boxes_case(m: int = 512, n: int = 256, l: int = 256, steps: int = 200, seed: int = 7,
                iter_max: int = 100, binary: bool = False, nboxes: int = 40,
-               side_range: Tuple[int, int] = (4, 32)) -> Tuple[CaseParams, np.ndarray]:
-    """3D mixed-precision case: 512x256x256, 40 seeded boxes with sides 4-32 cells, U = 1,
-    xnue = 5e-4, dt = 5e-5.  ``binary=True`` hands the 0/1 voxels to the solver as they
-    are (clamped to the threshold, the step diverges within a few steps in the reference
-    too: the porosity-gradient terms are divided by porosity); the default converts the
-    voxels with the reference's tanh interface (:func:`voxel_porosity`)."""
-    p = CaseParams(dim=3, m=m, n=n, l=l, width=2.0, height=1.0, depth=1.0, xnue=0.0005, time=5e-5 * steps,
+               side_range: Tuple[int, int] = (4, 32), cell_re: Optional[float] = None,
+               clear_faces: bool = False) -> Tuple[CaseParams, np.ndarray]:
+    """3D mixed-precision case: 512x256x256 over 2x1x1 m, 40 seeded boxes with sides 4-32
+    cells, U = 1, xnue = 5e-4, dt = 5e-5.  ``binary=True`` hands the 0/1 voxels to the
+    solver as they are (clamped to the threshold, the step diverges within a few steps in
+    the reference too: the porosity-gradient terms are divided by porosity); the default
+    converts the voxels with the reference's tanh interface (:func:`voxel_porosity`).
+
+    At dx = 2/511 m, xnue = 5e-4 is a cell Reynolds number U dx / xnue of 7.8, where the
+    reference's explicit central step diverges on the box interfaces within a dozen steps
+    at full size.  ``cell_re`` keeps grid and boxes and sets xnue = U dx / cell_re with dt
+    from :func:`stable_dt` instead; ``clear_faces`` keeps the boxes off the inlet, the
+    outlet region and the periodic faces (:func:`random_boxes_3d`)."""
+    xnue, dt = 0.0005, 5e-5
+    if cell_re is not None:
+        dx = 2.0 / (m - 1)
+        xnue = dx / cell_re
+        dt = stable_dt(dx, xnue, 1.0, dim=3)
+    p = CaseParams(dim=3, m=m, n=n, l=l, width=2.0, height=1.0, depth=1.0, xnue=xnue, time=dt * steps,
                    istep_max=steps, inlet_velocity=1.0, iter_max=iter_max, relux_factor=1.7)
-    vox = random_boxes_3d(m, n, l, nboxes, side_range, seed)
+    vox = random_boxes_3d(m, n, l, nboxes, side_range, seed, clear_faces=clear_faces)
     return p, (vox if binary else voxel_porosity(vox, 1.5))
 
 

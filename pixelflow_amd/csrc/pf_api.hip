@@ -38,7 +38,8 @@ struct pf_ctx {
     int sor_variant = 2;
     int sor_forced = -1;
     int xcd_remap = 1;
-    int sweep_kb = 1;               // 3D sweeps: cells per thread (1, 2 or 4)
+    int sweep_kb = 1;               // 3D FP64 sweeps: cells per thread (1, 2 or 4)
+    int sweep_kb32 = 1;             // 3D FP32 sweeps: cells per thread
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
     unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
     unsigned long long *res_part = nullptr;  // mixed: per-block max |r/ap|
@@ -190,10 +191,16 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     {
         const char *v = std::getenv("PF_SOR_VARIANT");
         if (v && *v) c->sor_forced = std::atoi(v);
+        // FP32 sweeps whose arrays (12 B per cell) fit well inside the 256 MB last-level
+        // cache run 6-8 % faster with two cells per thread (all loads issued together);
+        // past that size one cell per thread is faster (measured on 256x128x128, 192^3
+        // and 512x256x256).  FP64 sweeps are fastest with one.
+        const int64_t cells = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
+        c->sweep_kb32 = (cells * 12 <= ((int64_t)128 << 20)) ? 2 : 1;
         const char *kb = std::getenv("PF_SWEEP_KB");
         if (kb && *kb) {
             const int v = std::atoi(kb);
-            c->sweep_kb = (v == 1 || v == 2 || v == 4) ? v : 1;
+            c->sweep_kb = c->sweep_kb32 = (v == 1 || v == 2 || v == 4) ? v : 1;
         }
         const char *x = std::getenv("PF_XCD_REMAP");
         if (x && *x) c->xcd_remap = std::atoi(x) != 0;
@@ -450,7 +457,7 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
                 s.fuse_z = fz;
                 s.track_err = (g.dim == 2) || (half == 1);
                 s.xcd_remap = c->xcd_remap;
-                s.kb = c->sweep_kb;
+                s.kb = c->sweep_kb32;
                 pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot);
                 if (ry || rz) pf::launch_ghost_refresh32(c->st, g, ry, rz, c->E32);
             }

@@ -250,38 +250,74 @@ __device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, 
     return ((num / ap) * a.omega) + (pc * a.omega_c);
 }
 
+// The operands of one colour-sweep cell: the unknown at the cell and at its six neighbours
+// (other colour), porosity at the same seven places, and the right-hand side.
+template <typename T>
+struct SweepIn {
+    T c, w, e, s, n, b, t;
+    T fc, fw, fe, fs, fn, fb, ft;
+    T rhs;
+};
+
+// Load the operands of the cell at colour-split offset o (row shift sh).  Kernels that
+// update several cells per thread gather all of them before computing any, so the loads
+// of every cell are in flight together: in-place stores of the sweep only touch cells of
+// its own colour, which no gathered neighbour is, and the cell's own slot.
+template <int DIM, typename T>
+__device__ __forceinline__ void sweep_gather(const PfGeom &g, int sh, int64_t o, const T *__restrict__ Xoth,
+                                             const T *__restrict__ Xown, const T *__restrict__ Fo,
+                                             const T *__restrict__ Fx, const T *__restrict__ Ro, SweepIn<T> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    x.c = Xown[o];
+    x.w = Xoth[o - 1 + sh];
+    x.e = Xoth[o + sh];
+    x.s = Xoth[o - HP];
+    x.n = Xoth[o + HP];
+    x.fc = Fo[o];
+    x.fw = Fx[o - 1 + sh];
+    x.fe = Fx[o + sh];
+    x.fs = Fx[o - HP];
+    x.fn = Fx[o + HP];
+    x.rhs = Ro[o];
+    if (DIM == 3) {
+        x.b = Xoth[o - HPY];
+        x.t = Xoth[o + HPY];
+        x.fb = Fx[o - HPY];
+        x.ft = Fx[o + HPY];
+    } else {
+        x.b = x.t = x.fb = x.ft = (T)0;
+    }
+}
+
+// Store a new value at the cell and, with an even periodic extent, at the ghost slot that
+// mirrors it (same colour; no cell of the sweep reads it).
+template <int DIM, typename T>
+__device__ __forceinline__ void sweep_store(const PfGeom &g, const SweepArgs &s, int j, int k, int64_t o, T v,
+                                            T *__restrict__ Xown) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    Xown[o] = v;
+    if (s.fuse_y) {
+        if (j == g.n) Xown[o - HP * (int64_t)g.n] = v;
+        if (j == 1) Xown[o + HP * (int64_t)g.n] = v;
+    }
+    if (DIM == 3 && s.fuse_z) {
+        if (k == g.l) Xown[o - HPY * (int64_t)g.l] = v;
+        if (k == 1) Xown[o + HPY * (int64_t)g.l] = v;
+    }
+}
+
 template <int DIM, bool FAST>
 __device__ __forceinline__ double sor_phi_cell(const PfGeom &g, const StepArgs &a, const SweepArgs &s, int i, int j,
                                                int k, int sh, int64_t o, const double *__restrict__ Poth,
                                                double *__restrict__ Pown, const double *__restrict__ PHIo,
                                                const double *__restrict__ PHIx, const double *__restrict__ BBo,
                                                const double *__restrict__ phi) {
-    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-    const double pc = Pown[o];
-    const double pW = Poth[o - 1 + sh], pE = Poth[o + sh];
-    const double pS = Poth[o - HP], pN = Poth[o + HP];
-    const double fc = PHIo[o];
-    const double fW = PHIx[o - 1 + sh], fE = PHIx[o + sh];
-    const double fS = PHIx[o - HP], fN = PHIx[o + HP];
-    double pB = 0., pT = 0., fB = 0., fT = 0.;
-    if (DIM == 3) {
-        pB = Poth[o - HPY];
-        pT = Poth[o + HPY];
-        fB = PHIx[o - HPY];
-        fT = PHIx[o + HPY];
-    }
-    const double pn = sor_value<DIM, FAST>(g, a, i, j, k, pc, pW, pE, pS, pN, pB, pT, fc, fW, fE, fS, fN, fB, fT,
-                                           BBo[o], phi);
-    Pown[o] = pn;
-    if (s.fuse_y) {
-        if (j == g.n) Pown[o - HP * (int64_t)g.n] = pn;
-        if (j == 1) Pown[o + HP * (int64_t)g.n] = pn;
-    }
-    if (DIM == 3 && s.fuse_z) {
-        if (k == g.l) Pown[o - HPY * (int64_t)g.l] = pn;
-        if (k == 1) Pown[o + HPY * (int64_t)g.l] = pn;
-    }
-    return fabs(pn - pc);
+    SweepIn<double> c;
+    sweep_gather<DIM>(g, sh, o, Poth, Pown, PHIo, PHIx, BBo, c);
+    const double pn = sor_value<DIM, FAST>(g, a, i, j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw, c.fe, c.fs,
+                                           c.fn, c.fb, c.ft, c.rhs, phi);
+    sweep_store<DIM>(g, s, j, k, o, pn, Pown);
+    return fabs(pn - c.c);
 }
 
 __device__ __forceinline__ void project_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ P,
@@ -591,28 +627,12 @@ __device__ __forceinline__ double sor32_cell(const PfGeom &g, const StepArgs &a,
                                              float *__restrict__ Eown, const float *__restrict__ F32o,
                                              const float *__restrict__ F32x, const float *__restrict__ Ro,
                                              const double *__restrict__ phi, float cdx, float cdy, float cdz) {
-    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-    const float ec = Eown[o];
-    const float eW = Eoth[o - 1 + sh], eE = Eoth[o + sh], eS = Eoth[o - HP], eN = Eoth[o + HP];
-    float eB = 0.f, eT = 0.f, fB = 0.f, fT = 0.f;
-    if (DIM == 3) {
-        eB = Eoth[o - HPY];
-        eT = Eoth[o + HPY];
-        fB = F32x[o - HPY];
-        fT = F32x[o + HPY];
-    }
-    const float en = sor32_value<DIM>(g, a, i, j, k, ec, eW, eE, eS, eN, eB, eT, F32o[o], F32x[o - 1 + sh],
-                                      F32x[o + sh], F32x[o - HP], F32x[o + HP], fB, fT, Ro[o], phi, cdx, cdy, cdz);
-    Eown[o] = en;
-    if (s.fuse_y) {
-        if (j == g.n) Eown[o - HP * (int64_t)g.n] = en;
-        if (j == 1) Eown[o + HP * (int64_t)g.n] = en;
-    }
-    if (DIM == 3 && s.fuse_z) {
-        if (k == g.l) Eown[o - HPY * (int64_t)g.l] = en;
-        if (k == 1) Eown[o + HPY * (int64_t)g.l] = en;
-    }
-    return fabs((double)en - (double)ec);
+    SweepIn<float> c;
+    sweep_gather<DIM>(g, sh, o, Eoth, Eown, F32o, F32x, Ro, c);
+    const float en = sor32_value<DIM>(g, a, i, j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw, c.fe, c.fs, c.fn,
+                                      c.fb, c.ft, c.rhs, phi, cdx, cdy, cdz);
+    sweep_store<DIM>(g, s, j, k, o, en, Eown);
+    return fabs((double)en - (double)c.c);
 }
 
 }  // namespace pf

@@ -221,12 +221,24 @@ __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs
     const SweepThread t = sweep_thread<DIM, KB>(g, s);
     double err = 0.;
     if (t.j <= g.n && t.i <= g.m) {
+        SweepIn<double> x[KB];
+        int64_t o[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            // a plane past the last one loads plane l instead (in bounds) and is not updated
+            const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
+            o[q] = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
+            sweep_gather<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, BBo, x[q]);
+        }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             const int k = t.kbase + 2 * q;
             if (DIM == 3 && k > g.l) break;
-            const int64_t o = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
-            const double e = sor_phi_cell<DIM, FAST>(g, a, s, t.i, t.j, k, t.sh, o, Poth, Pown, PHIo, PHIx, BBo, phi);
+            const SweepIn<double> &c = x[q];
+            const double pn = sor_value<DIM, FAST>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw,
+                                                   c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, phi);
+            sweep_store<DIM>(g, s, t.j, k, o[q], pn, Pown);
+            const double e = fabs(pn - c.c);
             err = (e > err) ? e : err;
         }
     }

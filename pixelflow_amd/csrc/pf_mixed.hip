@@ -55,12 +55,24 @@ __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s
     const SweepThread t = sweep_thread<DIM, KB>(g, s);
     double err = 0.;
     if (t.j <= g.n && t.i <= g.m) {
+        SweepIn<float> x[KB];
+        int64_t o[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            // a plane past the last one loads plane l instead (in bounds) and is not updated
+            const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
+            o[q] = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
+            sweep_gather<DIM>(g, t.sh, o[q], Eoth, Eown, F32o, F32x, Ro, x[q]);
+        }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             const int k = t.kbase + 2 * q;
             if (DIM == 3 && k > g.l) break;
-            const int64_t o = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
-            const double e = sor32_cell<DIM>(g, a, s, t.i, t.j, k, t.sh, o, Eoth, Eown, F32o, F32x, Ro, phi, cdx, cdy, cdz);
+            const SweepIn<float> &c = x[q];
+            const float en = sor32_value<DIM>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw, c.fe,
+                                              c.fs, c.fn, c.fb, c.ft, c.rhs, phi, cdx, cdy, cdz);
+            sweep_store<DIM>(g, s, t.j, k, o[q], en, Eown);
+            const double e = fabs((double)en - (double)c.c);
             err = (e > err) ? e : err;
         }
     }

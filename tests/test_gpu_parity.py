@@ -239,3 +239,35 @@ def test_gpu_fused_mixed_matches_two_launch(name, monkeypatch):
     for f in _fields(p):
         np.testing.assert_array_equal(out[1][0][f], out[0][0][f], err_msg=f"{name}:{f}")
     np.testing.assert_array_equal(out[1][1], out[0][1])
+
+
+@pytest.mark.parametrize("kb", [1, 2, 4])
+@pytest.mark.parametrize("name", ["g3d_sph32", "g3d_odd_33_17_15", "g3d_air_room24", "g3d_weird", "g3d_boxes"])
+def test_gpu_sweep_cells_per_thread_matches_reference(name, kb, monkeypatch):
+    """3D colour sweeps with 1, 2 or 4 cells per thread (every cell's operands loaded before
+    any cell is updated; a plane past the last one is loaded from plane l and skipped)
+    reproduce the reference; odd l puts such planes in the last block row."""
+    monkeypatch.setenv("PF_SWEEP_KB", str(kb))
+    p, g = load_golden(name)
+    with _solver(p, g["phi_raw"]) as s:
+        st = s.step(p.istep_max)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), g[f], err_msg=f"{name}:{f}:kb{kb}")
+        np.testing.assert_array_equal(st["sor_error"], g["errs"])
+
+
+@pytest.mark.parametrize("name", ["3d_sphere_64x32x32", "3d_odd_45x27x19", "3d_air_room_40x36x32"])
+def test_gpu_mixed_cells_per_thread_identical(name, monkeypatch):
+    """FP32 sweeps give the same bits with 1, 2 or 4 cells per thread."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, phi = CASES[name]()
+    out = {}
+    for kb in (1, 2, 4):
+        monkeypatch.setenv("PF_SWEEP_KB", str(kb))
+        with Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=10) as s:
+            st = s.step(p.istep_max)
+            out[kb] = ({f: s.interior(f) for f in _fields(p)}, st["sor_error"])
+    for kb in (2, 4):
+        for f in _fields(p):
+            np.testing.assert_array_equal(out[kb][0][f], out[1][0][f], err_msg=f"{name}:{f}:kb{kb}")
+        np.testing.assert_array_equal(out[kb][1], out[1][1])
