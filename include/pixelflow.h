@@ -155,7 +155,8 @@ typedef struct pf_batch_config {
     int32_t persistent_blocks;  /* 0: one block per tile; > 0: this many blocks walk the tiles */
     int32_t use_graph;          /* capture the fixed-count step in a hipGraph and replay it */
     int32_t inner_iters;        /* mixed precision: FP32 iterations per defect-correction cycle */
-    int32_t tile_x;             /* colour-sweep tile width in slots: 32, 64 (default) or 128 */
+    int32_t tile_x;             /* colour-sweep tile width in slots: 32, 64 or 128; 0 = the
+                                   default (32 in 3D, 64 in 2D) */
 } pf_batch_config;
 
 /* kind: the shared settings; arena_bytes: device memory for the cases' fields. */

@@ -25,11 +25,14 @@ struct BatchCase {
     int slot;                     // the host's slot number of the case
 };
 
-// Kernel-side view of the batch: the case table and the tile prefix sums (ncases + 1 each).
+// Kernel-side view of the batch: the case table, the tile prefix sums (ncases + 1 each)
+// and, per tile, its case and its index inside the case (one load instead of a search).
 struct BatchView {
     const BatchCase *cases;
     const int *nat_prefix;
     const int *swp_prefix;
+    const int2 *nat_tile;
+    const int2 *swp_tile;
     const int *active;            // per case: 1 = takes part in the current cycle
     int ncases;
     int nat_tiles, swp_tiles;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -29,16 +30,6 @@
 #include "pf_kernels.h"
 
 namespace pf {
-
-__device__ __forceinline__ int find_case(const int *prefix, int ncases, int t) {
-    int lo = 0, hi = ncases - 1;   // largest c with prefix[c] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Block max of v into slots[t]; all threads of the block call it once per tile.
 __device__ __forceinline__ void tile_max_to_slot(double v, unsigned long long *slots, int t) {
@@ -74,12 +65,12 @@ enum NatOp {
 template <int OP>
 __device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, int aux) {
     {
-        const int ci = find_case(v.nat_prefix, v.ncases, t);
+        const int2 ti = v.nat_tile[t];
+        const int ci = ti.x, local = ti.y;
         if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) return;
         const BatchCase &C = v.cases[ci];
         const PfGeom &g = C.g;
         if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) return;
-        const int local = t - v.nat_prefix[ci];
         const int tx = local % C.nat_gx, r = local / C.nat_gx;
         const int ty = r % C.nat_gy, tz = r / C.nat_gy;
         const int i = tx * 64 + (int)threadIdx.x, j = ty * 4 + (int)threadIdx.y, k = tz;
@@ -147,48 +138,87 @@ enum SwpKind { SWP_FP64 = 0, SWP_FP32 = 1, SWP_RESIDUAL = 2 };
 // SEL picks the cases a launch serves: 1 = cases whose face coefficients passed the
 // fast-division check (fast path), 0 = the others (IEEE division); FP32 sweeps serve every
 // case.  Splitting them keeps the slow path's registers out of the fast kernel.
-template <int DIM, int KIND, int TX, int SEL>
+template <int DIM, int KIND, int TX, int SEL, int KB>
 __device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, int track,
                                          unsigned long long *slots) {
     constexpr int RY = 256 / TX;
     const int txi = (int)threadIdx.x % TX, tyi = (int)threadIdx.x / TX;
-    const int ci = find_case(v.swp_prefix, v.ncases, t);
+    const int2 ti = v.swp_tile[t];
+    const int ci = ti.x, local = ti.y;
     if (!v.active[ci]) return;   // block-uniform: every thread skips the tile together
     const BatchCase &C = v.cases[ci];
     if (KIND != SWP_FP32 && (int)C.fast != SEL) return;
     const PfGeom &g = C.g;
-    const int local = t - v.swp_prefix[ci];
     const int bx = local % C.swp_gx, r = local / C.swp_gx;
     const int by = r % C.swp_gy, bz = r / C.swp_gy;
-    int j, k;
+    int j, kbase;
     if (DIM == 3) {
         j = by * (RY / 2) + (tyi >> 1) + 1;
-        k = bz * 2 + 1 + (tyi & 1);
+        kbase = bz * 2 * KB + 1 + (tyi & 1);   // planes kbase, kbase + 2, ... (one parity)
     } else {
         j = by * RY + tyi + 1;
-        k = 0;
+        kbase = 0;
     }
-    const int sh = (colour + j + k) & 1;
+    const int sh = (colour + j + kbase) & 1;
     const int h = bx * TX + txi + 1 - sh;
     const int i = 2 * h + sh;
     double e = 0.;
-    if (j <= g.n && i <= g.m && (DIM == 2 || k <= g.l)) {
-        const int64_t o = (int64_t)h + (int64_t)g.HP * ((int64_t)j + (int64_t)g.Y * k);
+    if (j <= g.n && i <= g.m) {
         const int64_t own = (int64_t)colour * g.HS, oth = (int64_t)(1 - colour) * g.HS;
         SweepArgs s{};
         s.colour = colour;
         s.fuse_y = C.fuse_y;
         s.fuse_z = C.fuse_z;
+        int64_t o[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            // a plane past the last one stands in as plane l (in bounds) and is skipped
+            const int k = (DIM == 3 && kbase + 2 * q > g.l) ? g.l : kbase + 2 * q;
+            o[q] = (int64_t)h + (int64_t)g.HP * ((int64_t)j + (int64_t)g.Y * k);
+        }
         if (KIND == SWP_FP64) {
-            e = sor_phi_cell<DIM, SEL == 1>(g, C.a, s, i, j, k, sh, o, C.P + oth, C.P + own, C.PHIs + own,
-                                            C.PHIs + oth, C.BB + own, C.phi);
+            SweepIn<double> x[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q)
+                sweep_gather<DIM>(g, sh, o[q], C.P + oth, C.P + own, C.PHIs + own, C.PHIs + oth, C.BB + own, x[q]);
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const int k = kbase + 2 * q;
+                if (DIM == 3 && k > g.l) break;
+                const SweepIn<double> &c = x[q];
+                const double pn = sor_value<DIM, SEL == 1>(g, C.a, i, j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc,
+                                                           c.fw, c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, C.phi);
+                sweep_store<DIM>(g, s, j, k, o[q], pn, C.P + own);
+                const double d = fabs(pn - c.c);
+                e = (d > e) ? d : e;
+            }
         } else if (KIND == SWP_FP32) {
             const float cdx = (float)(C.a.dt / (C.a.dx * C.a.dx)), cdy = (float)(C.a.dt / (C.a.dy * C.a.dy)),
                         cdz = (float)(C.a.dt / (C.a.dz * C.a.dz));
-            e = sor32_cell<DIM>(g, C.a, s, i, j, k, sh, o, C.E + oth, C.E + own, C.PHI32 + own, C.PHI32 + oth,
-                                C.R + own, C.phi, cdx, cdy, cdz);
+            SweepIn<float> x[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q)
+                sweep_gather<DIM>(g, sh, o[q], C.E + oth, C.E + own, C.PHI32 + own, C.PHI32 + oth, C.R + own, x[q]);
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const int k = kbase + 2 * q;
+                if (DIM == 3 && k > g.l) break;
+                const SweepIn<float> &c = x[q];
+                const float en = sor32_value<DIM>(g, C.a, i, j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw,
+                                                  c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, C.phi, cdx, cdy, cdz);
+                sweep_store<DIM>(g, s, j, k, o[q], en, C.E + own);
+                const double d = fabs((double)en - (double)c.c);
+                e = (d > e) ? d : e;
+            }
         } else {
-            e = residual_cell<DIM, SEL == 1>(g, C.a, colour, i, j, k, sh, o, C.P, C.PHIs, C.BB, C.phi, C.R, C.E);
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const int k = kbase + 2 * q;
+                if (DIM == 3 && k > g.l) break;
+                const double d = residual_cell<DIM, SEL == 1>(g, C.a, colour, i, j, k, sh, o[q], C.P, C.PHIs, C.BB,
+                                                              C.phi, C.R, C.E);
+                e = (d > e) ? d : e;
+            }
         }
     }
     if (track) tile_max_to_slot(e, slots, t);
@@ -197,14 +227,14 @@ __device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, 
 // PERSIST = false: one block per tile, so every load of the case table precedes the
 // tile's stores and the compiler keeps the table in scalar registers; true: a fixed grid
 // walks the tile list.
-template <int DIM, int KIND, int TX, int SEL, bool PERSIST>
+template <int DIM, int KIND, int TX, int SEL, bool PERSIST, int KB>
 __global__ __launch_bounds__(256) void k_batch_swp(BatchView v, int colour, int track, unsigned long long *slots) {
     const int G = (int)gridDim.x;
     const int L = xcd_logical_block((int)blockIdx.x, G);
     if (PERSIST) {
-        for (int t = L; t < v.swp_tiles; t += G) swp_tile<DIM, KIND, TX, SEL>(v, t, colour, track, slots);
+        for (int t = L; t < v.swp_tiles; t += G) swp_tile<DIM, KIND, TX, SEL, KB>(v, t, colour, track, slots);
     } else if (L < v.swp_tiles) {
-        swp_tile<DIM, KIND, TX, SEL>(v, L, colour, track, slots);
+        swp_tile<DIM, KIND, TX, SEL, KB>(v, L, colour, track, slots);
     }
 }
 
@@ -309,7 +339,8 @@ struct pf_batch {
     std::vector<int> order;         // compact case index -> slot
     BatchCase *d_cases = nullptr;
     int *d_nat_prefix = nullptr, *d_swp_prefix = nullptr, *d_active = nullptr, *d_step = nullptr;
-    int nat_tiles = 0, swp_tiles = 0, table_cap = 0, tile_cap = 0;
+    int2 *d_nat_tile = nullptr, *d_swp_tile = nullptr;   // per tile: case, tile within the case
+    int nat_tiles = 0, swp_tiles = 0, table_cap = 0, tile_cap = 0, nat_cap = 0;
     unsigned long long *d_tile_slots = nullptr;   // per-tile SOR error maxima
     unsigned long long *d_res_slots = nullptr;    // per-tile residual maxima
     unsigned long long *d_err_hist = nullptr, *d_res_hist = nullptr;
@@ -320,6 +351,7 @@ struct pf_batch {
     bool any_refresh = false;
     bool any_fast = false, any_slow = false;   // cases on the fast / IEEE division path
     int tile_x = 64;                // sweep tile width in slots (32, 64 or 128)
+    int swp_kb = 1;                 // 3D sweep tiles: cells per thread along z (1, 2 or 4)
     int parity = 0;
     pf_batch_config cfg{};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -387,7 +419,19 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
     b->cfg.persistent_blocks = 0;
     b->cfg.use_graph = 1;
     b->cfg.inner_iters = kind->inner_iters;
-    b->cfg.tile_x = 64;
+    // Measured on eight served 3D rooms (64^3 to 160x128x96, 10 M cells): FP32 sweep tiles
+    // of 32 slots with two cells per thread run 1.25x faster than 64 slots with one;
+    // FP64 sweeps lose with more than one cell per thread (register pressure).
+    b->cfg.tile_x = (kind->dim == 3) ? 32 : 64;
+    b->tile_x = b->cfg.tile_x;
+    b->swp_kb = (kind->dim == 3 && kind->precision == PF_PREC_MIXED) ? 2 : 1;
+    {
+        const char *kb = std::getenv("PF_BATCH_KB");
+        if (kb && *kb) {
+            const int v = std::atoi(kb);
+            b->swp_kb = (v == 1 || v == 2 || v == 4) ? v : 1;
+        }
+    }
     b->arena_bytes = align256(arena_bytes);
     hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&b->arena, (size_t)b->arena_bytes);
@@ -413,7 +457,8 @@ extern "C" int pf_batch_destroy(pf_batch *b) {
     if (!b) return PF_EINVAL;
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graphs(b);
-    void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_active, b->d_step,
+    void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_nat_tile, b->d_swp_tile,
+                    b->d_active, b->d_step,
                     b->d_nactive, b->d_tile_slots, b->d_res_slots, b->d_err_hist, b->d_res_hist,
                     b->d_it_hist, b->d_cy_hist};
     for (void *p : ptrs)
@@ -460,6 +505,12 @@ extern "C" int64_t pf_batch_free_bytes(const pf_batch *b) {
     return best;
 }
 
+// Cells per thread of the 3D sweep tiles in the current grid kind (the persistent grid
+// walks one-cell tiles).
+static int tile_kb(const pf_batch *b) {
+    return (b->kind.dim == 3 && b->cfg.persistent_blocks == 0) ? b->swp_kb : 1;
+}
+
 static int rebuild_tables(pf_batch *b) {
     b->order.clear();
     for (int s = 0; s < b->max_cases; ++s)
@@ -477,7 +528,8 @@ static int rebuild_tables(pf_batch *b) {
         const int TX = b->tile_x, RY = 256 / TX;
         bc.swp_gx = ((g.m + 1) / 2 + TX - 1) / TX;
         bc.swp_gy = (g.dim == 3) ? (g.n + RY / 2 - 1) / (RY / 2) : (g.n + RY - 1) / RY;
-        const int swp_gz = (g.dim == 3) ? (g.l + 1) / 2 : 1;
+        const int kb = tile_kb(b);
+        const int swp_gz = (g.dim == 3) ? (g.l + 2 * kb - 1) / (2 * kb) : 1;
         np.push_back(np.back() + bc.nat_gx * bc.nat_gy * g.Z);
         sp.push_back(sp.back() + bc.swp_gx * bc.swp_gy * swp_gz);
         b->any_refresh = b->any_refresh || bc.ref_y || bc.ref_z;
@@ -503,19 +555,37 @@ static int rebuild_tables(pf_batch *b) {
     if (b->tile_cap < b->swp_tiles + 1) {
         if (b->d_tile_slots) (void)hipFree(b->d_tile_slots);
         if (b->d_res_slots) (void)hipFree(b->d_res_slots);
+        if (b->d_swp_tile) (void)hipFree(b->d_swp_tile);
         b->d_tile_slots = b->d_res_slots = nullptr;
+        b->d_swp_tile = nullptr;
         const int cap = b->swp_tiles + 1024;
         PFB_TRY(b, hipMalloc((void **)&b->d_tile_slots, sizeof(unsigned long long) * (size_t)cap));
         PFB_TRY(b, hipMalloc((void **)&b->d_res_slots, sizeof(unsigned long long) * (size_t)cap));
+        PFB_TRY(b, hipMalloc((void **)&b->d_swp_tile, sizeof(int2) * (size_t)cap));
         PFB_TRY(b, hipMemsetAsync(b->d_tile_slots, 0, sizeof(unsigned long long) * (size_t)cap, b->st));
         PFB_TRY(b, hipMemsetAsync(b->d_res_slots, 0, sizeof(unsigned long long) * (size_t)cap, b->st));
         b->tile_cap = cap;
     }
+    if (b->nat_cap < b->nat_tiles + 1) {
+        if (b->d_nat_tile) (void)hipFree(b->d_nat_tile);
+        b->d_nat_tile = nullptr;
+        const int cap = b->nat_tiles + 4096;
+        PFB_TRY(b, hipMalloc((void **)&b->d_nat_tile, sizeof(int2) * (size_t)cap));
+        b->nat_cap = cap;
+    }
+    // host sources of the asynchronous copies live until the synchronisation below
+    std::vector<int2> nt((size_t)b->nat_tiles), st((size_t)b->swp_tiles);
+    std::vector<int> ones((size_t)nc, 1);
     if (nc > 0) {
+        for (int c = 0; c < nc; ++c) {
+            for (int t = np[(size_t)c]; t < np[(size_t)c + 1]; ++t) nt[(size_t)t] = make_int2(c, t - np[(size_t)c]);
+            for (int t = sp[(size_t)c]; t < sp[(size_t)c + 1]; ++t) st[(size_t)t] = make_int2(c, t - sp[(size_t)c]);
+        }
+        PFB_TRY(b, hipMemcpyAsync(b->d_nat_tile, nt.data(), sizeof(int2) * nt.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY(b, hipMemcpyAsync(b->d_swp_tile, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice, b->st));
         PFB_TRY(b, hipMemcpyAsync(b->d_cases, cs.data(), sizeof(BatchCase) * (size_t)nc, hipMemcpyHostToDevice, b->st));
         PFB_TRY(b, hipMemcpyAsync(b->d_nat_prefix, np.data(), sizeof(int) * np.size(), hipMemcpyHostToDevice, b->st));
         PFB_TRY(b, hipMemcpyAsync(b->d_swp_prefix, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice, b->st));
-        std::vector<int> ones((size_t)nc, 1);
         PFB_TRY(b, hipMemcpyAsync(b->d_active, ones.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
     }
     PFB_TRY(b, hipStreamSynchronize(b->st));
@@ -529,6 +599,8 @@ static pf::BatchView view(const pf_batch *b) {
     v.cases = b->d_cases;
     v.nat_prefix = b->d_nat_prefix;
     v.swp_prefix = b->d_swp_prefix;
+    v.nat_tile = b->d_nat_tile;
+    v.swp_tile = b->d_swp_tile;
     v.active = b->d_active;
     v.ncases = (int)b->order.size();
     v.nat_tiles = b->nat_tiles;
@@ -680,10 +752,16 @@ static void launch_nat(pf_batch *b, int op, int parity, int aux) {
 template <int D, int K, int T, int SEL>
 static void launch_swp_t(pf_batch *b, dim3 gr, const pf::BatchView &v, int colour, int track,
                          unsigned long long *slots) {
+    // 3D tiles cover 2 * swp_kb planes (one grid kind per batch: the persistent grid keeps
+    // one cell per thread)
     if (b->cfg.persistent_blocks > 0)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+    else if (D == 3 && b->swp_kb == 2)
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 2>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+    else if (D == 3 && b->swp_kb == 4)
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 4>), gr, dim3(256), 0, b->st, v, colour, track, slots);
     else
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
 }
 
 template <int D, int K, int SEL>
@@ -999,12 +1077,13 @@ extern "C" int pf_batch_divergence(pf_batch *b, int slot, double *out) {
 
 extern "C" int pf_batch_set_config(pf_batch *b, const pf_batch_config *cfg) {
     if (!b || !cfg || cfg->persistent_blocks < 0 || cfg->inner_iters < 0) return PF_EINVAL;
-    const int tx = cfg->tile_x == 0 ? 64 : cfg->tile_x;
+    const int tx = cfg->tile_x == 0 ? (b->kind.dim == 3 ? 32 : 64) : cfg->tile_x;
     if (tx != 32 && tx != 64 && tx != 128) { b->msg = "tile_x must be 32, 64 or 128"; return PF_EINVAL; }
     PFB_TRY(b, hipStreamSynchronize(b->st));
+    const int kb_before = tile_kb(b);
     b->cfg = *cfg;
     b->cfg.tile_x = tx;
-    if (tx != b->tile_x) {
+    if (tx != b->tile_x || tile_kb(b) != kb_before) {   // tile geometry changed
         b->tile_x = tx;
         b->dirty = true;
     }

@@ -374,7 +374,7 @@ class Batch:
         return int(lib().pf_batch_case_steps(self._h, int(slot)))
 
     def set_config(self, persistent_blocks: int = 0, use_graph: bool = True, inner_iters: int = 0,
-                   tile_x: int = 64) -> None:
+                   tile_x: int = 0) -> None:
         c = PfBatchConfig(int(persistent_blocks), 1 if use_graph else 0, int(inner_iters), int(tile_x))
         self._check(lib().pf_batch_set_config(self._h, ctypes.byref(c)), "pf_batch_set_config")
 

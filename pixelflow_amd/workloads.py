@@ -9,7 +9,7 @@
 from __future__ import annotations
 
 import time
-from typing import Callable, Dict
+from typing import Callable, Dict, Optional
 
 import numpy as np
 
@@ -88,7 +88,8 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
 
 def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
                  inner: int = 50, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
-                 plain_sample: int = 3, plain_steps: int = 10, log: Callable = _noop) -> Dict[str, object]:
+                 plain_sample: Optional[int] = None, plain_steps: int = 10,
+                 log: Callable = _noop) -> Dict[str, object]:
     """Poisson-arriving room requests through the library's serving loop (pf_serve_*): a
     host thread submits each request at its arrival time while the worker steps the
     resident ones."""
@@ -128,8 +129,10 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
     span = max(t[2] for t in ok) - min(t[0] for t in ok) if ok else 0.0
     upd = sum(p.cells * steps for p, _ in reqs)             # cell-steps
     sor_upd = sum(p.cells * q[2] for (p, _), q in zip(reqs, polls))   # SOR cell-updates actually run
-    # plain GPU path on a sample of the same rooms: one case at a time, FP64, iter_max
-    # iterations every step (the reference's schedule)
+    # plain GPU path on the same rooms (all of them unless a sample size is given): one
+    # case at a time, FP64, iter_max iterations every step (the reference's schedule)
+    if plain_sample is None:
+        plain_sample = len(reqs)
     plain_upd, plain_time = 0, 0.0
     for p, phi in reqs[:plain_sample]:
         with Solver(p, phi) as s:

@@ -125,3 +125,37 @@ def test_batch_air_tolerance_mode_matches_single(max_cycles, graph):
             np.testing.assert_array_equal(st["residual"][:, s], rst["residual"])
             for f in _fields(p):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"{p.m}x{p.n}x{p.l} {f}")
+
+
+def _odd_even_3d_cases():
+    specs = [(32, 16, 16), (33, 17, 15), (24, 20, 18), (40, 24, 16)]
+    cs = []
+    for (m, n, l) in specs:
+        p = cases.CaseParams(dim=3, m=m, n=n, l=l, width=1.0, height=(n - 1) / (m - 1), depth=(l - 1) / (m - 1),
+                             time=4 * 1e-3, istep_max=4, iter_max=25, xlambda=1e-4)
+        cs.append((p, cases.sphere_3d(m, n, l, 1.0, p.height, p.depth, (0.3, p.height / 2, p.depth / 2), 0.15)))
+    return cs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kb", [2, 4])
+def test_batch_3d_cells_per_thread_matches_single(kb, precision, monkeypatch):
+    """3D sweep tiles of 2 or 4 cells per thread along z (odd l leaves planes past the last
+    one in the last tile row), then a switch to the persistent grid, which re-tiles with
+    one cell per thread: every case keeps the bits it gives alone."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Batch
+    monkeypatch.setenv("PF_BATCH_KB", str(kb))
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=10) if precision == "mixed" else {}
+    cs = _odd_even_3d_cases()
+    with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 28, **kw) as b:
+        b.set_config(tile_x=32, inner_iters=kw.get("inner_iters", 0))
+        slots = [b.add(p, phi) for p, phi in cs]
+        st = b.step(2)
+        b.set_config(tile_x=32, persistent_blocks=64, inner_iters=kw.get("inner_iters", 0))
+        st2 = b.step(2)
+        for (p, phi), s in zip(cs, slots):
+            ref, rst = _single(p, phi, 4, **kw)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"kb{kb} {p.m}x{p.n}x{p.l} {f}")
+            np.testing.assert_array_equal(np.concatenate([st["sor_error"][:, s], st2["sor_error"][:, s]]),
+                                          rst["sor_error"])
