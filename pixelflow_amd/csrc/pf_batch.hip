@@ -101,7 +101,7 @@ __device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, 
             if (rr >= 0) boundary_air_cell(g, C.a, aux, C.phi, C.U[p], C.V[p], C.W[p], C.P, rr, qq);
         } else if (OP == NAT_UPDATE) {
             const int64_t so = pf_soff(g, i, j, k);
-            C.P[so] = C.P[so] + (double)C.E[so];
+            gmem(C.P)[so] = gmem(C.P)[so] + (double)gmem(C.E)[so];
         } else if (OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) {
             const bool iin = i >= 1 && i <= g.m;
             int sj = -1, sk = -1;
@@ -113,8 +113,8 @@ __device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, 
                 sk = (k == 0) ? g.l : 1;
             }
             if (sj < 0) return;
-            if (OP == NAT_REFRESH_P) C.P[pf_soff(g, i, j, k)] = C.P[pf_soff(g, i, sj, sk)];
-            else C.E[pf_soff(g, i, j, k)] = C.E[pf_soff(g, i, sj, sk)];
+            if (OP == NAT_REFRESH_P) gmem(C.P)[pf_soff(g, i, j, k)] = gmem(C.P)[pf_soff(g, i, sj, sk)];
+            else gmem(C.E)[pf_soff(g, i, j, k)] = gmem(C.E)[pf_soff(g, i, sj, sk)];
         }
     }
 }
@@ -757,9 +757,11 @@ static void launch_swp_t(pf_batch *b, dim3 gr, const pf::BatchView &v, int colou
     if (b->cfg.persistent_blocks > 0)
         hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
     else if (D == 3 && b->swp_kb == 2)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 2>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 2 : 1>), gr, dim3(256), 0, b->st, v, colour,
+                           track, slots);
     else if (D == 3 && b->swp_kb == 4)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 4>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 4 : 1>), gr, dim3(256), 0, b->st, v, colour,
+                           track, slots);
     else
         hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
 }

@@ -9,16 +9,21 @@
 
 namespace pf {
 
-__device__ __forceinline__ double div3_at(const PfGeom &g, const StepArgs &a, const double *uo,
-                                          const double *vo, const double *wo, int i, int j, int k) {
+__device__ __forceinline__ double div3_at(const PfGeom &g, const StepArgs &a, const double *uo_,
+                                          const double *vo_, const double *wo_, int i, int j, int k) {
+    const auto uo = gmem(uo_);
+    const auto vo = gmem(vo_);
+    const auto wo = gmem(wo_);
     const int64_t o = pf_off(g, i, j, k);
     const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
     return ((((uo[o + 1] - uo[o - 1]) / a.dx) * 0.5) + (((vo[o + sy] - vo[o - sy]) / a.dy) * 0.5)) +
            (((wo[o + sz] - wo[o - sz]) / a.dz) * 0.5);
 }
 
-__device__ __forceinline__ double div2_at(const PfGeom &g, const StepArgs &a, const double *uo,
-                                          const double *vo, int i, int j) {
+__device__ __forceinline__ double div2_at(const PfGeom &g, const StepArgs &a, const double *uo_,
+                                          const double *vo_, int i, int j) {
+    const auto uo = gmem(uo_);
+    const auto vo = gmem(vo_);
     const int64_t o = pf_off(g, i, j, 0);
     // the 2D code divides the v-difference by dx (ibm_2d_uniform_omp_cpu.f90:175-176)
     return (((uo[o + 1] - uo[o - 1]) / a.dx) * .5) + (((vo[o + g.PX] - vo[o - g.PX]) / a.dx) * .5);
@@ -26,7 +31,8 @@ __device__ __forceinline__ double div2_at(const PfGeom &g, const StepArgs &a, co
 
 __device__ __forceinline__ void div_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo,
                                          const double *__restrict__ vo, const double *__restrict__ wo,
-                                         double *__restrict__ div, int i, int j, int k) {
+                                         double *__restrict__ div_, int i, int j, int k) {
+    const auto div = gmem(div_);
     const int m = g.m, n = g.n, l = g.l;
     const bool gi = (i == 0 || i == m + 1), gj = (j == 0 || j == n + 1);
     const bool gk = (g.dim == 3) && (k == 0 || k == l + 1);
@@ -45,11 +51,19 @@ __device__ __forceinline__ void div_cell(const PfGeom &g, const StepArgs &a, con
     div[pf_off(g, i, j, k)] = val;
 }
 
-__device__ __forceinline__ void predict3_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo,
-                                              const double *__restrict__ vo, const double *__restrict__ wo,
-                                              const double *__restrict__ div, const double *__restrict__ po,
-                                              double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+__device__ __forceinline__ void predict3_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo_,
+                                              const double *__restrict__ vo_, const double *__restrict__ wo_,
+                                              const double *__restrict__ div_, const double *__restrict__ po_,
+                                              double *__restrict__ u_, double *__restrict__ v_, double *__restrict__ w_,
                                               int i, int j, int k) {
+    const auto uo = gmem(uo_);
+    const auto vo = gmem(vo_);
+    const auto wo = gmem(wo_);
+    const auto div = gmem(div_);
+    const auto po = gmem(po_);
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto w = gmem(w_);
     const int64_t o = pf_off(g, i, j, k);
     if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1 || k == 0 || k == g.l + 1) {
         u[o] = uo[o]; v[o] = vo[o]; w[o] = wo[o];
@@ -124,10 +138,16 @@ __device__ __forceinline__ void predict3_cell(const PfGeom &g, const StepArgs &a
     }
 }
 
-__device__ __forceinline__ void predict2_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo,
-                                              const double *__restrict__ vo, const double *__restrict__ div,
-                                              const double *__restrict__ po, double *__restrict__ u,
-                                              double *__restrict__ v, int i, int j) {
+__device__ __forceinline__ void predict2_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo_,
+                                              const double *__restrict__ vo_, const double *__restrict__ div_,
+                                              const double *__restrict__ po_, double *__restrict__ u_,
+                                              double *__restrict__ v_, int i, int j) {
+    const auto uo = gmem(uo_);
+    const auto vo = gmem(vo_);
+    const auto div = gmem(div_);
+    const auto po = gmem(po_);
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
     const int64_t o = pf_off(g, i, j, 0);
     if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1) {
         u[o] = uo[o]; v[o] = vo[o];
@@ -174,8 +194,12 @@ __device__ __forceinline__ void predict2_cell(const PfGeom &g, const StepArgs &a
     }
 }
 
-__device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                                         const double *w, const double *po, int i, int j, int k) {
+__device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, const double *u_, const double *v_,
+                                         const double *w_, const double *po_, int i, int j, int k) {
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto w = gmem(w_);
+    const auto po = gmem(po_);
     const int64_t o = pf_off(g, i, j, k);
     const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
     const double pc = po[o], density = a.density;
@@ -184,8 +208,11 @@ __device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, con
            ((((((po[o + sz] * w[o]) + (pc * w[o + sz])) * 0.5) - (((po[o - sz] * w[o]) + (pc * w[o - sz])) * 0.5)) * density) / a.dz);
 }
 
-__device__ __forceinline__ double bb2_at(const PfGeom &g, const StepArgs &a, const double *u, const double *v,
-                                         const double *po, int i, int j) {
+__device__ __forceinline__ double bb2_at(const PfGeom &g, const StepArgs &a, const double *u_, const double *v_,
+                                         const double *po_, int i, int j) {
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto po = gmem(po_);
     const int64_t o = pf_off(g, i, j, 0);
     const int64_t sy = g.PX;
     const double pc = po[o], density = a.density;
@@ -195,9 +222,11 @@ __device__ __forceinline__ double bb2_at(const PfGeom &g, const StepArgs &a, con
 
 __device__ __forceinline__ void rhs_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ u,
                                          const double *__restrict__ v, const double *__restrict__ w,
-                                         const double *__restrict__ po, const double *__restrict__ P,
-                                         double *__restrict__ BB, int i, int j, int k) {
-    double bb = (g.dim == 3) ? bb3_at(g, a, u, v, w, po, i, j, k) : bb2_at(g, a, u, v, po, i, j);
+                                         const double *__restrict__ po_, const double *__restrict__ P,
+                                         double *__restrict__ BB_, int i, int j, int k) {
+    const auto po = gmem(po_);
+    const auto BB = gmem(BB_);
+    double bb = (g.dim == 3) ? bb3_at(g, a, u, v, w, po_, i, j, k) : bb2_at(g, a, u, v, po_, i, j);
     if (is_edge_row(g, a, i, j, k)) {
         // the row's six coefficients, evaluated exactly as solve_p does (3D :390-400)
         const int64_t o = pf_off(g, i, j, k);
@@ -211,8 +240,8 @@ __device__ __forceinline__ void rhs_cell(const PfGeom &g, const StepArgs &a, con
         c.at = (g.dim == 3) ? face_coef(po[o + sz], pc, a.dt, a.dz) : 0.;
         c.ab = (g.dim == 3) ? face_coef(pc, po[o - sz], a.dt, a.dz) : 0.;
         double bbk1 = 0.;
-        if (a.bc == PF_BC_AIR && k == g.l && a.face[0] == 2) bbk1 = bb3_at(g, a, u, v, w, po, i, j, 1);
-        row_rules<true>(g, a, i, j, k, c, &bb, P, po, bbk1);
+        if (a.bc == PF_BC_AIR && k == g.l && a.face[0] == 2) bbk1 = bb3_at(g, a, u, v, w, po_, i, j, 1);
+        row_rules<true>(g, a, i, j, k, c, &bb, P, po_, bbk1);
     }
     BB[pf_soff(g, i, j, k)] = bb;
 }
@@ -268,22 +297,23 @@ __device__ __forceinline__ void sweep_gather(const PfGeom &g, int sh, int64_t o,
                                              const T *__restrict__ Xown, const T *__restrict__ Fo,
                                              const T *__restrict__ Fx, const T *__restrict__ Ro, SweepIn<T> &x) {
     const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-    x.c = Xown[o];
-    x.w = Xoth[o - 1 + sh];
-    x.e = Xoth[o + sh];
-    x.s = Xoth[o - HP];
-    x.n = Xoth[o + HP];
-    x.fc = Fo[o];
-    x.fw = Fx[o - 1 + sh];
-    x.fe = Fx[o + sh];
-    x.fs = Fx[o - HP];
-    x.fn = Fx[o + HP];
-    x.rhs = Ro[o];
+    const auto X = gmem(Xoth), F = gmem(Fx);
+    x.c = gmem(Xown)[o];
+    x.w = X[o - 1 + sh];
+    x.e = X[o + sh];
+    x.s = X[o - HP];
+    x.n = X[o + HP];
+    x.fc = gmem(Fo)[o];
+    x.fw = F[o - 1 + sh];
+    x.fe = F[o + sh];
+    x.fs = F[o - HP];
+    x.fn = F[o + HP];
+    x.rhs = gmem(Ro)[o];
     if (DIM == 3) {
-        x.b = Xoth[o - HPY];
-        x.t = Xoth[o + HPY];
-        x.fb = Fx[o - HPY];
-        x.ft = Fx[o + HPY];
+        x.b = X[o - HPY];
+        x.t = X[o + HPY];
+        x.fb = F[o - HPY];
+        x.ft = F[o + HPY];
     } else {
         x.b = x.t = x.fb = x.ft = (T)0;
     }
@@ -295,14 +325,15 @@ template <int DIM, typename T>
 __device__ __forceinline__ void sweep_store(const PfGeom &g, const SweepArgs &s, int j, int k, int64_t o, T v,
                                             T *__restrict__ Xown) {
     const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-    Xown[o] = v;
+    const auto X = gmem(Xown);
+    X[o] = v;
     if (s.fuse_y) {
-        if (j == g.n) Xown[o - HP * (int64_t)g.n] = v;
-        if (j == 1) Xown[o + HP * (int64_t)g.n] = v;
+        if (j == g.n) X[o - HP * (int64_t)g.n] = v;
+        if (j == 1) X[o + HP * (int64_t)g.n] = v;
     }
     if (DIM == 3 && s.fuse_z) {
-        if (k == g.l) Xown[o - HPY * (int64_t)g.l] = v;
-        if (k == 1) Xown[o + HPY * (int64_t)g.l] = v;
+        if (k == g.l) X[o - HPY * (int64_t)g.l] = v;
+        if (k == 1) X[o + HPY * (int64_t)g.l] = v;
     }
 }
 
@@ -320,9 +351,13 @@ __device__ __forceinline__ double sor_phi_cell(const PfGeom &g, const StepArgs &
     return fabs(pn - c.c);
 }
 
-__device__ __forceinline__ void project_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ P,
-                                             double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+__device__ __forceinline__ void project_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ P_,
+                                             double *__restrict__ u_, double *__restrict__ v_, double *__restrict__ w_,
                                              int i, int j, int k) {
+    const auto P = gmem(P_);
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto w = gmem(w_);
     const int64_t o = pf_off(g, i, j, k);
     const double f = a.dt / a.density;
     u[o] = u[o] - (((f * (P[pf_soff(g, i + 1, j, k)] - P[pf_soff(g, i - 1, j, k)])) / a.dx) * 0.5);
@@ -332,9 +367,14 @@ __device__ __forceinline__ void project_cell(const PfGeom &g, const StepArgs &a,
 }
 
 __device__ __forceinline__ void boundary_uniform_cell(const PfGeom &g, const StepArgs &a,
-                                                      const double *__restrict__ po, double *__restrict__ u,
-                                                      double *__restrict__ v, double *__restrict__ w,
-                                                      double *__restrict__ P, int i, int j, int k) {
+                                                      const double *__restrict__ po_, double *__restrict__ u_,
+                                                      double *__restrict__ v_, double *__restrict__ w_,
+                                                      double *__restrict__ P_, int i, int j, int k) {
+    const auto po = gmem(po_);
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto w = gmem(w_);
+    const auto P = gmem(P_);
     const int m = g.m, n = g.n, l = g.l;
     // an i = 1 cell with interior (j, k) only takes the inlet velocity (its p is untouched)
     const int jj = (j == 0) ? n : (j == n + 1 ? 1 : j);
@@ -368,9 +408,14 @@ __device__ __forceinline__ void boundary_uniform_cell(const PfGeom &g, const Ste
 }
 
 __device__ __forceinline__ void boundary_air_cell(const PfGeom &g, const StepArgs &a, int face,
-                                                  const double *__restrict__ po, double *__restrict__ u,
-                                                  double *__restrict__ v, double *__restrict__ w,
-                                                  double *__restrict__ P, int r, int q) {
+                                                  const double *__restrict__ po_, double *__restrict__ u_,
+                                                  double *__restrict__ v_, double *__restrict__ w_,
+                                                  double *__restrict__ P_, int r, int q) {
+    const auto po = gmem(po_);
+    const auto u = gmem(u_);
+    const auto v = gmem(v_);
+    const auto w = gmem(w_);
+    const auto P = gmem(P_);
     const int m = g.m, n = g.n, l = g.l;
     const double U = a.inlet_velocity, pout = a.outlet_pressure;
 #define OFF(ii, jj, kk) pf_off(g, ii, jj, kk)
@@ -544,13 +589,18 @@ __device__ __forceinline__ CellIdx sweep_cell(const PfGeom &g, const SweepArgs &
 
 template <int DIM, bool FAST>
 __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs &a, int colour, int i, int j, int k,
-                                                int sh, int64_t o, const double *__restrict__ P,
-                                                const double *__restrict__ PHIs, const double *__restrict__ BB,
-                                                const double *__restrict__ phi, float *__restrict__ R,
-                                                float *__restrict__ E) {
+                                                int sh, int64_t o, const double *__restrict__ P_,
+                                                const double *__restrict__ PHIs_, const double *__restrict__ BB_,
+                                                const double *__restrict__ phi, float *__restrict__ R_,
+                                                float *__restrict__ E_) {
+    const auto P = gmem(P_);
+    const auto PHIs = gmem(PHIs_);
+    const auto BB = gmem(BB_);
+    const auto R = gmem(R_);
+    const auto E = gmem(E_);
     const int64_t own = (int64_t)colour * g.HS, oth = (int64_t)(1 - colour) * g.HS;
     const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-    const double *Po = P + own, *Px = P + oth, *Fo = PHIs + own, *Fx = PHIs + oth;
+    const auto Po = P + own, Px = P + oth, Fo = PHIs + own, Fx = PHIs + oth;
     const double pc = Po[o], pW = Px[o - 1 + sh], pE = Px[o + sh], pS = Px[o - HP], pN = Px[o + HP];
     const double fc = Fo[o], fW = Fx[o - 1 + sh], fE = Fx[o + sh], fS = Fx[o - HP], fN = Fx[o + HP];
     Coef c;

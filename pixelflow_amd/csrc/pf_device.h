@@ -24,6 +24,18 @@ struct CoefT {
 };
 using Coef = CoefT<double>;
 
+// Global-memory view of a pointer.  Loads and stores through it compile to global_*
+// instructions.  Pointers that a kernel reads from memory (the case table of a batch) are
+// otherwise generic, and generic accesses compile to flat_* instructions, which count on
+// both the vector-memory and the scalar-memory wait counters: every wait for a scalar
+// load of the case record then also waits for the tile's outstanding field loads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(T *p) {
+    return (gptr<T>)p;
+}
+
 template <typename T>
 __device__ __forceinline__ void zero6(CoefT<T> &c) { c.ae = T(0); c.aw = T(0); c.an = T(0); c.as = T(0); c.at = T(0); c.ab = T(0); }
 
@@ -38,40 +50,40 @@ __device__ void row_rules(const PfGeom &g, const StepArgs &a, int i, int j, int 
     if (a.bc == PF_BC_UNIFORM) {
         if (i == 1) { c.ae = c.ae + c.aw; c.aw = T(0); }
         if (i == g.m) {
-            if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, g.m + 1, j, k)]);
+            if (WITH_BB) *bb = *bb + (c.ae * gmem(P)[pf_soff(g, g.m + 1, j, k)]);
             zero6(c);
         }
         return;
     }
     const int m = g.m, n = g.n, l = g.l;
-    const double pcell = phi[pf_off(g, i, j, k)];
+    const double pcell = gmem(phi)[pf_off(g, i, j, k)];
     const bool fluid = !(pcell < 0.9);
     if (k == l) {
         if (a.face[0] == 0 || a.face[0] == 1 || !fluid) { c.ab = c.ab + c.at; c.at = T(0); }
         else {
-            if (WITH_BB) *bb = bb_k1 + (c.at * P[pf_soff(g, i, j, l + 1)]);
+            if (WITH_BB) *bb = bb_k1 + (c.at * gmem(P)[pf_soff(g, i, j, l + 1)]);
             zero6(c);
         }
     }
     if (k == 1) {
         if (a.face[1] == 0 || a.face[1] == 1 || !fluid) { c.at = c.at + c.ab; c.ab = T(0); }
-        else { if (WITH_BB) *bb = *bb + (c.ab * P[pf_soff(g, i, j, 0)]); zero6(c); }
+        else { if (WITH_BB) *bb = *bb + (c.ab * gmem(P)[pf_soff(g, i, j, 0)]); zero6(c); }
     }
     if (i == m) {
         if (a.face[2] == 0 || a.face[2] == 1 || !fluid) { c.aw = c.aw + c.ae; c.ae = T(0); }
-        else { if (WITH_BB) *bb = *bb + (c.ae * P[pf_soff(g, m + 1, j, k)]); zero6(c); }
+        else { if (WITH_BB) *bb = *bb + (c.ae * gmem(P)[pf_soff(g, m + 1, j, k)]); zero6(c); }
     }
     if (i == 1) {
         if (a.face[3] == 0 || a.face[3] == 1 || !fluid) { c.ae = c.ae + c.aw; c.aw = T(0); }
-        else { if (WITH_BB) *bb = *bb + (c.aw * P[pf_soff(g, 0, j, k)]); zero6(c); }
+        else { if (WITH_BB) *bb = *bb + (c.aw * gmem(P)[pf_soff(g, 0, j, k)]); zero6(c); }
     }
     if (j == n) {
         if (a.face[4] == 0 || a.face[4] == 1 || !fluid) { c.as = c.as + c.an; c.an = T(0); }
-        else { if (WITH_BB) *bb = *bb + (c.an * P[pf_soff(g, i, n + 1, k)]); zero6(c); }
+        else { if (WITH_BB) *bb = *bb + (c.an * gmem(P)[pf_soff(g, i, n + 1, k)]); zero6(c); }
     }
     if (j == 1) {
         if (a.face[5] == 0 || a.face[5] == 1 || !fluid) { c.an = c.an + c.as; c.as = T(0); }
-        else { if (WITH_BB) *bb = *bb + (c.as * P[pf_soff(g, i, 0, k)]); zero6(c); }
+        else { if (WITH_BB) *bb = *bb + (c.as * gmem(P)[pf_soff(g, i, 0, k)]); zero6(c); }
     }
 }
 

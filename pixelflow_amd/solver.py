@@ -425,7 +425,7 @@ class NativeServer:
         self.params: Dict[int, CaseParams] = {}
         if batch_config:
             c = PfBatchConfig(int(batch_config.get("persistent_blocks", 0)), 1 if batch_config.get("use_graph", True) else 0,
-                              int(batch_config.get("inner_iters", 0)), int(batch_config.get("tile_x", 64)))
+                              int(batch_config.get("inner_iters", 0)), int(batch_config.get("tile_x", 0)))
             rc = lib().pf_batch_set_config(lib().pf_serve_batch(self._h), ctypes.byref(c))
             if rc != 0:
                 raise PixelFlowError(f"pf_batch_set_config: {ERRORS.get(rc, rc)}")

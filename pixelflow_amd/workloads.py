@@ -89,7 +89,7 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
 def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
                  inner: int = 50, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
                  plain_sample: Optional[int] = None, plain_steps: int = 10,
-                 log: Callable = _noop) -> Dict[str, object]:
+                 batch_config: Optional[dict] = None, log: Callable = _noop) -> Dict[str, object]:
     """Poisson-arriving room requests through the library's serving loop (pf_serve_*): a
     host thread submits each request at its arrival time while the worker steps the
     resident ones."""
@@ -108,7 +108,7 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
     log(f"[serve] {n_requests} requests, rate {rate_per_s}/s, {steps} steps each; largest case {peak / 2**20:.0f} MiB")
     free, _total = device_memory()
     with NativeServer(kind, max_cases=min(n_requests, 64), arena_bytes=int(free * 0.3), chunk=chunk,
-                      precision=PF_PREC_MIXED, **kw) as srv:
+                      precision=PF_PREC_MIXED, batch_config=batch_config, **kw) as srv:
         ids = []
         t0 = time.perf_counter()
         for (p, phi), at in zip(reqs, arrivals):
