@@ -18,6 +18,7 @@ struct BatchCase {
     double *P, *BB, *PHIs;        // colour-split FP64 (2 * HS)
     float *E, *R, *PHI32;         // colour-split FP32 (mixed precision)
     int nat_gx, nat_gy;           // 64 x 4 tiles over (m+2) x (n+2), one plane each
+    int face_gx[3];               // 64-wide tile columns of each face shape (xy, yz, xz)
     int swp_gx, swp_gy;           // colour-sweep tiles (sweep_shape with one cell per thread)
     int fuse_y, fuse_z;           // periodic ghost copies written inside the sweep
     int ref_y, ref_z;             // periodic ghost refresh pass needed (odd extents)
@@ -33,9 +34,11 @@ struct BatchView {
     const int *swp_prefix;
     const int2 *nat_tile;
     const int2 *swp_tile;
+    const int2 *face_tile[3];     // 64 x 4 tiles over one face: xy (top/bottom), yz (west/east), xz
     const int *active;            // per case: 1 = takes part in the current cycle
     int ncases;
     int nat_tiles, swp_tiles;
+    int face_tiles[3];
 };
 
 }  // namespace pf

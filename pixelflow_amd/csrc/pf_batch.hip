@@ -238,6 +238,20 @@ __global__ __launch_bounds__(256) void k_batch_swp(BatchView v, int colour, int 
     }
 }
 
+// Air-conditioning boundary of one face for every case: 64 x 4 tiles over the face only
+// (top/bottom: (m+2) x (n+2); west/east: (n+2) x (l+2); north/south: (m+2) x (l+2)).
+__global__ __launch_bounds__(256) void k_batch_face(BatchView v, int parity, int face) {
+    const int shape = face >> 1;
+    const int G = (int)gridDim.x;
+    const int t = xcd_logical_block((int)blockIdx.x, G);
+    if (t >= v.face_tiles[shape]) return;
+    const int2 ti = v.face_tile[shape][t];
+    const BatchCase &C = v.cases[ti.x];
+    const int gx = C.face_gx[shape];
+    const int r = (ti.y % gx) * 64 + (int)threadIdx.x, q = (ti.y / gx) * 4 + (int)threadIdx.y;
+    boundary_air_cell(C.g, C.a, face, C.phi, C.U[parity], C.V[parity], C.W[parity], C.P, r, q);
+}
+
 // Per case: max over its tile slots -> hist[step * stride + slot] (step from *d_step, or
 // the fixed row `row` when row >= 0); the slots are cleared.  One block per case.
 __global__ void k_batch_reduce(BatchView v, unsigned long long *slots, unsigned long long *hist, const int *d_step,
@@ -340,6 +354,8 @@ struct pf_batch {
     BatchCase *d_cases = nullptr;
     int *d_nat_prefix = nullptr, *d_swp_prefix = nullptr, *d_active = nullptr, *d_step = nullptr;
     int2 *d_nat_tile = nullptr, *d_swp_tile = nullptr;   // per tile: case, tile within the case
+    int2 *d_face_tile[3] = {nullptr, nullptr, nullptr};
+    int face_tiles[3] = {0, 0, 0}, face_cap[3] = {0, 0, 0};
     int nat_tiles = 0, swp_tiles = 0, table_cap = 0, tile_cap = 0, nat_cap = 0;
     unsigned long long *d_tile_slots = nullptr;   // per-tile SOR error maxima
     unsigned long long *d_res_slots = nullptr;    // per-tile residual maxima
@@ -458,6 +474,7 @@ extern "C" int pf_batch_destroy(pf_batch *b) {
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graphs(b);
     void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_nat_tile, b->d_swp_tile,
+                    b->d_face_tile[0], b->d_face_tile[1], b->d_face_tile[2],
                     b->d_active, b->d_step,
                     b->d_nactive, b->d_tile_slots, b->d_res_slots, b->d_err_hist, b->d_res_hist,
                     b->d_it_hist, b->d_cy_hist};
@@ -518,6 +535,7 @@ static int rebuild_tables(pf_batch *b) {
     const int nc = (int)b->order.size();
     std::vector<BatchCase> cs;
     std::vector<int> np(1, 0), sp(1, 0);
+    std::vector<int> fp[3] = {std::vector<int>(1, 0), std::vector<int>(1, 0), std::vector<int>(1, 0)};
     b->any_refresh = false;
     b->any_fast = b->any_slow = false;
     for (int s : b->order) {
@@ -531,6 +549,14 @@ static int rebuild_tables(pf_batch *b) {
         const int kb = tile_kb(b);
         const int swp_gz = (g.dim == 3) ? (g.l + 2 * kb - 1) / (2 * kb) : 1;
         np.push_back(np.back() + bc.nat_gx * bc.nat_gy * g.Z);
+        {
+            const int X = g.m + 2, Y = g.n + 2, Zf = g.l + 2;
+            const int R[3] = {X, Y, X}, Q[3] = {Y, Zf, Zf};
+            for (int f = 0; f < 3; ++f) {
+                bc.face_gx[f] = (R[f] + 63) / 64;
+                fp[f].push_back(fp[f].back() + ((g.dim == 3 && b->kind.bc == PF_BC_AIR) ? bc.face_gx[f] * ((Q[f] + 3) / 4) : 0));
+            }
+        }
         sp.push_back(sp.back() + bc.swp_gx * bc.swp_gy * swp_gz);
         b->any_refresh = b->any_refresh || bc.ref_y || bc.ref_z;
         b->any_fast = b->any_fast || bc.fast;
@@ -540,6 +566,7 @@ static int rebuild_tables(pf_batch *b) {
     }
     b->nat_tiles = np.back();
     b->swp_tiles = sp.back();
+    for (int f = 0; f < 3; ++f) b->face_tiles[f] = fp[f].back();
     if (b->table_cap < nc + 1) {
         if (b->d_cases) (void)hipFree(b->d_cases);
         if (b->d_nat_prefix) (void)hipFree(b->d_nat_prefix);
@@ -573,8 +600,26 @@ static int rebuild_tables(pf_batch *b) {
         PFB_TRY(b, hipMalloc((void **)&b->d_nat_tile, sizeof(int2) * (size_t)cap));
         b->nat_cap = cap;
     }
+    for (int f = 0; f < 3; ++f) {
+        if (b->face_cap[f] < b->face_tiles[f] + 1) {
+            if (b->d_face_tile[f]) (void)hipFree(b->d_face_tile[f]);
+            b->d_face_tile[f] = nullptr;
+            const int cap = b->face_tiles[f] + 1024;
+            PFB_TRY(b, hipMalloc((void **)&b->d_face_tile[f], sizeof(int2) * (size_t)cap));
+            b->face_cap[f] = cap;
+        }
+    }
     // host sources of the asynchronous copies live until the synchronisation below
     std::vector<int2> nt((size_t)b->nat_tiles), st((size_t)b->swp_tiles);
+    std::vector<int2> ft[3];
+    for (int f = 0; f < 3; ++f) {
+        ft[f].resize((size_t)b->face_tiles[f]);
+        for (int c = 0; c < nc; ++c)
+            for (int t = fp[f][(size_t)c]; t < fp[f][(size_t)c + 1]; ++t) ft[f][(size_t)t] = make_int2(c, t - fp[f][(size_t)c]);
+        if (b->face_tiles[f] > 0)
+            PFB_TRY(b, hipMemcpyAsync(b->d_face_tile[f], ft[f].data(), sizeof(int2) * ft[f].size(), hipMemcpyHostToDevice,
+                                      b->st));
+    }
     std::vector<int> ones((size_t)nc, 1);
     if (nc > 0) {
         for (int c = 0; c < nc; ++c) {
@@ -601,6 +646,10 @@ static pf::BatchView view(const pf_batch *b) {
     v.swp_prefix = b->d_swp_prefix;
     v.nat_tile = b->d_nat_tile;
     v.swp_tile = b->d_swp_tile;
+    for (int f = 0; f < 3; ++f) {
+        v.face_tile[f] = b->d_face_tile[f];
+        v.face_tiles[f] = b->face_tiles[f];
+    }
     v.active = b->d_active;
     v.ncases = (int)b->order.size();
     v.nat_tiles = b->nat_tiles;
@@ -853,9 +902,17 @@ static void enqueue_pre(pf_batch *b, int p) {
 
 static void enqueue_post(pf_batch *b, int p) {
     launch_nat(b, pf::NAT_PROJECT, p, 0);
-    if (b->kind.bc == PF_BC_UNIFORM) launch_nat(b, pf::NAT_BOUNDARY_UNIFORM, p, 0);
-    else
-        for (int f = 0; f < 6; ++f) launch_nat(b, pf::NAT_BOUNDARY_AIR, p, f);
+    if (b->kind.bc == PF_BC_UNIFORM) {
+        launch_nat(b, pf::NAT_BOUNDARY_UNIFORM, p, 0);
+    } else {
+        // faces in the reference's order (top, bottom, west, east, north, south): later
+        // faces overwrite the edges they share with earlier ones
+        const pf::BatchView v = view(b);
+        for (int f = 0; f < 6; ++f) {
+            const int n = v.face_tiles[f >> 1];
+            if (n > 0) hipLaunchKernelGGL(pf::k_batch_face, dim3((unsigned)n), dim3(64, 4, 1), 0, b->st, v, p, f);
+        }
+    }
 }
 
 static void enqueue_step_fixed(pf_batch *b, int p) {
