@@ -181,6 +181,8 @@ def main(argv=None):
     ap.add_argument("--serve-requests", type=int, default=16,
                     help="requests of the served 3D-room workload (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--rooms-batch", type=int, default=16,
+                    help="the served rooms resident together in one batch, against each alone (0 = skip)")
     ap.add_argument("--serve-steps", type=int, default=300)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain device-copy calibration kernel and print its GB/s")
@@ -278,6 +280,10 @@ def main(argv=None):
                                               steps=args.serve_steps, log=log)
         log(f"[bench] serve: p50 {res['serve']['p50_time_to_solution_s']:.2f} s, "
             f"p99 {res['serve']['p99_time_to_solution_s']:.2f} s")
+    if args.rooms_batch > 0:
+        from pixelflow_amd import workloads
+        res["rooms_batch"] = workloads.rooms_batch(n_requests=args.rooms_batch, log=log)
+        log(f"[bench] rooms batch: mixed {res['rooms_batch']['batch_mixed_vs_alone_fp64']:.2f}x the rooms alone in FP64")
     if args.cpu_steps > 0:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")

@@ -4,7 +4,9 @@
   FP64 defect correction, tuned by the autotuner, against the plain one-case-at-a-time FP64
   path on a sample of the same cases;
 * :func:`served_rooms` -- Poisson-arriving 3D room-ventilation requests through the
-  continuous-batching server, mixed precision, reporting p50/p99 time-to-solution.
+  continuous-batching server, mixed precision, reporting p50/p99 time-to-solution;
+* :func:`rooms_batch` -- the same rooms resident together in one batch (no arrivals), FP64
+  and mixed, beside each room run alone: the batch ceiling the server works under.
 """
 from __future__ import annotations
 
@@ -86,6 +88,18 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
     }
 
 
+def _room_requests(n_requests: int, rate_per_s: float, steps: int, iter_max: int, seed: int, sizes=None):
+    """The served rooms and their arrival times (seconds from the first)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, size=n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for r in range(n_requests):
+        size = None if sizes is None else sizes[r % len(sizes)]
+        reqs.append(cases.room_case(int(rng.integers(0, 2**31 - 1)), steps=steps, iter_max=iter_max, size=size))
+    return reqs, arrivals
+
+
 def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
                  inner: int = 50, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
                  plain_sample: Optional[int] = None, plain_steps: int = 10,
@@ -94,14 +108,7 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
     host thread submits each request at its arrival time while the worker steps the
     resident ones."""
     from .solver import NativeServer, SERVE_DONE
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate_per_s, size=n_requests)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    reqs = []
-    for r in range(n_requests):
-        size = None if sizes is None else sizes[r % len(sizes)]
-        p, phi = cases.room_case(int(rng.integers(0, 2**31 - 1)), steps=steps, iter_max=iter_max, size=size)
-        reqs.append((p, phi))
+    reqs, arrivals = _room_requests(n_requests, rate_per_s, steps, iter_max, seed, sizes)
     kind = reqs[0][0]
     kw = dict(inner_iters=inner, residual_tol=tol, max_cycles=max(1, iter_max // inner))
     peak = max(batch_case_bytes(p, precision=PF_PREC_MIXED, **kw) for p, _ in reqs)
@@ -163,3 +170,40 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
         "final_step_iterations_median": float(np.median([l["sor_iters"] for l in lasts])),
         "sizes": sorted({f"{p.m}x{p.n}x{p.l}" for p, _ in reqs}),
     }
+
+
+def rooms_batch(n_requests: int = 16, steps: int = 4, iter_max: int = 100, inner: int = 50, seed: int = 5,
+                rate_per_s: float = 2.0, log: Callable = _noop) -> Dict[str, object]:
+    """The served rooms (same generator and seed) resident together in one batch, fixed
+    count: SOR cell-updates/s of the batch in FP64 and in mixed precision, and of each room
+    run alone on the single-case path in both precisions."""
+    from .solver import PF_PREC_FP64
+    reqs, _ = _room_requests(n_requests, rate_per_s, steps + 1, iter_max, seed)
+    cells = sum(p.cells for p, _ in reqs)
+    log(f"[rooms] {n_requests} rooms, {cells} cells, {steps} steps")
+    out: Dict[str, object] = {"workload": f"rooms_batch_{n_requests}", "cells": cells, "steps": steps,
+                              "iter_max": iter_max, "inner_iters": inner}
+    for label, kw in (("fp64", dict(precision=PF_PREC_FP64)), ("mixed", dict(precision=PF_PREC_MIXED, inner_iters=inner))):
+        arena = sum(batch_case_bytes(p, **kw) for p, _ in reqs) + (256 << 20)
+        with Batch(reqs[0][0], max_cases=n_requests, arena_bytes=arena, **kw) as b:
+            slots = [b.add(p, phi) for p, phi in reqs]
+            b.step(1)
+            t0 = time.perf_counter()
+            st = b.step(steps)
+            dt = time.perf_counter() - t0
+        upd = float(sum(p.cells * st["sor_iters"][:, s].sum() for s, (p, _) in zip(slots, reqs)))
+        out[f"batch_{label}_sor_cell_updates_per_s"] = upd / dt
+        out[f"batch_{label}_ms_per_step"] = dt * 1e3 / steps
+        t_alone = 0.0
+        for p, phi in reqs:
+            with Solver(p, phi, **kw) as s1:
+                s1.step(1)
+                s1.sync()
+                t0 = time.perf_counter()
+                s1.step(steps)
+                s1.sync()
+                t_alone += time.perf_counter() - t0
+        out[f"alone_{label}_sor_cell_updates_per_s"] = cells * iter_max * steps / t_alone
+    out["batch_mixed_vs_alone_fp64"] = out["batch_mixed_sor_cell_updates_per_s"] / out["alone_fp64_sor_cell_updates_per_s"]
+    out["alone_mixed_vs_alone_fp64"] = out["alone_mixed_sor_cell_updates_per_s"] / out["alone_fp64_sor_cell_updates_per_s"]
+    return out
