@@ -38,6 +38,8 @@ struct BatchView {
     const int *active;            // per case: 1 = takes part in the current cycle
     int ncases;
     int nat_tiles, swp_tiles;
+    int swp_base, swp_end;        // the tile range a launch covers (a group of whole cases)
+    int nat_base, nat_end;
     int face_tiles[3];
 };
 

@@ -124,9 +124,9 @@ __global__ __launch_bounds__(256) void k_batch_nat(BatchView v, int parity, int 
     const int G = (int)gridDim.x;
     const int L = xcd_logical_block((int)blockIdx.x, G);
     if (PERSIST) {
-        for (int t = L; t < v.nat_tiles; t += G) nat_tile<OP>(v, t, parity, aux);
-    } else if (L < v.nat_tiles) {
-        nat_tile<OP>(v, L, parity, aux);
+        for (int t = v.nat_base + L; t < v.nat_end; t += G) nat_tile<OP>(v, t, parity, aux);
+    } else if (v.nat_base + L < v.nat_end) {
+        nat_tile<OP>(v, v.nat_base + L, parity, aux);
     }
 }
 
@@ -232,9 +232,9 @@ __global__ __launch_bounds__(256) void k_batch_swp(BatchView v, int colour, int 
     const int G = (int)gridDim.x;
     const int L = xcd_logical_block((int)blockIdx.x, G);
     if (PERSIST) {
-        for (int t = L; t < v.swp_tiles; t += G) swp_tile<DIM, KIND, TX, SEL, KB>(v, t, colour, track, slots);
-    } else if (L < v.swp_tiles) {
-        swp_tile<DIM, KIND, TX, SEL, KB>(v, L, colour, track, slots);
+        for (int t = v.swp_base + L; t < v.swp_end; t += G) swp_tile<DIM, KIND, TX, SEL, KB>(v, t, colour, track, slots);
+    } else if (v.swp_base + L < v.swp_end) {
+        swp_tile<DIM, KIND, TX, SEL, KB>(v, v.swp_base + L, colour, track, slots);
     }
 }
 
@@ -368,6 +368,14 @@ struct pf_batch {
     bool any_fast = false, any_slow = false;   // cases on the fast / IEEE division path
     int tile_x = 64;                // sweep tile width in slots (32, 64 or 128)
     int swp_kb = 1;                 // 3D sweep tiles: cells per thread along z (1, 2 or 4)
+    // The solve runs its sweeps group by group: consecutive cases whose sweep arrays
+    // together stay under group_bytes (0: one group) iterate while those arrays stay
+    // in the last-level cache, then the next group.
+    int64_t group_bytes = 0;        // set at creation (pf_batch_create)
+    struct Group {
+        int swp0, swp1, nat0, nat1;
+    };
+    std::vector<Group> groups;
     int parity = 0;
     pf_batch_config cfg{};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -441,7 +449,13 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
     b->cfg.tile_x = (kind->dim == 3) ? 32 : 64;
     b->tile_x = b->cfg.tile_x;
     b->swp_kb = (kind->dim == 3 && kind->precision == PF_PREC_MIXED) ? 2 : 1;
+    // Sweep groups of at most 192 MB (mixed) / 240 MB (FP64) of arrays, against the 256 MB
+    // Infinity Cache: measured on the 16 served rooms (56 M cells) as 1.17x (mixed) and
+    // 1.24x (FP64) the rate of sweeping all cases in every launch; 320 MB and more lose it.
+    b->group_bytes = (int64_t)(kind->precision == PF_PREC_MIXED ? 192 : 240) << 20;
     {
+        const char *gk = std::getenv("PF_BATCH_GROUP_KB");   // 0: every case in every launch
+        if (gk && *gk) b->group_bytes = (int64_t)std::max(0, std::atoi(gk)) << 10;
         const char *kb = std::getenv("PF_BATCH_KB");
         if (kb && *kb) {
             const int v = std::atoi(kb);
@@ -566,6 +580,23 @@ static int rebuild_tables(pf_batch *b) {
     }
     b->nat_tiles = np.back();
     b->swp_tiles = sp.back();
+    b->groups.clear();
+    {
+        // sweep arrays per slot: unknown (both colours), porosity, right-hand side
+        const int64_t per_slot = (b->kind.precision == PF_PREC_MIXED) ? 3 * 4 : 3 * 8;
+        int64_t acc = 0;
+        int start = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int64_t bytes = 2 * cs[(size_t)c].g.HS * per_slot;
+            if (c > start && b->group_bytes > 0 && acc + bytes > b->group_bytes) {
+                b->groups.push_back({sp[(size_t)start], sp[(size_t)c], np[(size_t)start], np[(size_t)c]});
+                start = c;
+                acc = 0;
+            }
+            acc += bytes;
+        }
+        if (nc > 0) b->groups.push_back({sp[(size_t)start], sp[(size_t)nc], np[(size_t)start], np[(size_t)nc]});
+    }
     for (int f = 0; f < 3; ++f) b->face_tiles[f] = fp[f].back();
     if (b->table_cap < nc + 1) {
         if (b->d_cases) (void)hipFree(b->d_cases);
@@ -654,6 +685,10 @@ static pf::BatchView view(const pf_batch *b) {
     v.ncases = (int)b->order.size();
     v.nat_tiles = b->nat_tiles;
     v.swp_tiles = b->swp_tiles;
+    v.swp_base = 0;
+    v.swp_end = b->swp_tiles;
+    v.nat_base = 0;
+    v.nat_end = b->nat_tiles;
     return v;
 }
 
@@ -772,9 +807,13 @@ static int grid_blocks(const pf_batch *b, int tiles) {
     return (P > 0 && P < tiles) ? P : tiles;
 }
 
-static void launch_nat(pf_batch *b, int op, int parity, int aux) {
-    const pf::BatchView v = view(b);
-    const int nb = grid_blocks(b, v.nat_tiles);
+static void launch_nat(pf_batch *b, int op, int parity, int aux, const pf_batch::Group *grp = nullptr) {
+    pf::BatchView v = view(b);
+    if (grp) {
+        v.nat_base = grp->nat0;
+        v.nat_end = grp->nat1;
+    }
+    const int nb = grid_blocks(b, v.nat_end - v.nat_base);
     if (nb == 0) return;
     dim3 blk(64, 4, 1), gr((unsigned)nb);
 #define PFB_NAT(OP)                                                                              \
@@ -840,9 +879,13 @@ static void launch_swp_dim(pf_batch *b, int kind, dim3 gr, const pf::BatchView &
     }
 }
 
-static void launch_swp(pf_batch *b, int kind, int colour, int track) {
-    const pf::BatchView v = view(b);
-    const int nb = grid_blocks(b, v.swp_tiles);
+static void launch_swp(pf_batch *b, int kind, int colour, int track, const pf_batch::Group *grp = nullptr) {
+    pf::BatchView v = view(b);
+    if (grp) {
+        v.swp_base = grp->swp0;
+        v.swp_end = grp->swp1;
+    }
+    const int nb = grid_blocks(b, v.swp_end - v.swp_base);
     if (nb == 0) return;
     const dim3 gr((unsigned)nb);
     unsigned long long *slots = (kind == pf::SWP_RESIDUAL) ? b->d_res_slots : b->d_tile_slots;
@@ -872,12 +915,14 @@ static void enqueue_solve_fixed(pf_batch *b) {
     const int first = (dim == 3) ? 0 : 1;
     const int iters = b->kind.iter_max;
     auto sweeps = [&](int kind, int count) {
-        for (int it = 0; it < count; ++it)
-            for (int half = 0; half < 2; ++half) {
-                const int colour = half == 0 ? first : 1 - first;
-                launch_swp(b, kind, colour, (dim == 2) || (half == 1));
-                if (b->any_refresh) launch_nat(b, kind == pf::SWP_FP64 ? pf::NAT_REFRESH_P : pf::NAT_REFRESH_E, 0, 0);
-            }
+        for (const pf_batch::Group &grp : b->groups)
+            for (int it = 0; it < count; ++it)
+                for (int half = 0; half < 2; ++half) {
+                    const int colour = half == 0 ? first : 1 - first;
+                    launch_swp(b, kind, colour, (dim == 2) || (half == 1), &grp);
+                    if (b->any_refresh)
+                        launch_nat(b, kind == pf::SWP_FP64 ? pf::NAT_REFRESH_P : pf::NAT_REFRESH_E, 0, 0, &grp);
+                }
     };
     if (b->kind.precision == PF_PREC_FP64) {
         sweeps(pf::SWP_FP64, iters);
@@ -974,12 +1019,13 @@ static void enqueue_tol_cycle(pf_batch *b, int cyc) {
                        b->d_it_hist, b->d_cy_hist, b->d_step, b->max_cases, b->kind.residual_tol, cyc, maxc, K,
                        b->d_nactive);
     if (cyc == maxc) return;
-    for (int it = 0; it < K; ++it)
-        for (int half = 0; half < 2; ++half) {
-            const int colour = half == 0 ? first : 1 - first;
-            launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1));
-            if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0);
-        }
+    for (const pf_batch::Group &grp : b->groups)
+        for (int it = 0; it < K; ++it)
+            for (int half = 0; half < 2; ++half) {
+                const int colour = half == 0 ? first : 1 - first;
+                launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1), &grp);
+                if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0, &grp);
+            }
     if (mixed) launch_nat(b, pf::NAT_UPDATE, 0, 0);
 }
 

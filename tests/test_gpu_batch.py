@@ -159,3 +159,35 @@ def test_batch_3d_cells_per_thread_matches_single(kb, precision, monkeypatch):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"kb{kb} {p.m}x{p.n}x{p.l} {f}")
             np.testing.assert_array_equal(np.concatenate([st["sor_error"][:, s], st2["sor_error"][:, s]]),
                                           rst["sor_error"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_grouped_solve_matches_single(precision, monkeypatch):
+    """Solve sweeps run group by group (here every case its own group, 64 KiB cap): every
+    case keeps the bits it gives alone, fixed count and tolerance mode."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Batch
+    monkeypatch.setenv("PF_BATCH_GROUP_KB", "64")
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=10) if precision == "mixed" else {}
+    cs = _odd_even_3d_cases()
+    with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 28, **kw) as b:
+        b.set_config(inner_iters=kw.get("inner_iters", 0))
+        slots = [b.add(p, phi) for p, phi in cs]
+        st = b.step(3)
+        for (p, phi), s in zip(cs, slots):
+            ref, rst = _single(p, phi, 3, **kw)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"grouped {p.m}x{p.n}x{p.l} {f}")
+            np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])
+    rooms = [cases.room_case(seed, steps=3, iter_max=20, size=size)
+             for seed, size in ((1, (24, 24, 20)), (2, (32, 24, 24)), (3, (20, 28, 22)))]
+    tkw = dict(precision=PF_PREC_MIXED if precision == "mixed" else 0, inner_iters=10, residual_tol=1e-6,
+               max_cycles=4)
+    with Batch(rooms[0][0], max_cases=4, arena_bytes=1 << 27, **tkw) as b:
+        b.set_config(inner_iters=10)
+        slots = [b.add(p, phi) for p, phi in rooms]
+        st = b.step(3)
+        for (p, phi), s in zip(rooms, slots):
+            ref, rst = _single(p, phi, 3, **tkw)
+            np.testing.assert_array_equal(st["sor_iters"][:, s], rst["sor_iters"])
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"grouped room {p.m}x{p.n}x{p.l} {f}")
