@@ -51,6 +51,62 @@ def measured_traffic(kernel: str, workload: str):
     return d.get("bytes_per_launch")
 
 
+def cpu_baseline_reference(p, phi, steps: int, threads: int):
+    """Time the reference's own OpenMP build (oracle/_ref/fp64/ref_ibm_3d_uniform, compiled
+    from its sources with -fdefault-real-8) on the same case for steps + 1 steps.  Its
+    output goes to a pseudo-terminal, so the Fortran runtime writes each log line as it
+    happens; the time between the first and the last per-step 'SOR iteration' line is
+    `steps` whole steps, without the CSV read before them or the output files after (the
+    process is stopped once the last step is logged).  Returns (cell-steps/s, seconds) or
+    None when the build is absent."""
+    import dataclasses
+    import pty
+    import select
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "fp64", "ref_ibm_3d_uniform")
+    if not os.access(exe, os.X_OK):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    q = dataclasses.replace(p, istep_max=steps + 1, time=p.dt * (steps + 1), istep_out=10 ** 6)
+    stamps = []
+    with tempfile.TemporaryDirectory() as root:
+        cases.write_case_dir(root, q, phi)
+        master, slave = pty.openpty()
+        proc = subprocess.Popen([exe], cwd=root, stdin=subprocess.DEVNULL, stdout=slave, stderr=slave, env=env)
+        os.close(slave)
+        buf = b""
+        deadline = time.time() + 600
+        try:
+            while len(stamps) < steps + 1 and time.time() < deadline:
+                ready, _, _ = select.select([master], [], [], 5.0)
+                if not ready:
+                    if proc.poll() is not None:
+                        break
+                    continue
+                try:
+                    chunk = os.read(master, 65536)
+                except OSError:
+                    break
+                if not chunk:
+                    break
+                now = time.perf_counter()
+                buf += chunk
+                while b"\n" in buf:
+                    line, buf = buf.split(b"\n", 1)
+                    if b"SOR iteration" in line:
+                        stamps.append(now)
+        finally:
+            if proc.poll() is None:
+                proc.kill()
+            proc.wait()
+            os.close(master)
+    if len(stamps) < steps + 1:
+        return None
+    dt = stamps[-1] - stamps[0]
+    return p.cells * steps / dt, dt
+
+
 def cpu_baseline(p, phi, steps: int, threads: int):
     """Time the oracle (CPU restatement of the reference) for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -166,6 +222,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--cpu-ref-steps", type=int, default=5,
+                    help="steps timed on the reference's own OpenMP build, when built (0 = skip)")
     ap.add_argument("--m", type=int, default=256)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--l", type=int, default=128)
@@ -288,8 +346,20 @@ def main(argv=None):
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")
         v, dt = cpu_baseline(p, phi, args.cpu_steps, threads)
-        res["cpu_baseline"] = {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-                               "sample": f"{args.cpu_steps} steps of the same {workload} workload ({dt:.1f} s)"}
+        port = {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+                "sample": f"{args.cpu_steps} steps of the same {workload} workload on the C restatement ({dt:.1f} s)"}
+        ref = None
+        if args.cpu_ref_steps > 0:
+            log(f"[bench] CPU baseline: reference OpenMP build, {args.cpu_ref_steps} steps on {threads} threads")
+            ref = cpu_baseline_reference(p, phi, args.cpu_ref_steps, threads)
+        if ref is not None:
+            res["cpu_baseline"] = {"value": ref[0], "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+                                   "sample": f"{args.cpu_ref_steps} steps of the same {workload} workload on the "
+                                             f"reference's OpenMP build, FP64 ({ref[1]:.1f} s, from its per-step log "
+                                             f"lines; CSV read and output files excluded)",
+                                   "port": port}
+        else:
+            res["cpu_baseline"] = port
     print(json.dumps(res))
     return 0
 

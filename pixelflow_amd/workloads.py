@@ -102,15 +102,20 @@ def _room_requests(n_requests: int, rate_per_s: float, steps: int, iter_max: int
 
 def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300, iter_max: int = 100,
                  inner: int = 50, tol: float = 1e-8, seed: int = 5, sizes=None, chunk: int = 5,
-                 plain_sample: Optional[int] = None, plain_steps: int = 10,
+                 plain_sample: Optional[int] = None, plain_steps: int = 10, accuracy_sample: int = 2,
                  batch_config: Optional[dict] = None, log: Callable = _noop) -> Dict[str, object]:
     """Poisson-arriving room requests through the library's serving loop (pf_serve_*): a
     host thread submits each request at its arrival time while the worker steps the
-    resident ones."""
+    resident ones.  The `accuracy_sample` largest requests keep their fields, which are
+    compared afterwards with the FP64 path run alone on the same schedule (tolerance mode,
+    the same cap and cycle length): the FP32 rounding of the capped, unconverged solves
+    accumulates over the steps and is largest on the largest rooms."""
     from .solver import NativeServer, SERVE_DONE
     reqs, arrivals = _room_requests(n_requests, rate_per_s, steps, iter_max, seed, sizes)
     kind = reqs[0][0]
     kw = dict(inner_iters=inner, residual_tol=tol, max_cycles=max(1, iter_max // inner))
+    by_size = sorted(range(len(reqs)), key=lambda i: (-reqs[i][0].cells, i))
+    sample = set(by_size[:accuracy_sample])
     peak = max(batch_case_bytes(p, precision=PF_PREC_MIXED, **kw) for p, _ in reqs)
     log(f"[serve] {n_requests} requests, rate {rate_per_s}/s, {steps} steps each; largest case {peak / 2**20:.0f} MiB")
     free, _total = device_memory()
@@ -122,8 +127,11 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
             delay = at - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            ids.append(srv.submit(p, phi, steps, fields=()))
+            keep = ("u", "v", "w", "p") if len(ids) in sample else ()
+            ids.append(srv.submit(p, phi, steps, fields=keep))
         states = [srv.wait(rid) for rid in ids]
+        kept = {i: {f: srv.interior(ids[i], f) for f in ("u", "v", "w", "p")}
+                for i in sorted(sample) if states[i] == SERVE_DONE}
         wall = time.perf_counter() - t0
         times = [srv.times(rid) for rid in ids]
         polls = [srv.poll(rid) for rid in ids]
@@ -152,6 +160,14 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
             plain_upd += p.cells * p.iter_max * plain_steps
     plain = plain_upd / plain_time
     served_busy = sor_upd / max(counters["busy_s"], 1e-9)
+    accuracy = []
+    for i, fields in kept.items():
+        p, phi = reqs[i]
+        with Solver(p, phi, **kw) as s:       # FP64, same tolerance schedule
+            s.step(steps)
+            errs = {f: float(np.linalg.norm(fields[f] - s.interior(f)) / max(np.linalg.norm(s.interior(f)), 1e-300))
+                    for f in ("u", "v", "w", "p")}
+        accuracy.append({"room": f"{p.m}x{p.n}x{p.l}", **errs})
     return {
         "workload": f"served_rooms_{n_requests}_requests",
         "server": "pf_serve (library worker thread, continuous batching)",
@@ -169,6 +185,8 @@ def served_rooms(n_requests: int = 16, rate_per_s: float = 2.0, steps: int = 300
         "final_residual_median": float(np.median([l["residual"] for l in lasts])),
         "final_step_iterations_median": float(np.median([l["sor_iters"] for l in lasts])),
         "sizes": sorted({f"{p.m}x{p.n}x{p.l}" for p, _ in reqs}),
+        "field_rel_l2_vs_fp64": accuracy,
+        "inner_iters": inner,
     }
 
 
