@@ -1,0 +1,33 @@
+"""bench.py pieces that run without a GPU: the one-GPU guard and the CPU baseline leg that
+times the reference's own OpenMP build (skipped where that build is absent)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["n_gpus"] == 2 and "one GPU" in out["error"]
+
+
+@pytest.mark.skipif(not os.access(os.path.join(ROOT, "oracle", "_ref", "fp64", "ref_ibm_3d_uniform"), os.X_OK),
+                    reason="the reference's OpenMP build is not built here (make -C oracle ref)")
+def test_cpu_baseline_times_reference_build_per_step():
+    """The reference build runs steps + 1 steps on a small sphere; the baseline is the time
+    between its first and last per-step log line, so it covers exactly `steps` steps."""
+    import bench
+    from pixelflow_amd import cases
+    p, phi = cases.sphere_case(m=32, n=16, l=16, steps=4, iter_max=20)
+    res = bench.cpu_baseline_reference(p, phi, 3, 2)
+    assert res is not None
+    rate, seconds = res
+    assert seconds > 0 and rate == pytest.approx(p.cells * 3 / seconds)
