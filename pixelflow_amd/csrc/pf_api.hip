@@ -59,10 +59,13 @@ struct pf_ctx {
 
 static thread_local std::string g_create_error;
 
+// On a failure the context's stream is drained before the early return, so no asynchronous
+// copy from or into a host buffer of the calling function is still in flight afterwards.
 #define PF_TRY_HIP(ctx, call)                                                                   \
     do {                                                                                        \
         hipError_t e_ = (call);                                                                 \
         if (e_ != hipSuccess) {                                                                 \
+            if ((ctx)->st) (void)hipStreamSynchronize((ctx)->st);                               \
             (ctx)->msg = std::string(#call) + ": " + hipGetErrorString(e_);                     \
             return PF_EHIP;                                                                     \
         }                                                                                       \

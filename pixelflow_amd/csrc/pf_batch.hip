@@ -392,6 +392,18 @@ struct pf_batch {
         }                                                                                       \
     } while (0)
 
+// For asynchronous copies from or into host vectors: on a failure the stream is drained
+// before the early return, so no copy still in flight outlives the vector it uses.
+#define PFB_TRY_SYNC(b, call)                                                                   \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (void)hipStreamSynchronize((b)->st);                                                \
+            (b)->msg = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+            return PF_EHIP;                                                                     \
+        }                                                                                       \
+    } while (0)
+
 static thread_local std::string g_batch_create_error;
 
 static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
@@ -648,7 +660,7 @@ static int rebuild_tables(pf_batch *b) {
         for (int c = 0; c < nc; ++c)
             for (int t = fp[f][(size_t)c]; t < fp[f][(size_t)c + 1]; ++t) ft[f][(size_t)t] = make_int2(c, t - fp[f][(size_t)c]);
         if (b->face_tiles[f] > 0)
-            PFB_TRY(b, hipMemcpyAsync(b->d_face_tile[f], ft[f].data(), sizeof(int2) * ft[f].size(), hipMemcpyHostToDevice,
+            PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_face_tile[f], ft[f].data(), sizeof(int2) * ft[f].size(), hipMemcpyHostToDevice,
                                       b->st));
     }
     std::vector<int> ones((size_t)nc, 1);
@@ -657,12 +669,12 @@ static int rebuild_tables(pf_batch *b) {
             for (int t = np[(size_t)c]; t < np[(size_t)c + 1]; ++t) nt[(size_t)t] = make_int2(c, t - np[(size_t)c]);
             for (int t = sp[(size_t)c]; t < sp[(size_t)c + 1]; ++t) st[(size_t)t] = make_int2(c, t - sp[(size_t)c]);
         }
-        PFB_TRY(b, hipMemcpyAsync(b->d_nat_tile, nt.data(), sizeof(int2) * nt.size(), hipMemcpyHostToDevice, b->st));
-        PFB_TRY(b, hipMemcpyAsync(b->d_swp_tile, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice, b->st));
-        PFB_TRY(b, hipMemcpyAsync(b->d_cases, cs.data(), sizeof(BatchCase) * (size_t)nc, hipMemcpyHostToDevice, b->st));
-        PFB_TRY(b, hipMemcpyAsync(b->d_nat_prefix, np.data(), sizeof(int) * np.size(), hipMemcpyHostToDevice, b->st));
-        PFB_TRY(b, hipMemcpyAsync(b->d_swp_prefix, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice, b->st));
-        PFB_TRY(b, hipMemcpyAsync(b->d_active, ones.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_nat_tile, nt.data(), sizeof(int2) * nt.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_swp_tile, st.data(), sizeof(int2) * st.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_cases, cs.data(), sizeof(BatchCase) * (size_t)nc, hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_nat_prefix, np.data(), sizeof(int) * np.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_swp_prefix, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(b->d_active, ones.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice, b->st));
     }
     PFB_TRY(b, hipStreamSynchronize(b->st));
     drop_graphs(b);
@@ -744,7 +756,7 @@ extern "C" int pf_batch_add(pf_batch *b, const pf_desc *d, const double *raw, do
     // porosity, its colour-split copies, the fast-division check
     std::vector<double> phi;
     pf::host_porosity(g, d->bc, raw, threshold, phi);
-    PFB_TRY(b, hipMemcpyAsync(bc.phi, phi.data(), phi.size() * 8, hipMemcpyHostToDevice, b->st));
+    PFB_TRY_SYNC(b, hipMemcpyAsync(bc.phi, phi.data(), phi.size() * 8, hipMemcpyHostToDevice, b->st));
     pf::launch_split(b->st, g, bc.phi, bc.PHIs);
     if (bc.PHI32) pf::launch_split32(b->st, g, bc.phi, bc.PHI32);
     int *bad = nullptr, hbad = 1, zero = 0;
@@ -1070,7 +1082,7 @@ static int step_tolerance_eager(pf_batch *b, int p) {
         enqueue_tol_cycle(b, cyc);
         if (cyc < maxc && (cyc + 1) % check_every == 0) {
             int nact = 0;
-            PFB_TRY(b, hipMemcpyAsync(&nact, b->d_nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
+            PFB_TRY_SYNC(b, hipMemcpyAsync(&nact, b->d_nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
             PFB_TRY(b, hipStreamSynchronize(b->st));
             if (nact == 0) break;
         }
@@ -1105,14 +1117,14 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
     }
     std::vector<unsigned long long> eh((size_t)nsteps * b->max_cases), rh((size_t)nsteps * b->max_cases);
     std::vector<int> ih, ch;
-    PFB_TRY(b, hipMemcpyAsync(eh.data(), b->d_err_hist, eh.size() * 8, hipMemcpyDeviceToHost, b->st));
+    PFB_TRY_SYNC(b, hipMemcpyAsync(eh.data(), b->d_err_hist, eh.size() * 8, hipMemcpyDeviceToHost, b->st));
     if (tol || b->kind.report_residual)
-        PFB_TRY(b, hipMemcpyAsync(rh.data(), b->d_res_hist, rh.size() * 8, hipMemcpyDeviceToHost, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(rh.data(), b->d_res_hist, rh.size() * 8, hipMemcpyDeviceToHost, b->st));
     if (tol) {
         ih.resize(eh.size());
         ch.resize(eh.size());
-        PFB_TRY(b, hipMemcpyAsync(ih.data(), b->d_it_hist, ih.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
-        PFB_TRY(b, hipMemcpyAsync(ch.data(), b->d_cy_hist, ch.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(ih.data(), b->d_it_hist, ih.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+        PFB_TRY_SYNC(b, hipMemcpyAsync(ch.data(), b->d_cy_hist, ch.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
     }
     PFB_TRY(b, hipStreamSynchronize(b->st));
     bool finite = true;
@@ -1175,7 +1187,7 @@ extern "C" int pf_batch_divergence(pf_batch *b, int slot, double *out) {
     pf::launch_divergence_out(b->st, bc.g, bc.a, bc.phi, bc.U[q], bc.V[q], bc.W[q], bc.dout);
     PFB_TRY(b, hipGetLastError());
     const size_t n = (size_t)bc.g.m * bc.g.n * bc.g.l;
-    PFB_TRY(b, hipMemcpyAsync(out, bc.dout, n * 8, hipMemcpyDeviceToHost, b->st));
+    PFB_TRY_SYNC(b, hipMemcpyAsync(out, bc.dout, n * 8, hipMemcpyDeviceToHost, b->st));
     PFB_TRY(b, hipStreamSynchronize(b->st));
     return PF_OK;
 }
