@@ -178,6 +178,15 @@ def test_batch_grouped_solve_matches_single(precision, monkeypatch):
             for f in _fields(p):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"grouped {p.m}x{p.n}x{p.l} {f}")
             np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])
+    cs2 = _mixed_2d_cases()     # includes odd extents: the periodic ghost refresh runs per group
+    with Batch(cs2[0][0], max_cases=8, arena_bytes=1 << 28, **kw) as b:
+        b.set_config(inner_iters=kw.get("inner_iters", 0))
+        slots = [b.add(p, phi) for p, phi in cs2]
+        b.step(3)
+        for (p, phi), s in zip(cs2, slots):
+            ref, _ = _single(p, phi, 3, **kw)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"grouped 2D {p.m}x{p.n} {f}")
     rooms = [cases.room_case(seed, steps=3, iter_max=20, size=size)
              for seed, size in ((1, (24, 24, 20)), (2, (32, 24, 24)), (3, (20, 28, 22)))]
     tkw = dict(precision=PF_PREC_MIXED if precision == "mixed" else 0, inner_iters=10, residual_tol=1e-6,
