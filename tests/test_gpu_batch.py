@@ -200,3 +200,36 @@ def test_batch_grouped_solve_matches_single(precision, monkeypatch):
             np.testing.assert_array_equal(st["sor_iters"][:, s], rst["sor_iters"])
             for f in _fields(p):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"grouped room {p.m}x{p.n}x{p.l} {f}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_backstep_variant_beside_plain_inlet(precision):
+    """The backward-step variant (inlet and initial velocity scaled by porosity) in a batch
+    beside the same grid with the plain inlet and a cylinder case: FP64 reproduces the
+    reference's backstep fixture bit for bit, and every case gives the bits it gives alone."""
+    import dataclasses
+
+    from conftest import load_golden
+    from pixelflow_amd.solver import PF_PREC_MIXED, Batch
+    p0, g = load_golden("g2d_backstep")
+    p1 = dataclasses.replace(p0, inlet_porosity=False)
+    m, n = 64, 32
+    dx = 2.0 / (m - 1)
+    dt = cases.stable_dt(dx, dx)             # cell Reynolds number 1
+    p2 = cases.CaseParams(dim=2, m=m, n=n, width=2.0, height=dx * (n - 1), xnue=dx, time=p0.istep_max * dt,
+                          istep_max=p0.istep_max, iter_max=p0.iter_max)
+    cs = [(p0, g["phi_raw"]), (p1, g["phi_raw"]), (p2, cases.random_circles_2d(m, n, 2, (3.0, 6.0), seed=3))]
+    kw = dict(precision=PF_PREC_MIXED, inner_iters=10) if precision == "mixed" else {}
+    with Batch(p0, max_cases=4, arena_bytes=1 << 27, **kw) as b:
+        b.set_config(inner_iters=kw.get("inner_iters", 0))
+        slots = [b.add(p, phi) for p, phi in cs]
+        st = b.step(p0.istep_max)
+        if precision == "fp64":
+            for f in ("u", "v", "p"):
+                np.testing.assert_array_equal(b.interior(slots[0], f), g[f], err_msg=f"backstep {f}")
+            np.testing.assert_array_equal(st["sor_error"][:, slots[0]], g["errs"])
+        for (p, phi), s in zip(cs, slots):
+            ref, rst = _single(p, phi, p0.istep_max, **kw)
+            for f in _fields(p):
+                np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"slot {s} {f}")
+            np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])

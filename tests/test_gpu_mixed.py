@@ -47,7 +47,21 @@ CASES = {
     "2d_c1_256x128": lambda: cases.cylinder_case(steps=30),
     "3d_air_room_40x36x32": lambda: cases.room_case(4, steps=15, iter_max=100, size=(40, 36, 32)),
     "3d_boxes_96x48x48": lambda: cases.boxes_case(m=96, n=48, l=48, steps=10, nboxes=6, side_range=(4, 12)),
+    "2d_backstep_48x24": lambda: _golden_case("g2d_backstep"),
 }
+
+
+# The divergence diagnostic is compared in the max norm, relative to its own maximum.  On the
+# backward step it peaks at the step's corner, where the FP32 rounding of the unconverged
+# pressure iterate is largest: measured 1.4e-6 with cycles of 10 over 8 steps, while the
+# velocities stay within 5e-7 (rel-L2) there.
+DIV_REL_TOL_CASE = {"2d_backstep_48x24": 5e-6}
+
+
+def _golden_case(name):
+    from conftest import load_golden
+    p, g = load_golden(name)
+    return p, g["phi_raw"]
 
 
 @pytest.mark.parametrize("inner", [10, 25])
@@ -59,7 +73,7 @@ def test_mixed_fixed_count_matches_fp64(name, inner):
     print(f"[gpu] mixed {name} inner={inner} rel-L2 {errs} max|ddiv| {ddiv:.3e}", flush=True)
     assert all(e <= FIELD_TOL for f, e in errs.items() if f != "p"), errs
     assert errs["p"] <= P_FIXED_TOL, errs
-    assert ddiv <= DIV_REL_TOL
+    assert ddiv <= DIV_REL_TOL_CASE.get(name, DIV_REL_TOL)
     assert np.all(sb["sor_iters"] == p.iter_max)
     assert np.all(sb["residual"] >= 0)
     # the SOR diagnostic tracks the FP64 one (same iteration, FP32 increments)
