@@ -372,6 +372,12 @@ struct pf_batch {
     // together stay under group_bytes (0: one group) iterate while those arrays stay
     // in the last-level cache, then the next group.
     int64_t group_bytes = 0;        // set at creation (pf_batch_create)
+    // Groups may iterate on side streams, forked from and joined back to st, so that one
+    // group's sweep launches overlap the launch and drain of another's.
+    int nstreams = 1;
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+    hipStream_t lst = nullptr;      // stream of the sweep / refresh launches being enqueued
     struct Group {
         int swp0, swp1, nat0, nat1;
     };
@@ -441,6 +447,15 @@ static void drop_graphs(pf_batch *b) {
     }
 }
 
+static void destroy_streams(pf_batch *b) {
+    for (int q = 0; q < 2; ++q) {
+        if (b->ev_join[q]) (void)hipEventDestroy(b->ev_join[q]);
+        if (b->side[q]) (void)hipStreamDestroy(b->side[q]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->st) (void)hipStreamDestroy(b->st);
+}
+
 extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena_bytes, pf_batch **out) {
     if (!out) return PF_EINVAL;
     *out = nullptr;
@@ -461,13 +476,19 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
     b->cfg.tile_x = (kind->dim == 3) ? 32 : 64;
     b->tile_x = b->cfg.tile_x;
     b->swp_kb = (kind->dim == 3 && kind->precision == PF_PREC_MIXED) ? 2 : 1;
-    // Sweep groups of at most 192 MB (mixed) / 240 MB (FP64) of arrays, against the 256 MB
-    // Infinity Cache: measured on the 16 served rooms (56 M cells) as 1.17x (mixed) and
-    // 1.24x (FP64) the rate of sweeping all cases in every launch; 320 MB and more lose it.
-    b->group_bytes = (int64_t)(kind->precision == PF_PREC_MIXED ? 192 : 240) << 20;
+    // Sweep groups against the 256 MB Infinity Cache.  FP64: groups of at most 240 MB on
+    // one stream, 1.24x the rate of sweeping all cases in every launch on the 16 served
+    // rooms (56 M cells); 320 MB and more lose it, side streams lose 3 %.  Mixed: groups
+    // of at most 96 MB dealt to two side streams, 1.03x (16 rooms) and 1.08x (512 2D cases)
+    // the rate of 192 MB groups on one stream, itself 1.17x the ungrouped rate.
+    const bool mixed = kind->precision == PF_PREC_MIXED;
+    b->group_bytes = (int64_t)(mixed ? 96 : 240) << 20;
+    b->nstreams = mixed ? 2 : 1;
     {
         const char *gk = std::getenv("PF_BATCH_GROUP_KB");   // 0: every case in every launch
         if (gk && *gk) b->group_bytes = (int64_t)std::max(0, std::atoi(gk)) << 10;
+        const char *ns = std::getenv("PF_BATCH_STREAMS");
+        if (ns && *ns) b->nstreams = std::max(1, std::min(2, std::atoi(ns)));
         const char *kb = std::getenv("PF_BATCH_KB");
         if (kb && *kb) {
             const int v = std::atoi(kb);
@@ -476,6 +497,10 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
     }
     b->arena_bytes = align256(arena_bytes);
     hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) e = hipStreamCreateWithFlags(&b->side[q], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    for (int q = 0; q < 2 && e == hipSuccess; ++q) e = hipEventCreateWithFlags(&b->ev_join[q], hipEventDisableTiming);
+    b->lst = b->st;
     if (e == hipSuccess) e = hipMalloc((void **)&b->arena, (size_t)b->arena_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_step, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_active, sizeof(int) * (size_t)max_cases);
@@ -486,7 +511,7 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
         if (b->d_step) (void)hipFree(b->d_step);
         if (b->d_active) (void)hipFree(b->d_active);
         if (b->d_nactive) (void)hipFree(b->d_nactive);
-        if (b->st) (void)hipStreamDestroy(b->st);
+        destroy_streams(b);
         delete b;
         return PF_ENOMEM;
     }
@@ -506,7 +531,9 @@ extern "C" int pf_batch_destroy(pf_batch *b) {
                     b->d_it_hist, b->d_cy_hist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (b->st) (void)hipStreamDestroy(b->st);
+    for (int q = 0; q < 2; ++q)
+        if (b->side[q]) (void)hipStreamSynchronize(b->side[q]);
+    destroy_streams(b);
     delete b;
     return PF_OK;
 }
@@ -831,9 +858,9 @@ static void launch_nat(pf_batch *b, int op, int parity, int aux, const pf_batch:
 #define PFB_NAT(OP)                                                                              \
     do {                                                                                         \
         if (b->cfg.persistent_blocks > 0)                                                        \
-            hipLaunchKernelGGL((pf::k_batch_nat<OP, true>), gr, blk, 0, b->st, v, parity, aux);  \
+            hipLaunchKernelGGL((pf::k_batch_nat<OP, true>), gr, blk, 0, b->lst, v, parity, aux); \
         else                                                                                     \
-            hipLaunchKernelGGL((pf::k_batch_nat<OP, false>), gr, blk, 0, b->st, v, parity, aux); \
+            hipLaunchKernelGGL((pf::k_batch_nat<OP, false>), gr, blk, 0, b->lst, v, parity, aux); \
     } while (0)
     switch (op) {
         case pf::NAT_DIV: PFB_NAT(pf::NAT_DIV); break;
@@ -855,15 +882,15 @@ static void launch_swp_t(pf_batch *b, dim3 gr, const pf::BatchView &v, int colou
     // 3D tiles cover 2 * swp_kb planes (one grid kind per batch: the persistent grid keeps
     // one cell per thread)
     if (b->cfg.persistent_blocks > 0)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, true, 1>), gr, dim3(256), 0, b->lst, v, colour, track, slots);
     else if (D == 3 && b->swp_kb == 2)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 2 : 1>), gr, dim3(256), 0, b->st, v, colour,
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 2 : 1>), gr, dim3(256), 0, b->lst, v, colour,
                            track, slots);
     else if (D == 3 && b->swp_kb == 4)
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 4 : 1>), gr, dim3(256), 0, b->st, v, colour,
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, D == 3 ? 4 : 1>), gr, dim3(256), 0, b->lst, v, colour,
                            track, slots);
     else
-        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 1>), gr, dim3(256), 0, b->st, v, colour, track, slots);
+        hipLaunchKernelGGL((pf::k_batch_swp<D, K, T, SEL, false, 1>), gr, dim3(256), 0, b->lst, v, colour, track, slots);
 }
 
 template <int D, int K, int SEL>
@@ -920,22 +947,43 @@ static void launch_fill_active(pf_batch *b, int val) {
                        b->d_active, v.ncases, val);
 }
 
+// `count` red/black iterations of every group.  With side streams, the groups are dealt to
+// them in turn after a fork from st and joined back to st afterwards (also inside a graph
+// capture, where the fork and join become graph edges).
+static void group_sweeps(pf_batch *b, int kind, int count) {
+    const int dim = b->kind.dim;
+    const int first = (dim == 3) ? 0 : 1;
+    auto one = [&](const pf_batch::Group &grp) {
+        for (int it = 0; it < count; ++it)
+            for (int half = 0; half < 2; ++half) {
+                const int colour = half == 0 ? first : 1 - first;
+                launch_swp(b, kind, colour, (dim == 2) || (half == 1), &grp);
+                if (b->any_refresh) launch_nat(b, kind == pf::SWP_FP64 ? pf::NAT_REFRESH_P : pf::NAT_REFRESH_E, 0, 0, &grp);
+            }
+    };
+    const int S = (b->nstreams > 1 && b->groups.size() > 1) ? b->nstreams : 1;
+    if (S == 1) {
+        for (const pf_batch::Group &grp : b->groups) one(grp);
+        return;
+    }
+    (void)hipEventRecord(b->ev_fork, b->st);
+    for (int q = 0; q < S; ++q) (void)hipStreamWaitEvent(b->side[q], b->ev_fork, 0);
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        b->lst = b->side[gi % (size_t)S];
+        one(b->groups[gi]);
+    }
+    b->lst = b->st;
+    for (int q = 0; q < S; ++q) {
+        (void)hipEventRecord(b->ev_join[q], b->side[q]);
+        (void)hipStreamWaitEvent(b->st, b->ev_join[q], 0);
+    }
+}
+
 // Iterations of the pressure solve (fixed count): FP64 reference sweeps, or FP64 residual +
 // FP32 sweeps + FP64 update cycles.
 static void enqueue_solve_fixed(pf_batch *b) {
-    const int dim = b->kind.dim;
-    const int first = (dim == 3) ? 0 : 1;
     const int iters = b->kind.iter_max;
-    auto sweeps = [&](int kind, int count) {
-        for (const pf_batch::Group &grp : b->groups)
-            for (int it = 0; it < count; ++it)
-                for (int half = 0; half < 2; ++half) {
-                    const int colour = half == 0 ? first : 1 - first;
-                    launch_swp(b, kind, colour, (dim == 2) || (half == 1), &grp);
-                    if (b->any_refresh)
-                        launch_nat(b, kind == pf::SWP_FP64 ? pf::NAT_REFRESH_P : pf::NAT_REFRESH_E, 0, 0, &grp);
-                }
-    };
+    auto sweeps = [&](int kind, int count) { group_sweeps(b, kind, count); };
     if (b->kind.precision == PF_PREC_FP64) {
         sweeps(pf::SWP_FP64, iters);
         return;
@@ -1019,8 +1067,6 @@ static int tol_inner(const pf_batch *b) {
 // one cycle of K sweeps (FP32 on the error equation + FP64 update, or FP64 sweeps) for the
 // cases still active.
 static void enqueue_tol_cycle(pf_batch *b, int cyc) {
-    const int dim = b->kind.dim;
-    const int first = (dim == 3) ? 0 : 1;
     const bool mixed = b->kind.precision == PF_PREC_MIXED;
     const int K = tol_inner(b), maxc = tol_max_cycles(b);
     launch_swp(b, pf::SWP_RESIDUAL, 0, 1);
@@ -1031,13 +1077,7 @@ static void enqueue_tol_cycle(pf_batch *b, int cyc) {
                        b->d_it_hist, b->d_cy_hist, b->d_step, b->max_cases, b->kind.residual_tol, cyc, maxc, K,
                        b->d_nactive);
     if (cyc == maxc) return;
-    for (const pf_batch::Group &grp : b->groups)
-        for (int it = 0; it < K; ++it)
-            for (int half = 0; half < 2; ++half) {
-                const int colour = half == 0 ? first : 1 - first;
-                launch_swp(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, colour, (dim == 2) || (half == 1), &grp);
-                if (b->any_refresh) launch_nat(b, mixed ? pf::NAT_REFRESH_E : pf::NAT_REFRESH_P, 0, 0, &grp);
-            }
+    group_sweeps(b, mixed ? pf::SWP_FP32 : pf::SWP_FP64, K);
     if (mixed) launch_nat(b, pf::NAT_UPDATE, 0, 0);
 }
 

@@ -161,12 +161,15 @@ def test_batch_3d_cells_per_thread_matches_single(kb, precision, monkeypatch):
                                           rst["sor_error"])
 
 
+@pytest.mark.parametrize("streams", [1, 2])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_grouped_solve_matches_single(precision, monkeypatch):
-    """Solve sweeps run group by group (here every case its own group, 64 KiB cap): every
-    case keeps the bits it gives alone, fixed count and tolerance mode."""
+def test_batch_grouped_solve_matches_single(precision, streams, monkeypatch):
+    """Solve sweeps run group by group (here every case its own group, 64 KiB cap), on the
+    batch's stream or dealt to two side streams: every case keeps the bits it gives alone,
+    fixed count and tolerance mode, graph-captured."""
     from pixelflow_amd.solver import PF_PREC_MIXED, Batch
     monkeypatch.setenv("PF_BATCH_GROUP_KB", "64")
+    monkeypatch.setenv("PF_BATCH_STREAMS", str(streams))
     kw = dict(precision=PF_PREC_MIXED, inner_iters=10) if precision == "mixed" else {}
     cs = _odd_even_3d_cases()
     with Batch(cs[0][0], max_cases=4, arena_bytes=1 << 28, **kw) as b:
