@@ -24,7 +24,7 @@ def _noop(*a):
 
 
 def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: int = 11, tune_steps: int = 2,
-               plain_sample: int = 5, plain_steps: int = 20,
+               plain_sample: int = 16, plain_steps: int = 20, accuracy_sample: int = 5,
                log: Callable = _noop) -> Dict[str, object]:
     cl = cases.batch_cases_2d(count=count, steps=steps, seed=seed, iter_max=iter_max)
     kind = cl[0][0]
@@ -32,6 +32,7 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
     arena = sum(batch_case_bytes(p, **kw) for p, _ in cl) + (64 << 20)
     total_cells = sum(p.cells for p, _ in cl)
     sample_idx = list(range(0, len(cl), max(1, len(cl) // plain_sample)))[:plain_sample]
+    accuracy_idx = list(range(0, len(cl), max(1, len(cl) // accuracy_sample)))[:accuracy_sample]
     key = autotune.size_class_key(2, "uniform", True, iter_max, [p.cells for p, _ in cl])
     log(f"[batch] {count} cases, {total_cells} cells, arena {arena / 2**30:.2f} GiB; autotuning")
     with Batch(kind, max_cases=count, arena_bytes=arena, **kw) as tb:
@@ -53,7 +54,7 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
         diverged = int(np.sum(~np.all(np.isfinite(st["sor_error"][:, slots]), axis=0)))
         # accuracy on a sample: the batched mixed run vs each case alone in FP64
         accuracy = []
-        for i in sample_idx:
+        for i in accuracy_idx:
             s = slots[i]
             p, phi = cl[i]
             with Solver(p, phi) as ref:
