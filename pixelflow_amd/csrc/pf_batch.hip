@@ -1,17 +1,20 @@
 // pf_batch.hip -- many independent cases of uneven size advanced together.
 //
 // All cases of a batch live in one device arena; a table of BatchCase records gives each
-// one's geometry, constants and field addresses.  Every stage of the step is ONE launch for
-// the whole batch: the work is cut into tiles (64 x 4 threads; natural-cell tiles for the
-// stencil stages, colour-slot tiles for the sweeps), tiles of all cases are numbered
-// consecutively (prefix sums per case), and a persistent grid of blocks walks the tile list
-// (block b takes tiles b, b + G, b + 2G, ... after an XCD-contiguous renumbering, so each
-// XCD's L2 sees neighbouring tiles).  The per-cell arithmetic is pf_cells.h, shared with the
-// single-case path, so a case gives the same bits alone and inside any batch.
-// The launch sequence of one step is captured once into a hipGraph per velocity parity and
-// replayed; per-step diagnostics go to a history indexed by a device step counter.
-// Cases can be added and removed between steps (continuous batching): the arena has a
-// first-fit allocator, and the tables and graphs are rebuilt when membership changes.
+// one's geometry, constants and field addresses.  Every stencil stage of the step is ONE
+// launch for the whole batch: the work is cut into tiles (natural-cell tiles for the
+// stencil stages, colour-slot tiles for the sweeps, face tiles for the air boundary), the
+// tiles of all cases are numbered consecutively, and a per-tile table gives each block its
+// case.  Blocks take one tile each (or a persistent grid walks the tile list), after an
+// XCD-contiguous renumbering so each XCD's L2 sees neighbouring tiles.  The SOR sweeps run
+// per group of consecutive cases sized for the Infinity Cache (dealt to side streams in
+// mixed precision), since sweeps do not couple cases.  The per-cell arithmetic is
+// pf_cells.h, shared with the single-case path, so a case gives the same bits alone and
+// inside any batch.  The launch sequence of one step is captured once into a hipGraph per
+// velocity parity and replayed; per-step diagnostics go to a history indexed by a device
+// step counter.  Cases can be added and removed between steps (continuous batching): the
+// arena has a first-fit allocator, and the tables and graphs are rebuilt when membership
+// changes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
