@@ -253,7 +253,7 @@ def main(argv=None):
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py runs on one GPU; multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         return 2
 

@@ -1,6 +1,6 @@
 /*
  * pf_oracle.c -- CPU restatement of PixelFlow's time step.  TEST INFRASTRUCTURE ONLY:
- * the checker for the HIP path and the CPU baseline timed beside it.  The product never
+ * the results the HIP path is compared with, and the CPU baseline timed beside it.  The product never
  * links, loads or calls this file.
  *
  * Each function restates the reference's Fortran (paths relative to the reference tree)

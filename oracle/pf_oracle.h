@@ -1,7 +1,8 @@
 /*
  * pf_oracle.h -- CPU restatement of PixelFlow's time step (TEST INFRASTRUCTURE ONLY).
  *
- * This library is the checker for the HIP path and the CPU baseline timed beside it.
+ * This library gives the results the HIP path is compared with, and the CPU baseline timed
+ * beside it.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
  * product path never does. Every function follows the reference's Fortran line by line
  * in arithmetic order (file:line cited at each definition in pf_oracle.c), compiled with

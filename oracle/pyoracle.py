@@ -1,7 +1,8 @@
 """ctypes wrapper of the C restatement in oracle/pf_oracle.c.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.  The
+cpu_baseline leg of bench.py, always as the reference results the HIP path is compared
+with, or as the timed CPU baseline.  The
 product (pixelflow_amd) never imports it.
 """
 from __future__ import annotations
