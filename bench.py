@@ -15,8 +15,12 @@ sweeps), projection and boundary update, all resident in HBM.  The line printed:
               the streamed model 8*(2d+2 + 6K + 2d+1) B per cell-step) / average launch
               time measured with HIP events on the solver's stream; traffic = measured
               HBM bytes per launch from rocprofv3 counters when a profile is committed
-  cpu_baseline the CPU restatement of the reference (oracle/pf_oracle.c, OpenMP) on the
-              host cores for a bounded number of steps of the same workload
+  cpu_baseline the reference's own OpenMP build (oracle/_ref, compiled from its sources)
+              on the host cores for a bounded number of steps of the same workload, with
+              the CPU restatement (oracle/pf_oracle.c) beside it; the restatement alone
+              when the reference build is absent
+Further sections: mixed precision on the same workload and on the 512x256x256 boxes,
+the batched 2D workload, the served 3D rooms and the same rooms resident together.
 """
 from __future__ import annotations
 
