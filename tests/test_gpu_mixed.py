@@ -110,3 +110,22 @@ def test_mixed_boxes_512x256x256_full_size():
     assert all(e <= FIELD_TOL for f, e in errs.items() if f != "p"), errs
     assert errs["p"] <= P_FIXED_TOL, errs
     assert ddiv <= DIV_REL_TOL
+
+
+@pytest.mark.parametrize("inner", [10, 25, 50])
+def test_mixed_closer_to_fp64_than_reference_fp32_build(inner):
+    """The reference as shipped computes in default REAL (FP32).  On the 48x24 cylinder
+    fixture its deviation from the FP64 build is stored beside the FP64 fields (u 2e-7,
+    v 8e-7, p 2e-5 after 20 steps).  The mixed path, with FP64 everywhere but the SOR
+    correction sweeps, must stay closer to FP64 than that in every field."""
+    from conftest import load_golden
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, g = load_golden("g2d_cyl48")
+    with Solver(p, g["phi_raw"], precision=PF_PREC_MIXED, inner_iters=inner) as s:
+        s.step(p.istep_max)
+        for f in ("u", "v", "p"):
+            ref = g[f]
+            ours = float(np.linalg.norm(s.interior(f) - ref) / np.linalg.norm(ref))
+            shipped = float(np.linalg.norm(g[f + "_fp32"] - ref) / np.linalg.norm(ref))
+            print(f"[gpu] {f}: mixed {ours:.2e}, reference FP32 build {shipped:.2e}", flush=True)
+            assert ours < shipped, (f, ours, shipped)
