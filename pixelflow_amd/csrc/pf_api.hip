@@ -37,7 +37,8 @@ struct pf_ctx {
     // k_faces_check finds every face identical to IEEE division).
     int sor_variant = 2;
     int sor_forced = -1;
-    int xcd_remap = 1;
+    int xcd_remap = 1;              // XCD-contiguous block order for FP32 and residual kernels
+    int xcd_remap64 = 1;            // the same for FP64 SOR sweeps
     int sweep_kb = 1;               // 3D FP64 sweeps: cells per thread (1, 2 or 4)
     int sweep_kb32 = 1;             // 3D FP32 sweeps: cells per thread
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
@@ -205,8 +206,12 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
             const int v = std::atoi(kb);
             c->sweep_kb = c->sweep_kb32 = (v == 1 || v == 2 || v == 4) ? v : 1;
         }
+        // FP64 sweeps whose arrays (24 B per cell) are well past the last-level cache run
+        // 7-9 % faster in hardware block order (512x256x256: 183 against 199 us per
+        // sweep); at 384x192x192 the two orders tie and below that the remap wins.
+        c->xcd_remap64 = (cells * 24 > ((int64_t)512 << 20)) ? 0 : 1;
         const char *x = std::getenv("PF_XCD_REMAP");
-        if (x && *x) c->xcd_remap = std::atoi(x) != 0;
+        if (x && *x) c->xcd_remap = c->xcd_remap64 = std::atoi(x) != 0;
     }
     *out = c;
     return PF_OK;
@@ -384,7 +389,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
             s.fuse_z = fz;
             // 3D reduces |dp| after the second sweep only (:575-583); 2D after both (:351, :385)
             s.track_err = (g.dim == 2) || (half == 1);
-            s.xcd_remap = c->xcd_remap;
+            s.xcd_remap = c->xcd_remap64;
             s.gx = s.gy = 1;
             s.kb = c->sweep_kb;
             if (c->sor_variant == 0)
