@@ -62,28 +62,21 @@ enum NatOp {
     NAT_BOUNDARY_AIR = 5,   // aux = face (0 top, 1 bottom, 2 west, 3 east, 4 north, 5 south)
     NAT_UPDATE = 6,         // p += e (mixed)
     NAT_REFRESH_P = 7,      // periodic ghost refresh for odd extents
-    NAT_REFRESH_E = 8
+    NAT_REFRESH_E = 8,
+    NAT_PREDICT2 = 9        // NAT_PREDICT of a 2D batch (chosen by the launcher)
 };
 
 template <int OP>
-__device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, int aux) {
+__device__ __forceinline__ void nat_cell(const BatchCase &C, int i, int j, int k, int parity, int aux) {
     {
-        const int2 ti = v.nat_tile[t];
-        const int ci = ti.x, local = ti.y;
-        if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) return;
-        const BatchCase &C = v.cases[ci];
         const PfGeom &g = C.g;
-        if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) return;
-        const int tx = local % C.nat_gx, r = local / C.nat_gx;
-        const int ty = r % C.nat_gy, tz = r / C.nat_gy;
-        const int i = tx * 64 + (int)threadIdx.x, j = ty * 4 + (int)threadIdx.y, k = tz;
-        if (i > g.m + 1 || j > g.n + 1) return;
         const int p = parity, q = 1 - parity;
         if (OP == NAT_DIV) {
             div_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, i, j, k);
         } else if (OP == NAT_PREDICT) {
-            if (g.dim == 3) predict3_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, C.phi, C.U[p], C.V[p], C.W[p], i, j, k);
-            else predict2_cell(g, C.a, C.U[q], C.V[q], C.div, C.phi, C.U[p], C.V[p], i, j);
+            predict3_cell(g, C.a, C.U[q], C.V[q], C.W[q], C.div, C.phi, C.U[p], C.V[p], C.W[p], i, j, k);
+        } else if (OP == NAT_PREDICT2) {
+            predict2_cell(g, C.a, C.U[q], C.V[q], C.div, C.phi, C.U[p], C.V[p], i, j);
         } else if (OP == NAT_RHS || OP == NAT_PROJECT) {
             const bool interior = i >= 1 && i <= g.m && j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
             if (!interior) return;
@@ -120,6 +113,26 @@ __device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, 
             else gmem(C.E)[pf_soff(g, i, j, k)] = gmem(C.E)[pf_soff(g, i, sj, sk)];
         }
     }
+}
+
+// Natural-layout tiles are 64 x 4 cells of one plane over i = 1..m, j = 0..n+1; the threads
+// at i = 1 and i = m also take the ghost cells i = 0 and i = m + 1 of their row, so that a
+// row of 256 cells fills four waves instead of leaving a fifth one with two live lanes.
+template <int OP>
+__device__ __forceinline__ void nat_tile(const BatchView &v, int t, int parity, int aux) {
+    const int2 ti = v.nat_tile[t];
+    const int ci = ti.x, local = ti.y;
+    if ((OP == NAT_UPDATE || OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !v.active[ci]) return;
+    const BatchCase &C = v.cases[ci];
+    const PfGeom &g = C.g;
+    if ((OP == NAT_REFRESH_P || OP == NAT_REFRESH_E) && !C.ref_y && !C.ref_z) return;
+    const int tx = local % C.nat_gx, r = local / C.nat_gx;
+    const int ty = r % C.nat_gy, tz = r / C.nat_gy;
+    const int i = tx * 64 + (int)threadIdx.x + 1, j = ty * 4 + (int)threadIdx.y, k = tz;
+    if (i > g.m || j > g.n + 1) return;
+    if (i == 1) nat_cell<OP>(C, 0, j, k, parity, aux);
+    if (i == g.m) nat_cell<OP>(C, g.m + 1, j, k, parity, aux);
+    nat_cell<OP>(C, i, j, k, parity, aux);
 }
 
 template <int OP, bool PERSIST>
@@ -597,7 +610,7 @@ static int rebuild_tables(pf_batch *b) {
     for (int s : b->order) {
         BatchCase bc = b->slots[(size_t)s].bc;
         const PfGeom &g = bc.g;
-        bc.nat_gx = (g.m + 2 + 63) / 64;
+        bc.nat_gx = (g.m + 63) / 64;   // i = 1..m; the x ghosts ride with i = 1 and i = m
         bc.nat_gy = (g.n + 2 + 3) / 4;
         const int TX = b->tile_x, RY = 256 / TX;
         bc.swp_gx = ((g.m + 1) / 2 + TX - 1) / TX;
@@ -867,7 +880,10 @@ static void launch_nat(pf_batch *b, int op, int parity, int aux, const pf_batch:
     } while (0)
     switch (op) {
         case pf::NAT_DIV: PFB_NAT(pf::NAT_DIV); break;
-        case pf::NAT_PREDICT: PFB_NAT(pf::NAT_PREDICT); break;
+        case pf::NAT_PREDICT:
+            if (b->kind.dim == 3) PFB_NAT(pf::NAT_PREDICT);
+            else PFB_NAT(pf::NAT_PREDICT2);
+            break;
         case pf::NAT_RHS: PFB_NAT(pf::NAT_RHS); break;
         case pf::NAT_PROJECT: PFB_NAT(pf::NAT_PROJECT); break;
         case pf::NAT_BOUNDARY_UNIFORM: PFB_NAT(pf::NAT_BOUNDARY_UNIFORM); break;

@@ -51,147 +51,224 @@ __device__ __forceinline__ void div_cell(const PfGeom &g, const StepArgs &a, con
     div[pf_off(g, i, j, k)] = val;
 }
 
-__device__ __forceinline__ void predict3_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo_,
-                                              const double *__restrict__ vo_, const double *__restrict__ wo_,
-                                              const double *__restrict__ div_, const double *__restrict__ po_,
-                                              double *__restrict__ u_, double *__restrict__ v_, double *__restrict__ w_,
-                                              int i, int j, int k) {
+// Divisors of the momentum predictor: the grid spacings and the squared wall thicknesses
+// (thickness * d)^2 of the wall force, each prepared once per wave (DivK).
+struct PredK {
+    DivK x, y, z, tx, ty, tz;
+};
+__device__ __forceinline__ PredK pred_consts(const StepArgs &a, int dim) {
+    PredK K;
+    K.x = divk_make(a.dx);
+    K.y = divk_make(a.dy);
+    const double thx = a.thickness * a.dx;
+    K.tx = divk_make(thx * thx);
+    if (dim == 3) {
+        K.z = divk_make(a.dz);
+        const double thy = a.thickness * a.dy, thz = a.thickness * a.dz;
+        K.ty = divk_make(thy * thy);
+        K.tz = divk_make(thz * thz);
+    } else {
+        K.z = K.ty = K.tz = K.x;
+    }
+    return K;
+}
+
+// Operands of the predictor at one cell: porosity p, divergence d and the old velocity
+// components at the cell (c) and its neighbours (E/W in x, N/S in y, T/B in z).
+struct PredIn {
+    double pc, pE, pW, pN, pS, pT, pB;
+    double dc, dE, dW, dN, dS, dT, dB;
+    double uc, uE, uW, uN, uS, uT, uB;
+    double vc, vE, vW, vN, vS, vT, vB;
+    double wc, wE, wW, wN, wS, wT, wB;
+};
+
+// The 3D predictor (:233-376) with every division by a spacing or a squared thickness
+// taken by dv, in the reference's operand order.  A gradient half such as
+// ((uE - uW) / dx) * 0.5 appears twice in one expression and again in a neighbouring
+// component; it is evaluated once (gux), the same value each time.
+template <class DIV>
+__device__ __forceinline__ void predict3_values(const StepArgs &a, const PredIn &x, const PredK &K, DIV &dv,
+                                                double r3[3]) {
+    const double dt = a.dt, xnue = a.xnue, xlambda = a.xlambda;
+    const double wall = PF_ALPHA;
+    const double pc = x.pc;
+    const double gux = dv(x.uE - x.uW, K.x) * 0.5, guy = dv(x.uN - x.uS, K.y) * 0.5, guz = dv(x.uT - x.uB, K.z) * 0.5;
+    const double gvx = dv(x.vE - x.vW, K.x) * 0.5, gvy = dv(x.vN - x.vS, K.y) * 0.5, gvz = dv(x.vT - x.vB, K.z) * 0.5;
+    const double gwx = dv(x.wE - x.wW, K.x) * 0.5, gwy = dv(x.wN - x.wS, K.y) * 0.5, gwz = dv(x.wT - x.wB, K.z) * 0.5;
+    const double dpx = x.pE - x.pW, dpy = x.pN - x.pS, dpz = x.pT - x.pB;
+    {   // u (:233-271)
+        double r = x.uc - (dv((dt * x.uc) * (x.uE - x.uW), K.x) * 0.5);
+        r = r - (dv((dt * x.vc) * (x.uN - x.uS), K.y) * 0.5);
+        r = r - (dv((dt * x.wc) * (x.uT - x.uB), K.z) * 0.5);
+        r = r + dv(dv((dt * xnue) * ((x.uE - (2. * x.uc)) + x.uW), K.x), K.x);
+        r = r + dv(dv((dt * xnue) * ((x.uN - (2. * x.uc)) + x.uS), K.y), K.y);
+        r = r + dv(dv((dt * xnue) * ((x.uT - (2. * x.uc)) + x.uB), K.z), K.z);
+        r = r + (dv((dt * (xnue + xlambda)) * (x.dE - x.dW), K.x) * 0.5);
+        const double t1 = dv(((gux + gux) * xnue) * dpx, K.x) * 0.5;
+        const double t2 = dv(((guy + gvx) * xnue) * dpy, K.y) * 0.5;
+        const double t3 = dv(((guz + gwx) * xnue) * dpz, K.z) * 0.5;
+        const double t4 = (dv(x.dc * dpx, K.x) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) r = r - ((((dv((dt * xnue) * x.uc, K.tx) * wall) * pc) * (1. - pc)) * (1. - pc));
+        r3[0] = r;
+    }
+    {   // v (:286-323)
+        double r = x.vc - (dv((dt * x.uc) * (x.vE - x.vW), K.x) * 0.5);
+        r = r - (dv((dt * x.vc) * (x.vN - x.vS), K.y) * 0.5);
+        r = r - (dv((dt * x.wc) * (x.vT - x.vB), K.z) * 0.5);
+        r = r + dv(dv((dt * xnue) * ((x.vE - (2. * x.vc)) + x.vW), K.x), K.x);
+        r = r + dv(dv((dt * xnue) * ((x.vN - (2. * x.vc)) + x.vS), K.y), K.y);
+        r = r + dv(dv((dt * xnue) * ((x.vT - (2. * x.vc)) + x.vB), K.z), K.z);
+        r = r + (dv((dt * (xnue + xlambda)) * (x.dN - x.dS), K.y) * 0.5);
+        const double t1 = dv(((gvx + guy) * xnue) * dpx, K.x) * 0.5;
+        const double t2 = dv(((gvy + gvy) * xnue) * dpy, K.y) * 0.5;
+        const double t3 = dv(((gvz + gwy) * xnue) * dpz, K.z) * 0.5;
+        const double t4 = (dv(x.dc * dpy, K.y) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) r = r - ((((dv((dt * xnue) * x.vc, K.ty) * wall) * pc) * (1. - pc)) * (1. - pc));
+        r3[1] = r;
+    }
+    {   // w (:339-376)
+        double r = x.wc - (dv((dt * x.uc) * (x.wE - x.wW), K.x) * 0.5);
+        r = r - (dv((dt * x.vc) * (x.wN - x.wS), K.y) * 0.5);
+        r = r - (dv((dt * x.wc) * (x.wT - x.wB), K.z) * 0.5);
+        r = r + dv(dv((dt * xnue) * ((x.wE - (2. * x.wc)) + x.wW), K.x), K.x);
+        r = r + dv(dv((dt * xnue) * ((x.wN - (2. * x.wc)) + x.wS), K.y), K.y);
+        r = r + dv(dv((dt * xnue) * ((x.wT - (2. * x.wc)) + x.wB), K.z), K.z);
+        r = r + (dv((dt * (xnue + xlambda)) * (x.dT - x.dB), K.z) * 0.5);
+        const double t1 = dv(((gwx + guz) * xnue) * dpx, K.x) * 0.5;
+        const double t2 = dv(((gwy + gvz) * xnue) * dpy, K.y) * 0.5;
+        const double t3 = dv(((gwz + gwz) * xnue) * dpz, K.z) * 0.5;
+        const double t4 = (dv(x.dc * dpz, K.z) * 0.5) * xlambda;
+        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
+        if (a.nonslip) r = r - ((((dv((dt * xnue) * x.wc, K.tz) * wall) * pc) * (1. - pc)) * (1. - pc));
+        r3[2] = r;
+    }
+}
+
+// The 2D predictor (ibm_2d_uniform_omp_cpu.f90:204-255); the v wall force uses dx as well
+// (:254).
+template <class DIV>
+__device__ __forceinline__ void predict2_values(const StepArgs &a, const PredIn &x, const PredK &K, DIV &dv,
+                                                double r2[2]) {
+    const double dt = a.dt, xnue = a.xnue, xlambda = a.xlambda;
+    const double pc = x.pc;
+    const double gux = dv(x.uE - x.uW, K.x) * .5, guy = dv(x.uN - x.uS, K.y) * .5;
+    const double gvx = dv(x.vE - x.vW, K.x) * .5, gvy = dv(x.vN - x.vS, K.y) * .5;
+    const double dpx = x.pE - x.pW, dpy = x.pN - x.pS;
+    {   // u (:204-224)
+        double r = x.uc - (dt * (dv(x.uc * (x.uE - x.uW), K.x) / 2.));
+        r = r - (dt * (dv(x.vc * (x.uN - x.uS), K.y) / 2.));
+        r = r + dv(dv((dt * xnue) * ((x.uE - (2. * x.uc)) + x.uW), K.x), K.x);
+        r = r + dv(dv((dt * xnue) * ((x.uN - (2. * x.uc)) + x.uS), K.y), K.y);
+        r = r + (dv((dt * (xnue + xlambda)) * (x.dE - x.dW), K.x) * .5);
+        const double t1 = dv(((gux + gux) * xnue) * dpx, K.x) * .5;
+        const double t2 = dv(((guy + gvx) * xnue) * dpy, K.y) * .5;
+        const double t3 = (dv(x.dc * dpx, K.x) * 0.5) * xlambda;
+        r = r + ((dt * ((t1 + t2) + t3)) / pc);
+        if (a.nonslip) r = r - ((((dv((dt * xnue) * x.uc, K.tx) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
+        r2[0] = r;
+    }
+    {   // v (:235-255)
+        double r = x.vc - (dt * (dv(x.uc * (x.vE - x.vW), K.x) / 2.));
+        r = r - (dt * (dv(x.vc * (x.vN - x.vS), K.y) / 2.));
+        r = r + dv(dv((dt * xnue) * ((x.vE - (2. * x.vc)) + x.vW), K.x), K.x);
+        r = r + dv(dv((dt * xnue) * ((x.vN - (2. * x.vc)) + x.vS), K.y), K.y);
+        r = r + (dv((dt * (xnue + xlambda)) * (x.dN - x.dS), K.y) * .5);
+        const double t1 = dv(((gvx + guy) * xnue) * dpx, K.x) * .5;
+        const double t2 = dv(((gvy + gvy) * xnue) * dpy, K.y) * .5;
+        const double t3 = (dv(x.dc * dpy, K.y) * 0.5) * xlambda;
+        r = r + ((dt * ((t1 + t2) + t3)) / pc);
+        if (a.nonslip) r = r - ((((dv((dt * xnue) * x.vc, K.tx) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
+        r2[1] = r;
+    }
+}
+
+__device__ __forceinline__ void predict3_load(const PfGeom &g, const double *uo_, const double *vo_, const double *wo_,
+                                              const double *div_, const double *po_, int64_t o, PredIn &x) {
     const auto uo = gmem(uo_);
     const auto vo = gmem(vo_);
     const auto wo = gmem(wo_);
     const auto div = gmem(div_);
     const auto po = gmem(po_);
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    x.pc = po[o]; x.pE = po[o + 1]; x.pW = po[o - 1]; x.pN = po[o + sy]; x.pS = po[o - sy]; x.pT = po[o + sz]; x.pB = po[o - sz];
+    x.dc = div[o]; x.dE = div[o + 1]; x.dW = div[o - 1]; x.dN = div[o + sy]; x.dS = div[o - sy]; x.dT = div[o + sz]; x.dB = div[o - sz];
+    x.uc = uo[o]; x.uE = uo[o + 1]; x.uW = uo[o - 1]; x.uN = uo[o + sy]; x.uS = uo[o - sy]; x.uT = uo[o + sz]; x.uB = uo[o - sz];
+    x.vc = vo[o]; x.vE = vo[o + 1]; x.vW = vo[o - 1]; x.vN = vo[o + sy]; x.vS = vo[o - sy]; x.vT = vo[o + sz]; x.vB = vo[o - sz];
+    x.wc = wo[o]; x.wE = wo[o + 1]; x.wW = wo[o - 1]; x.wN = wo[o + sy]; x.wS = wo[o - sy]; x.wT = wo[o + sz]; x.wB = wo[o - sz];
+}
+
+__device__ __forceinline__ void predict3_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo_,
+                                              const double *__restrict__ vo_, const double *__restrict__ wo_,
+                                              const double *__restrict__ div_, const double *__restrict__ po_,
+                                              double *__restrict__ u_, double *__restrict__ v_, double *__restrict__ w_,
+                                              int i, int j, int k) {
     const auto u = gmem(u_);
     const auto v = gmem(v_);
     const auto w = gmem(w_);
     const int64_t o = pf_off(g, i, j, k);
     if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1 || k == 0 || k == g.l + 1) {
-        u[o] = uo[o]; v[o] = vo[o]; w[o] = wo[o];
+        u[o] = gmem(uo_)[o]; v[o] = gmem(vo_)[o]; w[o] = gmem(wo_)[o];
         return;
     }
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    const double dt = a.dt, dx = a.dx, dy = a.dy, dz = a.dz, xnue = a.xnue, xlambda = a.xlambda;
-    const double pc = po[o];
-    const double pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy], pT = po[o + sz], pB = po[o - sz];
-    const double dc = div[o];
-    const double uc = uo[o], vc = vo[o], wc = wo[o];
-    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy], uT = uo[o + sz], uB = uo[o - sz];
-    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy], vT = vo[o + sz], vB = vo[o - sz];
-    const double wE = wo[o + 1], wW = wo[o - 1], wN = wo[o + sy], wS = wo[o - sy], wT = wo[o + sz], wB = wo[o - sz];
-    const double wall = PF_ALPHA;
-    {   // u (:233-271)
-        double r = uc - ((((dt * uc) * (uE - uW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (uN - uS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (uT - uB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((uT - (2. * uc)) + uB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * 0.5);
-        const double t1 = (((((((uE - uW) / dx) * 0.5) + (((uE - uW) / dx) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((uN - uS) / dy) * 0.5) + (((vE - vW) / dx) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((uT - uB) / dz) * 0.5) + (((wE - wW) / dx) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * uc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        u[o] = r;
+    PredIn x;
+    predict3_load(g, uo_, vo_, wo_, div_, po_, o, x);
+    const PredK K = pred_consts(a, 3);
+    double r[3];
+    DivFast fast;
+    predict3_values(a, x, K, fast, r);
+    if (fast.slow) {
+        // the rare thread whose dividends left the range in which DivFast reuses the
+        // divisor's reciprocal: the cell again with true division, from operands loaded
+        // again (at an offset the compiler cannot fold, so the common path need not keep
+        // its operands live for this one)
+        predict3_load(g, uo_, vo_, wo_, div_, po_, o + opaque_zero(), x);
+        DivIeee ieee;
+        predict3_values(a, x, K, ieee, r);
     }
-    {   // v (:286-323)
-        double r = vc - ((((dt * uc) * (vE - vW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (vN - vS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (vT - vB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((vT - (2. * vc)) + vB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * 0.5);
-        const double t1 = (((((((vE - vW) / dx) * 0.5) + (((uN - uS) / dy) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((vT - vB) / dz) * .5) + (((wN - wS) / dy) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dy;
-            r = r - (((((((dt * xnue) * vc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        v[o] = r;
-    }
-    {   // w (:339-376)
-        double r = wc - ((((dt * uc) * (wE - wW)) / dx) * 0.5);
-        r = r - ((((dt * vc) * (wN - wS)) / dy) * 0.5);
-        r = r - ((((dt * wc) * (wT - wB)) / dz) * 0.5);
-        r = r + ((((dt * xnue) * ((wE - (2. * wc)) + wW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((wN - (2. * wc)) + wS)) / dy) / dy);
-        r = r + ((((dt * xnue) * ((wT - (2. * wc)) + wB)) / dz) / dz);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sz] - div[o - sz])) / dz) * 0.5);
-        const double t1 = (((((((wE - wW) / dx) * 0.5) + (((uT - uB) / dz) * 0.5)) * xnue) * (pE - pW)) / dx) * 0.5;
-        const double t2 = (((((((wN - wS) / dy) * 0.5) + (((vT - vB) / dz) * 0.5)) * xnue) * (pN - pS)) / dy) * 0.5;
-        const double t3 = (((((((wT - wB) / dz) * 0.5) + (((wT - wB) / dz) * 0.5)) * xnue) * (pT - pB)) / dz) * 0.5;
-        const double t4 = (((dc * (pT - pB)) / dz) * 0.5) * xlambda;
-        r = r + ((dt * (((t1 + t2) + t3) + t4)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dz;
-            r = r - (((((((dt * xnue) * wc) / (th * th)) * wall) * pc) * (1. - pc)) * (1. - pc));
-        }
-        w[o] = r;
-    }
+    u[o] = r[0]; v[o] = r[1]; w[o] = r[2];
+}
+
+__device__ __forceinline__ void predict2_load(const PfGeom &g, const double *uo_, const double *vo_,
+                                              const double *div_, const double *po_, int64_t o, PredIn &x) {
+    const auto uo = gmem(uo_);
+    const auto vo = gmem(vo_);
+    const auto div = gmem(div_);
+    const auto po = gmem(po_);
+    const int64_t sy = g.PX;
+    x.pc = po[o]; x.pE = po[o + 1]; x.pW = po[o - 1]; x.pN = po[o + sy]; x.pS = po[o - sy];
+    x.dc = div[o]; x.dE = div[o + 1]; x.dW = div[o - 1]; x.dN = div[o + sy]; x.dS = div[o - sy];
+    x.uc = uo[o]; x.uE = uo[o + 1]; x.uW = uo[o - 1]; x.uN = uo[o + sy]; x.uS = uo[o - sy];
+    x.vc = vo[o]; x.vE = vo[o + 1]; x.vW = vo[o - 1]; x.vN = vo[o + sy]; x.vS = vo[o - sy];
+    x.pT = x.pB = x.dT = x.dB = x.uT = x.uB = x.vT = x.vB = 0.;
+    x.wc = x.wE = x.wW = x.wN = x.wS = x.wT = x.wB = 0.;
 }
 
 __device__ __forceinline__ void predict2_cell(const PfGeom &g, const StepArgs &a, const double *__restrict__ uo_,
                                               const double *__restrict__ vo_, const double *__restrict__ div_,
                                               const double *__restrict__ po_, double *__restrict__ u_,
                                               double *__restrict__ v_, int i, int j) {
-    const auto uo = gmem(uo_);
-    const auto vo = gmem(vo_);
-    const auto div = gmem(div_);
-    const auto po = gmem(po_);
     const auto u = gmem(u_);
     const auto v = gmem(v_);
     const int64_t o = pf_off(g, i, j, 0);
     if (i == 0 || i == g.m + 1 || j == 0 || j == g.n + 1) {
-        u[o] = uo[o]; v[o] = vo[o];
+        u[o] = gmem(uo_)[o]; v[o] = gmem(vo_)[o];
         return;
     }
-    const int64_t sy = g.PX;
-    const double dt = a.dt, dx = a.dx, dy = a.dy, xnue = a.xnue, xlambda = a.xlambda;
-    const double pc = po[o], pE = po[o + 1], pW = po[o - 1], pN = po[o + sy], pS = po[o - sy];
-    const double dc = div[o];
-    const double uc = uo[o], vc = vo[o];
-    const double uE = uo[o + 1], uW = uo[o - 1], uN = uo[o + sy], uS = uo[o - sy];
-    const double vE = vo[o + 1], vW = vo[o - 1], vN = vo[o + sy], vS = vo[o - sy];
-    {   // u (ibm_2d_uniform_omp_cpu.f90:204-224)
-        double r = uc - (dt * (((uc * (uE - uW)) / dx) / 2.));
-        r = r - (dt * (((vc * (uN - uS)) / dy) / 2.));
-        r = r + ((((dt * xnue) * ((uE - (2. * uc)) + uW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((uN - (2. * uc)) + uS)) / dy) / dy);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + 1] - div[o - 1])) / dx) * .5);
-        const double t1 = (((((((uE - uW) / dx) * .5) + (((uE - uW) / dx) * .5)) * xnue) * (pE - pW)) / dx) * .5;
-        const double t2 = (((((((uN - uS) / dy) * .5) + (((vE - vW) / dx) * .5)) * xnue) * (pN - pS)) / dy) * .5;
-        const double t3 = (((dc * (pE - pW)) / dx) * 0.5) * xlambda;
-        r = r + ((dt * ((t1 + t2) + t3)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * uc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
-        }
-        u[o] = r;
+    PredIn x;
+    predict2_load(g, uo_, vo_, div_, po_, o, x);
+    const PredK K = pred_consts(a, 2);
+    double r[2];
+    DivFast fast;
+    predict2_values(a, x, K, fast, r);
+    if (fast.slow) {  // as in predict3_cell
+        predict2_load(g, uo_, vo_, div_, po_, o + opaque_zero(), x);
+        DivIeee ieee;
+        predict2_values(a, x, K, ieee, r);
     }
-    {   // v (:235-255); the wall force uses dx here as well (:254)
-        double r = vc - (dt * (((uc * (vE - vW)) / dx) / 2.));
-        r = r - (dt * (((vc * (vN - vS)) / dy) / 2.));
-        r = r + ((((dt * xnue) * ((vE - (2. * vc)) + vW)) / dx) / dx);
-        r = r + ((((dt * xnue) * ((vN - (2. * vc)) + vS)) / dy) / dy);
-        r = r + ((((dt * (xnue + xlambda)) * (div[o + sy] - div[o - sy])) / dy) * .5);
-        const double t1 = (((((((vE - vW) / dx) * .5) + (((uN - uS) / dy) * .5)) * xnue) * (pE - pW)) / dx) * .5;
-        const double t2 = (((((((vN - vS) / dy) * .5) + (((vN - vS) / dy) * .5)) * xnue) * (pN - pS)) / dy) * .5;
-        const double t3 = (((dc * (pN - pS)) / dy) * 0.5) * xlambda;
-        r = r + ((dt * ((t1 + t2) + t3)) / pc);
-        if (a.nonslip) {
-            const double th = a.thickness * dx;
-            r = r - (((((((dt * xnue) * vc) / (th * th)) * PF_ALPHA) * pc) * (1. - pc)) * (1. - pc));
-        }
-        v[o] = r;
-    }
+    u[o] = r[0]; v[o] = r[1];
 }
 
 __device__ __forceinline__ double bb3_at(const PfGeom &g, const StepArgs &a, const double *u_, const double *v_,

@@ -117,6 +117,62 @@ __device__ __forceinline__ double div_const(double a, double d, double rd) {
     const double e = __builtin_fma(-q0, d, a);
     return __builtin_fma(e, rd, q0);
 }
+// ------------------------------------------------------------------------------------
+// Division by a divisor shared by the whole wave (a grid spacing, the square of a wall
+// thickness), bit-identical to the compiler's IEEE division a / d.  gfx950 divides doubles
+// in eleven instructions: s = v_div_scale(d) (d itself unless the quotient's exponent is
+// near either end of the double range, then d * 2^+-128), the reciprocal r of s from
+// v_rcp_f64 and two Newton steps (four FMAs), n = v_div_scale(a), q = n*r,
+// e = fma(-s, q, n), v_div_fmas(e, r, q) and v_div_fixup(.., d, a).  DivK holds s and r of
+// the divisor as scaled for a unit dividend, computed by that same instruction sequence;
+// DivFast recomputes s per dividend and, where it agrees, reuses r: the same instructions
+// on the same operands, seven of them per quotient with the check.  A dividend whose s
+// differs sets `slow`, and the caller evaluates the cell again with DivIeee.  The check is
+// an ordered compare: v_div_scale gives NaN for a zero dividend, and there v_div_fixup
+// returns the signed zero whatever its quotient operand holds; a NaN or infinite dividend
+// likewise leaves v_div_fixup's result independent of the quotient operand.
+// ------------------------------------------------------------------------------------
+struct DivK {
+    double d, s, r;
+};
+// A double that every active lane holds, moved to scalar registers (VALU instructions read
+// one scalar register pair as an operand, which frees six vector registers per DivK).
+__device__ __forceinline__ double wave_uniform(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ DivK divk_make(double d) {
+    bool f;
+    const double s = __builtin_amdgcn_div_scale(1.0, d, false, &f);
+    const double r0 = __builtin_amdgcn_rcp(s);
+    const double r1 = __builtin_fma(r0, __builtin_fma(-s, r0, 1.0), r0);
+    const double r = __builtin_fma(r1, __builtin_fma(-s, r1, 1.0), r1);
+    return DivK{wave_uniform(d), wave_uniform(s), wave_uniform(r)};
+}
+struct DivFast {
+    bool slow = false;
+    __device__ __forceinline__ double operator()(double a, const DivK &k) {
+        bool f;
+        const double s = __builtin_amdgcn_div_scale(a, k.d, false, &f);
+        slow = slow || __builtin_islessgreater(s, k.s);
+        const double n = __builtin_amdgcn_div_scale(a, k.d, true, &f);
+        const double q = n * k.r;
+        const double e = __builtin_fma(-k.s, q, n);
+        return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(e, k.r, q, f), k.d, a);
+    }
+};
+// Zero, produced where the compiler cannot see its value.
+__device__ __forceinline__ int opaque_zero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+struct DivIeee {
+    __device__ __forceinline__ double operator()(double a, const DivK &k) const { return a / k.d; }
+};
+
 __device__ __forceinline__ double face_coef_fast(double pa, double pb, double dt, double d, double rd) {
     return div_const(div_const(dt * fmax_ref(PF_SMALL, (pa + pb) * 0.5), d, rd), d, rd);
 }

@@ -85,6 +85,52 @@ def test_gpu_matches_oracle(name):
                 np.testing.assert_array_equal(full[1:-1, :], ref[1:-1, :], err_msg=f"{name}:{f} x-ghosts")
 
 
+
+def _extreme_velocities(p, rng):
+    """Velocity fields whose magnitudes span 1e-305 .. 1e120 (random signs, zeros of both
+    signs), with a band of ordinary values: the predictor's dividends then fall below and
+    above the range where v_div_scale leaves the divisor unscaled, next to ordinary ones in
+    the same waves, without any intermediate of the step overflowing."""
+    shape = p.full_shape()
+    out = {}
+    for f in (("u", "v", "w") if p.dim == 3 else ("u", "v")):
+        a = rng.choice([-1.0, 1.0], size=shape) * 10.0 ** rng.uniform(-305.0, 120.0, size=shape)
+        r = rng.random(shape)
+        a[r < 0.1] = 0.0
+        a[(r >= 0.1) & (r < 0.15)] = -0.0
+        band = a[..., : shape[-1] // 3]
+        band[...] = rng.uniform(-2.0, 2.0, size=band.shape)
+        out[f] = a
+    return out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gpu_predictor_extreme_dividends_match_oracle(dim):
+    """The predictor divides by the grid spacings and squared wall thicknesses with the
+    divisor's reciprocal prepared once per wave, and falls back to true division for a
+    cell whose dividends make v_div_scale rescale the divisor.  One step from velocities
+    that reach both ends of the double range must equal the oracle bit for bit."""
+    import pyoracle
+    if dim == 3:
+        p, phi = cases.sphere_case(m=40, n=24, l=20, steps=1, iter_max=4)
+    else:
+        p, phi = cases.cylinder_case(m=96, n=48, steps=1, iter_max=4)
+    print(f"[gpu] predictor extreme dividends dim={dim}", flush=True)
+    vel = _extreme_velocities(p, np.random.default_rng(11 + dim))
+    o = pyoracle.OracleRun(p, phi)
+    for f, a in vel.items():
+        {"u": o.u, "v": o.v, "w": o.w}[f][...] = a
+    e_ref = o.step(1)
+    with _solver(p, phi) as s:
+        for f, a in vel.items():
+            s.set_field(f, a)
+        st = s.step(1)
+        np.testing.assert_array_equal(st["sor_error"], e_ref)
+        for f in _fields(p):
+            a, b = s.field(f), {"u": o.u, "v": o.v, "w": o.w, "p": o.p_}[f]
+            bad = np.count_nonzero(a.view(np.uint64) != b.view(np.uint64))
+            assert bad == 0, f"{f}: {bad} cells differ in their bits"
+
 def test_gpu_full_size_sphere_two_steps():
     """256x128x128 sphere, 2 steps of 100 SOR iterations: rel-L2 <= 1e-12 per field."""
     import pyoracle
