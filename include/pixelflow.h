@@ -217,7 +217,8 @@ int pf_serve_result(pf_server *s, int64_t id, int field, double *host, int64_t l
 int pf_serve_release(pf_server *s, int64_t id);
 /* Steps run, seconds spent inside batch steps, requests queued and running. */
 int pf_serve_counters(pf_server *s, int64_t *steps_run, double *busy_s, int *queued, int *running);
-/* Why a request failed (valid until pf_serve_release). */
+/* Why a request failed; for an unknown id, the server's last fatal error.  The text is a
+ * copy owned by the calling thread, valid until that thread's next pf_serve_error call. */
 const char *pf_serve_error(pf_server *s, int64_t id);
 
 #ifdef __cplusplus

@@ -332,9 +332,12 @@ extern "C" int pf_serve_counters(pf_server *s, int64_t *steps_run, double *busy_
 }
 
 extern "C" const char *pf_serve_error(pf_server *s, int64_t id) {
+    // a copy per calling thread: the request may be released, and the worker may set the
+    // server's fatal message, while the caller still reads the text
+    thread_local std::string text;
     if (!s) return "";
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->all.find(id);
-    if (it == s->all.end()) return s->fatal.c_str();
-    return it->second->error.c_str();
+    text = (it == s->all.end()) ? s->fatal : it->second->error;
+    return text.c_str();
 }
