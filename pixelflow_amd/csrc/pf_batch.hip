@@ -36,11 +36,7 @@ namespace pf {
 
 // Block max of v into slots[t]; all threads of the block call it once per tile.
 __device__ __forceinline__ void tile_max_to_slot(double v, unsigned long long *slots, int t) {
-    unsigned long long e = __double_as_longlong(v);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o2 = __shfl_xor(e, off, 64);
-        e = (o2 > e) ? o2 : e;
-    }
+    const unsigned long long e = wave_umax(__double_as_longlong(v));
     __shared__ unsigned long long red[4];
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
     if ((lane & 63) == 0) red[lane >> 6] = e;
@@ -48,7 +44,7 @@ __device__ __forceinline__ void tile_max_to_slot(double v, unsigned long long *s
     if (lane == 0) {
         unsigned long long b = red[0];
         for (int q = 1; q < 4; ++q) b = (red[q] > b) ? red[q] : b;
-        if (b > slots[t]) slots[t] = b;
+        atomicMax(slots + t, b);   // result unread: the block need not wait for the slot
     }
     __syncthreads();   // red[] is reused by the next tile
 }

@@ -175,11 +175,7 @@ __global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, 
     }
     if (!s.track_err) return;
     // |dp| >= 0, so its IEEE bit pattern orders like an unsigned integer
-    unsigned long long e = __double_as_longlong(err);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o2 = __shfl_xor(e, off, 64);
-        e = (o2 > e) ? o2 : e;
-    }
+    const unsigned long long e = wave_umax(__double_as_longlong(err));
     __shared__ unsigned long long red[4];
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
     if ((lane & 63) == 0) red[lane >> 6] = e;
@@ -191,7 +187,7 @@ __global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, 
         // one slot per block: no two blocks of a launch touch the same address
         const int64_t bid = (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
         unsigned long long *slot = err_slot + bid;
-        if (b > *slot) *slot = b;
+        atomicMax(slot, b);   // result unread: the block need not wait for the slot
     }
 }
 
@@ -251,7 +247,7 @@ __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs
             err = (e > err) ? e : err;
         }
     }
-    if (s.track_err) block_max_to_slot(err, err_slot);
+    if (s.track_err) wave_max_to_slot(err, err_slot);
 }
 
 // Porosity (natural layout, ghosts included) -> colour-split copy.
@@ -646,8 +642,9 @@ void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const doub
 }
 
 int sor_blocks(const PfGeom &g) {
-    // slots for every kernel that reduces into them: stored-face sweep, KB = 1, 2, 4 sweeps,
-    // the one-cell-per-thread residual and the fused sweep
+    // slots for every kernel that reduces into them: one per wave (four per 256-thread
+    // block) of the stored-face sweep, the KB = 1, 2, 4 sweeps and the one-cell-per-thread
+    // residual, one per block of the fused sweep
     dim3 b(64, 4, 1);
     dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
     int n = (int)(gr.x * gr.y * gr.z);
@@ -655,6 +652,7 @@ int sor_blocks(const PfGeom &g) {
         const int v = sweep_shape(g, kb).blocks();
         n = v > n ? v : n;
     }
+    n *= 4;
     const FusedGeom f = fused_geom(g, 1);
     const int nf = f.gx * f.gy * f.gz;
     return nf > n ? nf : n;

@@ -19,17 +19,11 @@
 
 namespace pf {
 
-// FP32 face coefficient: dt/d^2 * max(small, (pa+pb)/2) (the FP64 path's expression,
-// regrouped; exact order does not matter for a correction that the FP64 residual checks).
-
-
-
 // ------------------------------------------------------------------------------------
-// FP64 residual of the cells of one colour (launched for both colours) -> FP32 r; clears
-// e at the cell and at the periodic ghost slot the cell feeds; block max of |r / ap|.
+// FP64 residual of the cells of one colour (launched for both colours): r = bb - A p
+// stored as FP32 R, e cleared at the cell and at the periodic ghost slot it feeds, and the
+// wave max of |r / ap| into the wave's slot.
 // ------------------------------------------------------------------------------------
-// One cell: FP64 residual r = bb - A p -> R (FP32), e cleared at the cell and at the
-// periodic ghost slot it feeds; returns |r / ap|.
 
 template <int DIM, bool FAST>
 __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ P,
@@ -39,7 +33,7 @@ __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArg
     const CellIdx q = sweep_cell<DIM>(g, s);
     double res = 0.;
     if (q.active) res = residual_cell<DIM, FAST>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E);
-    if (s.track_err) block_max_to_slot(res, slots);
+    if (s.track_err) wave_max_to_slot(res, slots);
 }
 
 // ------------------------------------------------------------------------------------
@@ -76,7 +70,7 @@ __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s
             err = (e > err) ? e : err;
         }
     }
-    if (s.track_err) block_max_to_slot(err, err_slot);
+    if (s.track_err) wave_max_to_slot(err, err_slot);
 }
 
 // p += e over every slot of both colours: interior cells and the periodic ghost slots
