@@ -232,32 +232,36 @@ __device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const Sweep
 }
 
 // Max of 64-bit patterns over the wave, returned in every lane.  Cross-lane moves are DPP
-// (data-parallel primitives: each VALU instruction reads a neighbour lane's register), so
-// the six steps involve no LDS round trip; the 64-bit shuffles they replace cost six
-// dependent ds_bpermute pairs at the end of every wave, which lengthened a colour sweep
-// that reduces by 1.6 us.  Steps: swap pairs and quads (quad_perm), mirror half-rows and
-// rows (max over 16 lanes), then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows
-// 2 and 3, which leaves the wave's max in lane 63.  Every lane of the wave must be active.
-// Sweeps reduce |dp| as the bit patterns of non-negative doubles, which order the same; a
-// NaN (0x7ff8...) exceeds them all and so reaches the host.
+// (data-parallel primitives: a VALU instruction reads a neighbour lane's register), so no
+// step involves an LDS round trip; 64-bit shuffles cost six dependent ds_bpermute pairs at
+// the end of every wave, which lengthened a colour sweep that reduces by 1.6 us.  The max
+// is taken on the high words first, then on the low words of the lanes that hold the
+// winning high word: that order is the unsigned 64-bit order.  Each 32-bit max takes six
+// steps -- swap pairs and quads (quad_perm), mirror half-rows and rows (max over 16 lanes),
+// row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3 -- which leave it in lane
+// 63.  Every lane of the wave must be active.  Sweeps reduce |dp| as the bit patterns of
+// non-negative doubles, which order the same; a NaN (0x7ff8...) exceeds them all and so
+// reaches the host.
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ unsigned long long dpp_umax_step(unsigned long long e) {
-    const int lo = (int)(unsigned)e, hi = (int)(unsigned)(e >> 32);
-    const unsigned lo2 = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
-    const unsigned hi2 = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
-    const unsigned long long o = ((unsigned long long)hi2 << 32) | lo2;
-    return (o > e) ? o : e;
+__device__ __forceinline__ unsigned dpp_umax32_step(unsigned x) {
+    // old value 0, the identity of the max: lanes a masked row_bcast leaves unwritten keep x
+    const unsigned y = (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, false);
+    return (y > x) ? y : x;
+}
+__device__ __forceinline__ unsigned wave_umax32(unsigned x) {
+    x = dpp_umax32_step<0xB1, 0xF>(x);    // quad_perm [1,0,3,2]
+    x = dpp_umax32_step<0x4E, 0xF>(x);    // quad_perm [2,3,0,1]
+    x = dpp_umax32_step<0x141, 0xF>(x);   // row_half_mirror
+    x = dpp_umax32_step<0x140, 0xF>(x);   // row_mirror
+    x = dpp_umax32_step<0x142, 0xA>(x);   // row_bcast:15
+    x = dpp_umax32_step<0x143, 0xC>(x);   // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 __device__ __forceinline__ unsigned long long wave_umax(unsigned long long e) {
-    e = dpp_umax_step<0xB1, 0xF>(e);    // quad_perm [1,0,3,2]
-    e = dpp_umax_step<0x4E, 0xF>(e);    // quad_perm [2,3,0,1]
-    e = dpp_umax_step<0x141, 0xF>(e);   // row_half_mirror
-    e = dpp_umax_step<0x140, 0xF>(e);   // row_mirror
-    e = dpp_umax_step<0x142, 0xA>(e);   // row_bcast:15
-    e = dpp_umax_step<0x143, 0xC>(e);   // row_bcast:31
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)e, 63);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(e >> 32), 63);
-    return ((unsigned long long)hi << 32) | lo;
+    const unsigned hi = (unsigned)(e >> 32);
+    const unsigned mh = wave_umax32(hi);
+    const unsigned ml = wave_umax32(hi == mh ? (unsigned)e : 0u);
+    return ((unsigned long long)mh << 32) | ml;
 }
 
 // Max of a non-negative double over the wave, folded by lane 0 into the wave's own slot,
