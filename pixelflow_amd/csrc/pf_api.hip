@@ -41,6 +41,7 @@ struct pf_ctx {
     int xcd_remap64 = 1;            // the same for FP64 SOR sweeps
     int sweep_kb = 1;               // 3D FP64 sweeps: cells per thread (1, 2 or 4)
     int sweep_kb32 = 1;             // 3D FP32 sweeps: cells per thread
+    int sweep_xw = 64;              // colour slots per wave row in the sweeps (64 or 32)
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
     unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
     unsigned long long *res_part = nullptr;  // mixed: per-block max |r/ap|
@@ -210,6 +211,9 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         // 7-9 % faster in hardware block order (512x256x256: 183 against 199 us per
         // sweep); at 384x192x192 the two orders tie and below that the remap wins.
         c->xcd_remap64 = (cells * 24 > ((int64_t)512 << 20)) ? 0 : 1;
+        c->sweep_xw = pf::sweep_wave_width(g);
+        const char *xw = std::getenv("PF_SWEEP_XW");
+        if (xw && *xw) c->sweep_xw = (std::atoi(xw) == 32) ? 32 : 64;
         const char *x = std::getenv("PF_XCD_REMAP");
         if (x && *x) c->xcd_remap = c->xcd_remap64 = std::atoi(x) != 0;
     }
@@ -392,6 +396,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
             s.xcd_remap = c->xcd_remap64;
             s.gx = s.gy = 1;
             s.kb = c->sweep_kb;
+            s.xw = c->sweep_xw;
             if (c->sor_variant == 0)
                 pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
             else
@@ -466,6 +471,7 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
                 s.track_err = (g.dim == 2) || (half == 1);
                 s.xcd_remap = c->xcd_remap;
                 s.kb = c->sweep_kb32;
+                s.xw = c->sweep_xw;
                 pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot);
                 if (ry || rz) pf::launch_ghost_refresh32(c->st, g, ry, rz, c->E32);
             }

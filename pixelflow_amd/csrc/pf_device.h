@@ -184,24 +184,36 @@ __device__ __forceinline__ double face_any(double pa, double pb, double dt, doub
 
 // ------------------------------------------------------------------------------------
 // Thread geometry of the colour sweeps.  Block (64, 4): x runs over the colour's slots of
-// a row.  3D: y = (row offset 0..1, plane parity 0..1); a block covers 2 rows and 2*KB
-// planes and each thread updates KB cells of its colour two planes apart (same slot, same
-// x position; independent updates whose loads overlap in flight).  2D: y = 4 rows, KB = 1.
-// Blocks form a 1D grid renumbered XCD-contiguously (xcd_logical_block).
+// a row (or, with 32-slot waves, over 32 slots of two rows).  3D: y = (row offset 0..1,
+// plane parity 0..1); a block covers 2 (or 4) rows and 2*KB planes and each thread updates
+// KB cells of its colour two planes apart (same slot, same x position; independent updates
+// whose loads overlap in flight).  2D: y = 4 row groups, KB = 1.  Blocks form a 1D grid
+// renumbered XCD-contiguously (xcd_logical_block).
 // ------------------------------------------------------------------------------------
 struct SweepShape {
     int gx, gy, gz;
     PF_HD int blocks() const { return gx * gy * gz; }
 };
 
-PF_HD SweepShape sweep_shape(const PfGeom &g, int KB) {
+// Wave width of the colour sweeps: 64 colour slots of one row, or 32 slots of two rows
+// when that leaves fewer lanes idle (rows of 96 slots fill 3/4 of two 64-lane waves but
+// all of three 32-slot halves).  Rows of 128 slots keep 64.
+PF_HD int sweep_xw(const PfGeom &g) {
+    const int S = (g.m + 1) / 2;
+    const int w64 = (S + 63) / 64 * 64, w32 = (S + 31) / 32 * 32;
+    return (w32 < w64) ? 32 : 64;
+}
+
+PF_HD SweepShape sweep_shape(const PfGeom &g, int KB, int xw) {
     SweepShape t;
-    t.gx = ((g.m + 1) / 2 + 63) / 64;
+    xw = (xw == 32) ? 32 : 64;
+    const int rpw = 64 / xw;   // rows per wave
+    t.gx = ((g.m + 1) / 2 + xw - 1) / xw;
     if (g.dim == 3) {
-        t.gy = (g.n + 1) / 2;
+        t.gy = (g.n + 2 * rpw - 1) / (2 * rpw);
         t.gz = (g.l + 2 * KB - 1) / (2 * KB);
     } else {
-        t.gy = (g.n + 3) / 4;
+        t.gy = (g.n + 4 * rpw - 1) / (4 * rpw);
         t.gz = 1;
     }
     return t;
@@ -211,22 +223,27 @@ struct SweepThread {
     int j, kbase, sh, h, i;
 };
 
+// Block (64, 4): threadIdx.x runs over xw slots of rpw = 64 / xw rows; 3D: threadIdx.y =
+// (row pair offset, plane parity), 2D: threadIdx.y = row group.
 template <int DIM, int KB>
 __device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const SweepArgs &s) {
     const int nb = (int)gridDim.x;
     const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
     const int bx = L % s.gx, byz = L / s.gx;
     const int by = byz % s.gy, bz = byz / s.gy;
+    const int rpw = (s.xw == 32) ? 2 : 1;
+    const int sub = (s.xw == 32) ? (int)(threadIdx.x >> 5) : 0;
+    const int xl = (s.xw == 32) ? (int)(threadIdx.x & 31) : (int)threadIdx.x;
     SweepThread t;
     if (DIM == 3) {
-        t.j = by * 2 + (int)(threadIdx.y >> 1) + 1;
+        t.j = by * 2 * rpw + (int)(threadIdx.y >> 1) * rpw + sub + 1;
         t.kbase = bz * 2 * KB + 1 + (int)(threadIdx.y & 1);
     } else {
-        t.j = by * 4 + (int)threadIdx.y + 1;
+        t.j = by * 4 * rpw + (int)threadIdx.y * rpw + sub + 1;
         t.kbase = 0;
     }
     t.sh = (s.colour + t.j + t.kbase) & 1;   // the same for kbase + 2q
-    t.h = (int)(bx * 64 + threadIdx.x) + 1 - t.sh;
+    t.h = bx * s.xw + xl + 1 - t.sh;
     t.i = 2 * t.h + t.sh;
     return t;
 }

@@ -33,6 +33,7 @@ struct SweepArgs {
     int xcd_remap;      // renumber blocks XCD-contiguously
     int gx, gy;         // block grid extents of the sweep (set by the launcher)
     int kb;             // 3D: cells per thread, two planes apart (1, 2 or 4)
+    int xw;             // colour slots of one row a wave covers: 64 (one row) or 32 (two rows)
 };
 
 void launch_faces(hipStream_t st, const PfGeom &g, const double *phi, double *AW, double *AS, double *AB,
@@ -62,6 +63,7 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
 int sor_blocks(const PfGeom &g);
+int sweep_wave_width(const PfGeom &g);   // 64 or 32 colour slots per wave row
 // One red/black iteration in one launch, ping-pong A -> B (pf_fused.h); usable when
 // sor_fused_ok (even periodic extents).  B must hold A's non-periodic ghost values.
 bool sor_fused_ok(const PfGeom &g, int bc);

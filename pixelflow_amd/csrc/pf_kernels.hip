@@ -549,8 +549,9 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
     const int c = s.colour;
     const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
     const int KB = (g.dim == 3) ? s.kb : 1;
-    const SweepShape sp = sweep_shape(g, KB);
     SweepArgs s1 = s;
+    s1.xw = (s.xw == 32) ? 32 : 64;   // an unset width means one row per wave
+    const SweepShape sp = sweep_shape(g, KB, s1.xw);
     s1.gx = sp.gx;
     s1.gy = sp.gy;
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
@@ -641,6 +642,8 @@ void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const doub
     hipLaunchKernelGGL(k_force_final, dim3(1), dim3(256), 0, st, partial, nb, out);
 }
 
+int sweep_wave_width(const PfGeom &g) { return sweep_xw(g); }
+
 int sor_blocks(const PfGeom &g) {
     // slots for every kernel that reduces into them: one per wave (four per 256-thread
     // block) of the stored-face sweep, the KB = 1, 2, 4 sweeps and the one-cell-per-thread
@@ -648,10 +651,11 @@ int sor_blocks(const PfGeom &g) {
     dim3 b(64, 4, 1);
     dim3 gr = grid2((g.m + 1) / 2, g.n, g.dim == 3 ? g.l : 1, b);
     int n = (int)(gr.x * gr.y * gr.z);
-    for (int kb : {1, 2, 4}) {
-        const int v = sweep_shape(g, kb).blocks();
-        n = v > n ? v : n;
-    }
+    for (int kb : {1, 2, 4})
+        for (int xw : {32, 64}) {
+            const int v = sweep_shape(g, kb, xw).blocks();
+            n = v > n ? v : n;
+        }
     n *= 4;
     const FusedGeom f = fused_geom(g, 1);
     const int nf = f.gx * f.gy * f.gz;

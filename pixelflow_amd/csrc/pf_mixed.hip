@@ -127,7 +127,8 @@ void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepAr
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
                   const float *R, const double *phi, unsigned long long *err_slot) {
     const int KB = (g.dim == 3) ? s.kb : 1;
-    const SweepShape sp = sweep_shape(g, KB);
+    s.xw = (s.xw == 32) ? 32 : 64;   // an unset width means one row per wave
+    const SweepShape sp = sweep_shape(g, KB, s.xw);
     s.gx = sp.gx;
     s.gy = sp.gy;
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());

@@ -317,3 +317,35 @@ def test_gpu_mixed_cells_per_thread_identical(name, monkeypatch):
         for f in _fields(p):
             np.testing.assert_array_equal(out[kb][0][f], out[1][0][f], err_msg=f"{name}:{f}:kb{kb}")
         np.testing.assert_array_equal(out[kb][1], out[1][1])
+
+
+@pytest.mark.parametrize("name", ["g3d_sph32", "g3d_odd_33_17_15", "g3d_air_room24", "g3d_boxes", "g2d_cyl48",
+                                  "g2d_weird_odd"])
+@pytest.mark.parametrize("xw", [32, 64])
+def test_gpu_sweep_wave_width_matches_reference(name, xw, monkeypatch):
+    """Colour sweeps whose waves cover 64 slots of one row or 32 slots of two rows (the
+    library picks the one that leaves fewer lanes idle) reproduce the reference."""
+    monkeypatch.setenv("PF_SWEEP_XW", str(xw))
+    p, g = load_golden(name)
+    print(f"[gpu] wave width {xw} fixture {name}", flush=True)
+    with _solver(p, g["phi_raw"]) as s:
+        st = s.step(p.istep_max)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), g[f], err_msg=f"{name}:{f}:xw{xw}")
+        np.testing.assert_array_equal(st["sor_error"], g["errs"])
+
+
+@pytest.mark.parametrize("name", ["3d_sphere_64x32x32", "3d_odd_45x27x19", "2d_odd_97x51"])
+def test_gpu_mixed_wave_width_identical(name, monkeypatch):
+    """FP32 sweeps give the same bits with either wave width."""
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    p, phi = CASES[name]()
+    out = {}
+    for xw in (32, 64):
+        monkeypatch.setenv("PF_SWEEP_XW", str(xw))
+        with Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=10) as s:
+            st = s.step(p.istep_max)
+            out[xw] = ({f: s.interior(f) for f in _fields(p)}, st["sor_error"])
+    for f in _fields(p):
+        np.testing.assert_array_equal(out[32][0][f], out[64][0][f], err_msg=f"{name}:{f}")
+    np.testing.assert_array_equal(out[32][1], out[64][1])
