@@ -43,8 +43,8 @@ struct pf_ctx {
     int sweep_kb32 = 1;             // 3D FP32 sweeps: cells per thread
     int sweep_xw = 64;              // colour slots per wave row in the sweeps (64 or 32)
     unsigned long long *err = nullptr;       // one entry per step of the last pf_step
-    unsigned long long *err_part = nullptr;  // per-block partial maxima of the current step
-    unsigned long long *res_part = nullptr;  // mixed: per-block max |r/ap|
+    unsigned long long *err_part = nullptr;  // per-wave partial maxima of the current step
+    unsigned long long *res_part = nullptr;  // mixed: per-wave max |r/ap|
     unsigned long long *res_val = nullptr;   // mixed: reduced residual of the last evaluation
     float *E32 = nullptr, *R32 = nullptr, *PHI32 = nullptr;   // mixed: colour-split FP32 arrays
     int err_cap = 0, nparts = 0;

@@ -113,7 +113,8 @@ __global__ void k_rhs(PfGeom g, StepArgs a, const double *__restrict__ u, const 
 // this sweep reads it, so the refresh of the reference (:463-480) is fused here as a
 // second store; odd extents use k_ghost_refresh after the sweep instead.
 // The SOR error diagnostic is the max |p - p_old| over the updated cells, reduced per
-// wave with cross-lane shuffles, per block through LDS, then one atomic max per block.
+// wave with DPP cross-lane steps (wave_umax), per block through LDS, then one atomic max
+// per block (this stored-face kernel; the porosity sweeps keep one slot per wave).
 // ------------------------------------------------------------------------------------
 template <int DIM>
 __global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, 
     }
 }
 
-// Max of the per-block slots of one step into that step's entry; the slots are cleared
+// Max of the partial slots of one step into that step's entry; the slots are cleared
 // for the next step.  One block.
 __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned long long *step_err) {
     unsigned long long e = 0ull;
