@@ -18,7 +18,8 @@ sweeps), projection and boundary update, all resident in HBM.  The line printed:
   cpu_baseline the reference's own OpenMP build (oracle/_ref, compiled from its sources)
               on the host cores for a bounded number of steps of the same workload, with
               the CPU restatement (oracle/pf_oracle.c) beside it; the restatement alone
-              when the reference build is absent
+              when the reference build is absent; and `single_thread`, the restatement
+              on one thread for one step
 Further sections: mixed precision on the same workload and on the 512x256x256 boxes,
 the batched 2D workload, the served 3D rooms and the same rooms resident together.
 """
@@ -125,6 +126,28 @@ def cpu_baseline(p, phi, steps: int, threads: int):
     return p.cells * steps / dt, dt
 
 
+def cpu_baseline_one_thread(m: int, n: int, l: int, steps: int):
+    """The oracle on one thread, timed in a child process (the OpenMP runtime of this one
+    has already fixed its thread count): `steps` steps of the sphere workload of the given
+    size, without a warm-up step.  Returns (cell-steps/s, seconds) or None."""
+    import subprocess
+    code = ("import sys, json, time; sys.path.insert(0, %r); sys.path.insert(0, %r);"
+            "from pixelflow_amd import cases; import pyoracle;"
+            "p, phi = cases.sphere_case(m=%d, n=%d, l=%d);"
+            "o = pyoracle.OracleRun(p, phi); t0 = time.perf_counter(); o.step(%d);"
+            "dt = time.perf_counter() - t0; print(json.dumps([p.cells * %d / dt, dt]))"
+            % (ROOT, os.path.join(ROOT, "oracle"), m, n, l, steps, steps))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    try:
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return None
+    if out.returncode != 0:
+        return None
+    v = json.loads(out.stdout.strip().splitlines()[-1])
+    return v[0], v[1]
+
+
 def mixed_comparison(p, phi, steps: int, warmup: int, inner: int, fp64_value: float):
     """Same workload in mixed precision (fixed-count mode: the reference's iter_max), timed,
     then the fields after `steps` steps compared with an FP64 run of the same steps."""
@@ -226,6 +249,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=1,
+                    help="steps of the C restatement timed on one thread (0 = skip)")
     ap.add_argument("--cpu-ref-steps", type=int, default=5,
                     help="steps timed on the reference's own OpenMP build, when built (0 = skip)")
     ap.add_argument("--m", type=int, default=256)
@@ -364,6 +389,14 @@ def main(argv=None):
                                    "port": port}
         else:
             res["cpu_baseline"] = port
+        if args.cpu_steps_1t > 0:
+            log(f"[bench] CPU baseline: oracle, {args.cpu_steps_1t} step(s) on one thread")
+            one = cpu_baseline_one_thread(args.m, args.n, args.l, args.cpu_steps_1t)
+            if one is not None:
+                res["cpu_baseline"]["single_thread"] = {
+                    "value": one[0], "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                    "sample": f"{args.cpu_steps_1t} step(s) of the same {workload} workload on the C restatement, "
+                              f"one thread, in a child process ({one[1]:.1f} s)"}
     print(json.dumps(res))
     return 0
 

@@ -1,5 +1,6 @@
-"""bench.py pieces that run without a GPU: the one-GPU guard and the CPU baseline leg that
-times the reference's own OpenMP build (skipped where that build is absent)."""
+"""bench.py pieces that run without a GPU: the one-GPU guard and the CPU baseline legs (the
+reference's own OpenMP build, skipped where that build is absent, and the oracle on one
+thread)."""
 import json
 import os
 import subprocess
@@ -31,3 +32,14 @@ def test_cpu_baseline_times_reference_build_per_step():
     assert res is not None
     rate, seconds = res
     assert seconds > 0 and rate == pytest.approx(p.cells * 3 / seconds)
+
+
+def test_cpu_baseline_one_thread_runs_in_a_child_process():
+    """The single-thread baseline runs the oracle in a child process with OMP_NUM_THREADS=1
+    (the parent's OpenMP runtime has fixed its thread count) and reports the rate of the
+    timed steps."""
+    import bench
+    res = bench.cpu_baseline_one_thread(32, 16, 16, 1)
+    assert res is not None
+    rate, seconds = res
+    assert seconds > 0 and rate == pytest.approx(32 * 16 * 16 / seconds)
