@@ -16,7 +16,7 @@ from typing import Callable, Dict, Optional
 import numpy as np
 
 from . import autotune, cases
-from .solver import PF_PREC_MIXED, Batch, Solver, batch_case_bytes, device_memory
+from .solver import PF_PREC_FP64, PF_PREC_MIXED, Batch, Solver, batch_case_bytes, device_memory
 
 
 def _noop(*a):
@@ -24,7 +24,7 @@ def _noop(*a):
 
 
 def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: int = 11, tune_steps: int = 2,
-               plain_sample: int = 16, plain_steps: int = 20, accuracy_sample: int = 5,
+               plain_sample: int = 16, plain_steps: int = 20, accuracy_sample: int = 5, fp64_batch: bool = True,
                log: Callable = _noop) -> Dict[str, object]:
     cl = cases.batch_cases_2d(count=count, steps=steps, seed=seed, iter_max=iter_max)
     kind = cl[0][0]
@@ -53,15 +53,34 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
         wall = time.perf_counter() - t0
         diverged = int(np.sum(~np.all(np.isfinite(st["sor_error"][:, slots]), axis=0)))
         # accuracy on a sample: the batched mixed run vs each case alone in FP64
-        accuracy = []
+        accuracy, refs = [], {}
         for i in accuracy_idx:
             s = slots[i]
             p, phi = cl[i]
             with Solver(p, phi) as ref:
                 ref.step(steps + 1)
-                accuracy.append({"case": f"{p.m}x{p.n}", **{f: float(np.linalg.norm(b.interior(s, f) - ref.interior(f))
-                                                                       / max(np.linalg.norm(ref.interior(f)), 1e-300))
+                refs[i] = {f: ref.interior(f) for f in ("u", "v", "p")}
+                accuracy.append({"case": f"{p.m}x{p.n}", **{f: float(np.linalg.norm(b.interior(s, f) - refs[i][f])
+                                                                       / max(np.linalg.norm(refs[i][f]), 1e-300))
                                                               for f in ("u", "v", "p")}})
+    # the same cases batched in FP64: the batching gain alone, every case bit for bit as alone
+    fp64 = None
+    if fp64_batch:
+        kw64 = dict(precision=PF_PREC_FP64)
+        arena64 = sum(batch_case_bytes(p, **kw64) for p, _ in cl) + (64 << 20)
+        with Batch(kind, max_cases=count, arena_bytes=arena64, **kw64) as b64:
+            b64.set_config(tile_x=cfg["tile_x"], persistent_blocks=cfg["persistent_blocks"], use_graph=True)
+            slots64 = [b64.add(p, phi) for p, phi in cl]
+            b64.step(1)
+            b64.sync()
+            t0 = time.perf_counter()
+            b64.step(steps, raise_diverged=False)
+            b64.sync()
+            wall64 = time.perf_counter() - t0
+            identical = all(np.array_equal(b64.interior(slots64[i], f), refs[i][f])
+                            for i in accuracy_idx for f in ("u", "v", "p"))
+        fp64 = {"wall_s": wall64, "cases_per_s": count / wall64, "cell_steps_per_s": total_cells * steps / wall64,
+                "sample_bit_identical_to_alone": bool(identical), "sample": f"{len(accuracy_idx)} of the cases"}
     cell_steps = total_cells * steps / wall
     # plain GPU path: the same cases, one at a time, FP64 (the reference's arithmetic)
     plain_cells, plain_time = 0, 0.0
@@ -86,6 +105,8 @@ def batched_2d(count: int = 512, steps: int = 1000, iter_max: int = 60, seed: in
         "plain_sample": f"{plain_sample} of the cases, one at a time, {plain_steps} steps each",
         "autotune": {"best": cfg, "candidates": len(tuned.get("candidates", []))},
         "accuracy_vs_fp64_alone": accuracy, "diverged_cases": diverged,
+        "fp64_batch": (None if fp64 is None else
+                       {**fp64, "speedup_vs_plain_gpu": fp64["cell_steps_per_s"] / plain}),
     }
 
 
@@ -196,7 +217,6 @@ def rooms_batch(n_requests: int = 16, steps: int = 4, iter_max: int = 100, inner
     """The served rooms (same generator and seed) resident together in one batch, fixed
     count: SOR cell-updates/s of the batch in FP64 and in mixed precision, and of each room
     run alone on the single-case path in both precisions."""
-    from .solver import PF_PREC_FP64
     reqs, _ = _room_requests(n_requests, rate_per_s, steps + 1, iter_max, seed)
     cells = sum(p.cells for p, _ in reqs)
     log(f"[rooms] {n_requests} rooms, {cells} cells, {steps} steps")

@@ -236,3 +236,14 @@ def test_batch_backstep_variant_beside_plain_inlet(precision):
             for f in _fields(p):
                 np.testing.assert_array_equal(b.interior(s, f), ref[f], err_msg=f"slot {s} {f}")
             np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])
+
+
+def test_gpu_batched_2d_workload_fp64_leg_bit_identical():
+    """The batched 2D workload on a few cases: the FP64 batch of the same cases gives each
+    sampled case the bits it gets alone, and no case diverges in the mixed batch."""
+    from pixelflow_amd import workloads
+    print("[gpu] batched_2d workload, 8 cases", flush=True)
+    r = workloads.batched_2d(count=8, steps=20, plain_sample=2, plain_steps=4, accuracy_sample=2, tune_steps=1)
+    assert r["diverged_cases"] == 0
+    assert r["fp64_batch"]["sample_bit_identical_to_alone"] is True
+    assert r["fp64_batch"]["cases_per_s"] > 0 and r["cases_per_s"] > 0
