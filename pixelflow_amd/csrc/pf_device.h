@@ -257,8 +257,8 @@ __device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const Sweep
 // steps -- swap pairs and quads (quad_perm), mirror half-rows and rows (max over 16 lanes),
 // row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3 -- which leave it in lane
 // 63.  Every lane of the wave must be active.  Sweeps reduce |dp| as the bit patterns of
-// non-negative doubles, which order the same; a NaN (0x7ff8...) exceeds them all and so
-// reaches the host.
+// non-negative doubles, which order the same; a NaN pattern (0x7ff8...) given to the
+// reduction exceeds them all and is passed on.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ unsigned dpp_umax32_step(unsigned x) {
     // old value 0, the identity of the max: lanes a masked row_bcast leaves unwritten keep x
