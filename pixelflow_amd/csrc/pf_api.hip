@@ -448,7 +448,8 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
         int rc;
         if (tol_mode) {
             if ((rc = residual(true)) != PF_OK) return rc;
-            if (*res <= c->d.residual_tol || cyc == maxc) break;
+            // a non-finite residual never converges: stop at once (pf_step reports it)
+            if (*res <= c->d.residual_tol || cyc == maxc || !std::isfinite(*res)) break;
         } else {
             if (remaining <= 0) break;
             if ((rc = residual(false)) != PF_OK) return rc;
@@ -510,7 +511,7 @@ static int enqueue_fp64_tol(pf_ctx *c, double *res, int *iters) {
         PF_TRY_HIP(c, hipMemcpyAsync(&bits, c->res_val, sizeof(bits), hipMemcpyDeviceToHost, c->st));
         PF_TRY_HIP(c, hipStreamSynchronize(c->st));
         std::memcpy(res, &bits, sizeof(double));
-        if (*res <= c->d.residual_tol || cyc == maxc) break;
+        if (*res <= c->d.residual_tol || cyc == maxc || !std::isfinite(*res)) break;
         enqueue_sor(c, c->err_part, K);
         *iters += K;
         ++cycles;
@@ -591,18 +592,16 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
     for (int s = 0; s < nsteps; ++s) {
         double v;
         std::memcpy(&v, &e[(size_t)s], sizeof(double));
-        if (!std::isfinite(v)) finite = false;
+        const bool cyc = (c->d.precision == PF_PREC_MIXED) || c->d.residual_tol > 0;
+        if (!std::isfinite(v) || (cyc && !std::isfinite(c->mixed_res[(size_t)s]))) finite = false;
         if (stats) {
-            const bool mixed = (c->d.precision == PF_PREC_MIXED);
             stats[s].sor_error = v;
-            (void)mixed;
-            const bool cyc = mixed || c->d.residual_tol > 0;
             stats[s].residual = cyc ? c->mixed_res[(size_t)s] : -1.0;
             stats[s].sor_iters = cyc ? c->mixed_iters[(size_t)s] : c->d.iter_max;
             stats[s].cycles = cyc ? c->mixed_cycles[(size_t)s] : 0;
         }
     }
-    if (!finite) return fail(c, PF_EDIVERGED, "non-finite SOR error: the run diverged");
+    if (!finite) return fail(c, PF_EDIVERGED, "non-finite SOR error or residual: the run diverged");
     return PF_OK;
 }
 
@@ -617,19 +616,32 @@ extern "C" int pf_divergence(pf_ctx *c, double *out) {
     return PF_OK;
 }
 
-extern "C" int pf_force(pf_ctx *c, double *out) {
-    if (!c || !out) return PF_EINVAL;
-    if (!c->have_init) return fail(c, PF_ESTATE, "pf_force before pf_initial");
+namespace pf {
+// The body force of the context's current fields, reduced in the device buffer `work`
+// (force_blocks(g) * 6 partial sums, then the six totals), the totals copied to host `out`.
+int force_into(pf_ctx *c, double *work, double *out) {
     const PfGeom &g = c->g;
-    const int nb = pf::force_blocks(g);
-    if (!c->force_buf)
-        PF_TRY_HIP(c, hipMalloc((void **)&c->force_buf, sizeof(double) * ((size_t)nb * 6 + 6)));
-    double *tot = c->force_buf + (size_t)nb * 6;
-    pf::launch_force(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P, c->force_buf, tot);
+    double *tot = work + (size_t)pf::force_blocks(g) * 6;
+    // every partial the final reduction reads is written first, so nothing needs clearing
+    pf::launch_force(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P, work, tot);
     PF_TRY_HIP(c, hipGetLastError());
     PF_TRY_HIP(c, hipMemcpyAsync(out, tot, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     PF_TRY_HIP(c, hipStreamSynchronize(c->st));
     return PF_OK;
+}
+bool ctx_ready(const pf_ctx *c) { return c->have_init; }
+int ctx_device(const pf_ctx *c) { return c->device; }
+const PfGeom &ctx_geom(const pf_ctx *c) { return c->g; }
+int ctx_fail(pf_ctx *c, int code, const std::string &m) { return fail(c, code, m); }
+}  // namespace pf
+
+extern "C" int pf_force(pf_ctx *c, double *out) {
+    if (!c || !out) return PF_EINVAL;
+    if (!c->have_init) return fail(c, PF_ESTATE, "pf_force before pf_initial");
+    const int nb = pf::force_blocks(c->g);
+    if (!c->force_buf)
+        PF_TRY_HIP(c, hipMalloc((void **)&c->force_buf, sizeof(double) * ((size_t)nb * 6 + 6)));
+    return pf::force_into(c, c->force_buf, out);
 }
 
 extern "C" int pf_sor_fused(const pf_ctx *c) {

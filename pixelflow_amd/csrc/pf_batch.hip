@@ -202,7 +202,7 @@ __device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, 
                                                            c.fw, c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, C.phi);
                 sweep_store<DIM>(g, s, j, k, o[q], pn, C.P + own);
                 const double d = fabs(pn - c.c);
-                e = (d > e) ? d : e;
+                e = max_keep_nan(e, d);
             }
         } else if (KIND == SWP_FP32) {
             const float cdx = (float)(C.a.dt / (C.a.dx * C.a.dx)), cdy = (float)(C.a.dt / (C.a.dy * C.a.dy)),
@@ -220,7 +220,7 @@ __device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, 
                                                   c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, C.phi, cdx, cdy, cdz);
                 sweep_store<DIM>(g, s, j, k, o[q], en, C.E + own);
                 const double d = fabs((double)en - (double)c.c);
-                e = (d > e) ? d : e;
+                e = max_keep_nan(e, d);
             }
         } else {
 #pragma unroll
@@ -229,7 +229,7 @@ __device__ __forceinline__ void swp_tile(const BatchView &v, int t, int colour, 
                 if (DIM == 3 && k > g.l) break;
                 const double d = residual_cell<DIM, SEL == 1>(g, C.a, colour, i, j, k, sh, o[q], C.P, C.PHIs, C.BB,
                                                               C.phi, C.R, C.E);
-                e = (d > e) ? d : e;
+                e = max_keep_nan(e, d);
             }
         }
     }
@@ -310,7 +310,8 @@ __global__ void k_batch_tol_decide(BatchView v, int *active, const unsigned long
         }
         if (!active[ci]) continue;
         const double r = __longlong_as_double((long long)res_hist[row + slot]);
-        if (r <= tol || cyc == maxc) {
+        // a non-finite residual (NaN, or infinity) never converges: the case stops cycling
+        if (r <= tol || cyc == maxc || !(fabs(r) <= 1.7976931348623157e308)) {
             active[ci] = 0;
         } else {
             it_hist[row + slot] += K;
@@ -1195,6 +1196,8 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
                 std::memcpy(&e, &eh[(size_t)s * b->max_cases + slot], 8);
                 std::memcpy(&r, &rh[(size_t)s * b->max_cases + slot], 8);
                 if (!std::isfinite(e)) finite = false;
+                if ((tol || (b->kind.report_residual && b->kind.precision == PF_PREC_MIXED)) && !std::isfinite(r))
+                    finite = false;
                 o.sor_error = e;
                 if (tol) {
                     o.residual = r;
@@ -1207,7 +1210,7 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
                 }
             }
     }
-    if (!finite) { b->msg = "non-finite SOR error in a case: it diverged"; return PF_EDIVERGED; }
+    if (!finite) { b->msg = "non-finite SOR error or residual in a case: it diverged"; return PF_EDIVERGED; }
     return PF_OK;
 }
 

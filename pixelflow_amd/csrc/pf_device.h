@@ -11,6 +11,11 @@ namespace pf {
 
 __device__ __forceinline__ double fmax_ref(double a, double b) { return (a > b) ? a : b; }
 
+// Running max of |dp| (or |r/ap|) over the cells a thread updates.  For finite values it
+// is the reference's max; a NaN operand is kept and stays kept (every later compare with
+// it is false), so a non-finite pressure reaches the reduction and the step reports it.
+__device__ __forceinline__ double max_keep_nan(double e, double d) { return (d > e || d != d) ? d : e; }
+
 // ------------------------------------------------------------------------------------
 // boundary_matrix row edits, replayed per cell in the reference's order.
 // Uniform: inlet fold at i=1 then outlet row at i=m (3D :636-659, 2D :424-436).

@@ -227,7 +227,7 @@ __device__ double fused_tile(const PfGeom &g, const StepArgs &a, const FusedGeom
                 const int t = dx - 1 + s;
                 const bool own = t >= 0 && t < TS && dy >= 1 && dy <= TY && j == jw && (DIM == 2 || k == kw);
                 if (own) {
-                    if (DIM == 2) err = fmax(err, fabs((double)v - (double)old));
+                    if (DIM == 2) err = max_keep_nan(err, fabs((double)v - (double)old));
                     if (write) put(c0, h, j, k, v);
                 }
             }
@@ -270,7 +270,7 @@ __device__ double fused_tile(const PfGeom &g, const StepArgs &a, const FusedGeom
             else
                 v = (T)sor32_value<DIM>(g, a, i, j, k, old, pW, pE, pS, pN, pB, pT, fc, fW, fE, fS, fN, fB, fT,
                                         f.r1[e], p.phi, p.cdx, p.cdy, p.cdz);
-            err = fmax(err, fabs((double)v - (double)old));
+            err = max_keep_nan(err, fabs((double)v - (double)old));
             put(c1, h, j, k, v);
         }
     };

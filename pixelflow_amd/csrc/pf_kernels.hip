@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs
                                                    c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, phi);
             sweep_store<DIM>(g, s, t.j, k, o[q], pn, Pown);
             const double e = fabs(pn - c.c);
-            err = (e > err) ? e : err;
+            err = max_keep_nan(err, e);
         }
     }
     if (s.track_err) wave_max_to_slot(err, err_slot);

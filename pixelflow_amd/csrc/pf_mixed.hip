@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s
                                               c.fs, c.fn, c.fb, c.ft, c.rhs, phi, cdx, cdy, cdz);
             sweep_store<DIM>(g, s, t.j, k, o[q], en, Eown);
             const double e = fabs((double)en - (double)c.c);
-            err = (e > err) ? e : err;
+            err = max_keep_nan(err, e);
         }
     }
     if (s.track_err) wave_max_to_slot(err, err_slot);

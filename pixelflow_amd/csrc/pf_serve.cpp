@@ -70,6 +70,7 @@ struct pf_server {
     std::thread worker;
     int64_t steps_run = 0;
     double busy_s = 0;
+    int waiters = 0;                    // threads inside pf_serve_wait
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -175,7 +176,11 @@ void worker_loop(pf_server *s) {
                 Request &r = *kv.second;
                 r.steps_done += n;
                 bool finite = true;
-                for (int q = 0; q < n; ++q) finite = finite && std::isfinite(st[(size_t)q * s->max_cases + kv.first].sor_error);
+                for (int q = 0; q < n; ++q) {
+                    const pf_stats &x = st[(size_t)q * s->max_cases + kv.first];
+                    // residual is -1 where it is not evaluated; NaN or infinite where it is
+                    finite = finite && std::isfinite(x.sor_error) && std::isfinite(x.residual);
+                }
                 for (int q = 0; q < n; ++q) r.sor_iters += st[(size_t)q * s->max_cases + kv.first].sor_iters;
                 r.last = st[(size_t)(n - 1) * s->max_cases + kv.first];
                 if (!finite) diverged.push_back(kv.second);
@@ -185,7 +190,7 @@ void worker_loop(pf_server *s) {
             for (auto &r : diverged) s->running.erase(r->slot);
         }
         for (auto &r : finished) retire(s, r, false, "");
-        for (auto &r : diverged) retire(s, r, true, "diverged: non-finite SOR error");
+        for (auto &r : diverged) retire(s, r, true, "diverged: non-finite SOR error or residual");
     }
 }
 
@@ -227,6 +232,27 @@ extern "C" int pf_serve_destroy(pf_server *s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    {
+        // the worker has stopped: whatever is still queued or running will never finish
+        std::lock_guard<std::mutex> lk(s->mu);
+        const double t = s->now();
+        for (auto &kv : s->all) {
+            Request &r = *kv.second;
+            if (r.state == PF_SERVE_QUEUED || r.state == PF_SERVE_RUNNING) {
+                r.state = PF_SERVE_FAILED;
+                r.error = "server stopped";
+                r.t_finish = t;
+            }
+        }
+        s->queue.clear();
+        s->running.clear();
+        s->cv_done.notify_all();
+    }
+    {
+        // threads blocked in pf_serve_wait wake on the failed state; let them leave first
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+    }
     pf_batch_destroy(s->batch);
     delete s;
     return PF_OK;
@@ -277,9 +303,12 @@ extern "C" int pf_serve_wait(pf_server *s, int64_t id, double timeout_s, int *st
     if (it == s->all.end()) return PF_EINVAL;
     std::shared_ptr<Request> r = it->second;
     auto finished = [&] { return r->state == PF_SERVE_DONE || r->state == PF_SERVE_FAILED; };
+    ++s->waiters;
     if (timeout_s < 0) s->cv_done.wait(lk, finished);
     else s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
     *state = r->state;
+    // pf_serve_destroy waits until no thread is left in here before it frees the server
+    if (--s->waiters == 0) s->cv_done.notify_all();
     return PF_OK;
 }
 
