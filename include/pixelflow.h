@@ -115,6 +115,15 @@ int pf_divergence(pf_ctx *ctx, double *out_interior);
  * device tree (deterministic, equal to the reference's serial sum up to rounding). */
 int pf_force(pf_ctx *ctx, double *out);
 
+/* No reference counterpart.  pf_force without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same kernels and the same result bit for bit, with the reduction's
+ * partial sums in a fixed device array of the library (1024 partial blocks, about 48 KB
+ * per GPU, held until the process ends) instead of a buffer pf_force allocates on its first
+ * call.  Sizes past that array are rejected with PF_EINVAL (the force reduction never
+ * needs more).  All calls on one GPU share the array: calls that overlap on one GPU race. */
+int pf_force_noalloc(pf_ctx *ctx, double *out);
+
 /* Device bytes the context holds (admission control sizes cases with this). */
 int64_t pf_footprint_bytes(const pf_desc *desc);
 

@@ -59,7 +59,7 @@ _lib: Optional[ctypes.CDLL] = None
 
 # Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
 EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
-           "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_force", "pf_footprint_bytes", "pf_sync",
+           "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_force", "pf_force_noalloc", "pf_footprint_bytes", "pf_sync",
            "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_sor_fused", "pf_probe_copy",
            "pf_batch_create", "pf_batch_destroy", "pf_batch_last_error", "pf_batch_case_bytes", "pf_batch_free_bytes",
            "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
     L.pf_batch_last_error.restype = ctypes.c_char_p
     L.pf_batch_last_error.argtypes = [vp]
     L.pf_force.argtypes = [vp, dp]
+    L.pf_force_noalloc.argtypes = [vp, dp]
     L.pf_batch_case_bytes.restype = ctypes.c_int64
     L.pf_batch_case_bytes.argtypes = [ctypes.POINTER(PfDesc)]
     L.pf_batch_free_bytes.restype = ctypes.c_int64
