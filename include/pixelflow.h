@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PF_ABI_VERSION 2
+#define PF_ABI_VERSION 3
 
 enum pf_status {
     PF_OK = 0,
@@ -36,6 +36,13 @@ enum pf_status {
 
 enum pf_bc { PF_BC_UNIFORM = 0, PF_BC_AIR = 1 };
 enum pf_precision { PF_PREC_FP64 = 0, PF_PREC_MIXED = 1 };
+/* Pressure solve of tolerance mode (residual_tol > 0).  No reference counterpart: the
+ * reference runs iter_max SOR iterations with no convergence test
+ * (ibm_3d_uniform_omp_cpu.f90:460-611). */
+enum pf_solver {
+    PF_SOLVER_SOR = 0,   /* cycles of inner_iters red/black SOR iterations */
+    PF_SOLVER_MG = 1     /* BiCGStab preconditioned by a multigrid V-cycle, to inner_tol per cycle */
+};
 enum pf_field { PF_FIELD_U = 0, PF_FIELD_V = 1, PF_FIELD_W = 2, PF_FIELD_P = 3, PF_FIELD_POROSITY = 4 };
 
 /* Everything one case needs: the controlDict variables (src/omp_parallel/lib/global.f90:
@@ -55,6 +62,7 @@ typedef struct pf_desc {
     int32_t inlet_porosity;   /* 2D uniform: inlet and initial velocity scaled by porosity, the
                                  backward-facing-step executable (ibm_2d_backstep_omp_cpu.f90:
                                  533-534, 615-618); 0 for the others */
+    int32_t solver;           /* pf_solver (tolerance mode; fixed-count mode always runs SOR) */
     double dx, dy, dz, dt;
     double xnue, xlambda, density, thickness;
     double inlet_velocity, outlet_pressure, aoa;
@@ -62,16 +70,18 @@ typedef struct pf_desc {
     double residual_tol;      /* mixed: 0 = run iter_max iterations in cycles of inner_iters (the
                                  reference's fixed count); > 0 = cycles of inner_iters until the
                                  FP64 residual max|r/ap| <= residual_tol (or max_cycles) */
+    double inner_tol;         /* PF_SOLVER_MG: relative residual of each inner solve (0 = 1e-3);
+                                 inner_iters then caps its BiCGStab iterations (0 = 30) */
 } pf_desc;
 
 /* Per-step report. sor_error = the reference's "SOR iteration no. .. -- p error:" value
  * (running max of |p - p_old|, ibm_3d_uniform_omp_cpu.f90:575-583). */
 typedef struct pf_stats {
-    double sor_error;
+    double sor_error;         /* PF_SOLVER_MG: max |p_new - p_old| of the step's updates */
     double residual;          /* mixed: FP64 residual max|r/ap| (tolerance mode: the last one
                                  evaluated; fixed-count mode: after the step if report_residual,
                                  else -1); FP64 path: -1 */
-    int32_t sor_iters;        /* SOR iterations executed in the step */
+    int32_t sor_iters;        /* SOR iterations executed in the step (PF_SOLVER_MG: V-cycles) */
     int32_t cycles;           /* mixed: defect-correction cycles (FP64 path: 0) */
 } pf_stats;
 

@@ -25,7 +25,8 @@ OUT = os.path.join(PKG, "_build")
 LIB = os.path.join(OUT, "libpixelflow.so")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pf_kernels.hip", "pf_mixed.hip", "pf_api.hip", "pf_batch.hip", "pf_serve.cpp", "pf_noalloc.hip"]
+HIP_SOURCES = ["pf_kernels.hip", "pf_mixed.hip", "pf_api.hip", "pf_batch.hip", "pf_serve.cpp", "pf_noalloc.hip",
+               "pf_mg.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

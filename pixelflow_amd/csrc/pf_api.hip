@@ -16,6 +16,7 @@
 #include "../../include/pixelflow.h"
 #include "pf_host.h"
 #include "pf_kernels.h"
+#include "pf_mg.h"
 
 struct pf_ctx {
     pf_desc d{};
@@ -56,6 +57,10 @@ struct pf_ctx {
     std::vector<double> mixed_res;
     std::vector<int> mixed_cycles, mixed_iters;
     double solve_us_avg = 0.0;      // timing mode: average pressure-solve time per step
+    // tolerance mode with PF_SOLVER_MG: the multigrid-preconditioned solver of the correction
+    // equation, FP32 for mixed precision, FP64 otherwise
+    pf::MgSolver<float> *mg32 = nullptr;
+    pf::MgSolver<double> *mg64 = nullptr;
     std::string msg;
 };
 
@@ -90,7 +95,10 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     const int spl = (d->dim == 3) ? 6 : 5;
     const int64_t interior = (int64_t)d->m * d->n * (d->dim == 3 ? d->l : 1);
     const int64_t mixed = (d->precision == PF_PREC_MIXED) ? (int64_t)4 * 3 * 2 * g.HS : 0;
-    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + 4096;
+    const int64_t mg = (d->residual_tol > 0 && d->solver == PF_SOLVER_MG)
+                           ? pf::mg_bytes(g, d->bc, d->precision == PF_PREC_MIXED)
+                           : 0;
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + 4096;
 }
 
 static int alloc(pf_ctx *c, double **p, int64_t n) {
@@ -116,6 +124,8 @@ static void release(pf_ctx *c) {
     float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2};
     for (float **p : f32)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (c->mg32) { pf::mg_destroy(*c->mg32); delete c->mg32; c->mg32 = nullptr; }
+    if (c->mg64) { pf::mg_destroy(*c->mg64); delete c->mg64; c->mg64 = nullptr; }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
     c->evs.clear();
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -217,6 +227,32 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         const char *x = std::getenv("PF_XCD_REMAP");
         if (x && *x) c->xcd_remap = c->xcd_remap64 = std::atoi(x) != 0;
     }
+    if (desc->residual_tol > 0 && desc->solver == PF_SOLVER_MG) {
+        pf::MgCase mc;
+        mc.g = g;
+        mc.a = c->a;
+        mc.phi = c->phi;
+        mc.PHIs = c->PHIs;
+        mc.PHI32 = c->PHI32;
+        mc.kb32 = c->sweep_kb32;
+        mc.kb64 = c->sweep_kb;
+        mc.xw = c->sweep_xw;
+        mc.remap32 = c->xcd_remap;
+        mc.remap64 = c->xcd_remap64;
+        std::string why;
+        bool ok;
+        if (desc->precision == PF_PREC_MIXED) {
+            c->mg32 = new (std::nothrow) pf::MgSolver<float>();
+            ok = c->mg32 && pf::mg_create(*c->mg32, mc, &why);
+        } else {
+            c->mg64 = new (std::nothrow) pf::MgSolver<double>();
+            ok = c->mg64 && pf::mg_create(*c->mg64, mc, &why);
+        }
+        if (!ok) {
+            rc = fail(c, PF_ENOMEM, why.empty() ? std::string("multigrid solver allocation failed") : why);
+            goto bad;
+        }
+    }
     *out = c;
     return PF_OK;
 bad:
@@ -266,6 +302,9 @@ static int derive_from_porosity(pf_ctx *c) {
     PF_TRY_HIP(c, hipGetLastError());
     c->sor_variant = (hbad == 0) ? 2 : 1;
     if (c->sor_forced >= 0 && c->sor_forced <= 2 && !(c->sor_forced == 2 && hbad != 0)) c->sor_variant = c->sor_forced;
+    // the multigrid operators follow the porosity (rebuilt at the next solve)
+    if (c->mg32) { c->mg32->c.fast = (c->sor_variant == 2); c->mg32->built = false; }
+    if (c->mg64) { c->mg64->c.fast = (c->sor_variant == 2); c->mg64->built = false; }
     c->have_phi = true;
     return PF_OK;
 }
@@ -489,6 +528,45 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
     return cycles;
 }
 
+// Tolerance-mode solve with the multigrid-preconditioned inner solver (pf_mg.h): before
+// every cycle the FP64 residual r = bb - A p of the reference operator (k_residual, stored
+// in precision T as the inner right-hand side) and its max|r/ap|; then A e = r to inner_tol
+// by BiCGStab with V-cycles; then p += e in FP64, its max |e| kept as the step's diagnostic.
+template <typename T>
+static int enqueue_mg(pf_ctx *c, pf::MgSolver<T> &S, double *res, int *iters) {
+    const PfGeom &g = c->g;
+    const bool fast = (c->sor_variant == 2);
+    const int maxc = c->d.max_cycles > 0 ? c->d.max_cycles : 50;
+    const double rtol = c->d.inner_tol > 0 ? c->d.inner_tol : 1e-3;
+    const int maxit = c->d.inner_iters > 0 ? c->d.inner_iters : 30;
+    int cycles = 0;
+    *iters = 0;
+    *res = -1.0;
+    for (int cyc = 0; cyc <= maxc; ++cyc) {
+        for (int half = 0; half < 2; ++half) {
+            pf::SweepArgs s{};
+            s.colour = half;
+            s.track_err = 1;
+            s.xcd_remap = c->xcd_remap;
+            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, S.r, (T *)nullptr, c->res_part,
+                                fast);
+        }
+        pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
+        unsigned long long bits = 0;
+        PF_TRY_HIP(c, hipMemcpyAsync(&bits, c->res_val, sizeof(bits), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        std::memcpy(res, &bits, sizeof(double));
+        if (*res <= c->d.residual_tol || cyc == maxc || !std::isfinite(*res)) break;
+        int apps = 0;
+        double rel = 0.;
+        PF_TRY_HIP(c, pf::mg_solve(S, c->st, rtol, maxit, &apps, &rel));
+        *iters += apps;
+        pf::launch_update_track(c->st, g, S.x, c->P, c->err_part, c->nparts);
+        ++cycles;
+    }
+    return cycles;
+}
+
 // FP64 solve to a residual tolerance: cycles of inner_iters reference SOR iterations with an
 // FP64 residual check before each (the FP64 counterpart of the mixed tolerance mode).
 static int enqueue_fp64_tol(pf_ctx *c, double *res, int *iters) {
@@ -504,7 +582,8 @@ static int enqueue_fp64_tol(pf_ctx *c, double *res, int *iters) {
             s.colour = half;
             s.track_err = 1;
             s.xcd_remap = c->xcd_remap;
-            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, nullptr, nullptr, c->res_part, fast);
+            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, (float *)nullptr, (float *)nullptr, c->res_part,
+                                fast);
         }
         pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
         unsigned long long bits = 0;
@@ -552,7 +631,8 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         if (c->d.precision == PF_PREC_MIXED) {
             double res = -1.0;
             int iters = 0;
-            const int cyc = enqueue_mixed(c, c->err_part, &res, &iters);
+            const int cyc = c->mg32 ? enqueue_mg(c, *c->mg32, &res, &iters)
+                                    : enqueue_mixed(c, c->err_part, &res, &iters);
             if (cyc < 0) return cyc;
             c->mixed_res.push_back(res);
             c->mixed_cycles.push_back(cyc);
@@ -560,7 +640,7 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         } else if (c->d.residual_tol > 0) {
             double res = -1.0;
             int iters = 0;
-            const int cyc = enqueue_fp64_tol(c, &res, &iters);
+            const int cyc = c->mg64 ? enqueue_mg(c, *c->mg64, &res, &iters) : enqueue_fp64_tol(c, &res, &iters);
             if (cyc < 0) return cyc;
             c->mixed_res.push_back(res);
             c->mixed_cycles.push_back(cyc);

@@ -664,12 +664,15 @@ __device__ __forceinline__ CellIdx sweep_cell(const PfGeom &g, const SweepArgs &
     return c;
 }
 
-template <int DIM, bool FAST>
+// The FP64 residual r = bb - A p of one cell, stored as TR (float for the mixed solve,
+// double for the FP64 one); E (optional) is cleared at the cell and at the periodic ghost
+// slot it feeds.  Returns |r / ap|.
+template <int DIM, bool FAST, typename TR = float>
 __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs &a, int colour, int i, int j, int k,
                                                 int sh, int64_t o, const double *__restrict__ P_,
                                                 const double *__restrict__ PHIs_, const double *__restrict__ BB_,
-                                                const double *__restrict__ phi, float *__restrict__ R_,
-                                                float *__restrict__ E_) {
+                                                const double *__restrict__ phi, TR *__restrict__ R_,
+                                                TR *__restrict__ E_) {
     const auto P = gmem(P_);
     const auto PHIs = gmem(PHIs_);
     const auto BB = gmem(BB_);
@@ -707,14 +710,14 @@ __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs 
         r = r - c.ab * pB;
     }
     r = r - ap * pc;
-    if (R) R[own + o] = (float)r;
-    if (E) E[own + o] = 0.f;
+    if (R) R[own + o] = (TR)r;
+    if (E) E[own + o] = TR(0);
     if (E && a.bc == PF_BC_UNIFORM) {  // periodic ghost slots of e that a sweep or refresh writes
-        if (j == g.n) E[pf_soff(g, i, 0, k)] = 0.f;
-        if (j == 1) E[pf_soff(g, i, g.n + 1, k)] = 0.f;
+        if (j == g.n) E[pf_soff(g, i, 0, k)] = TR(0);
+        if (j == 1) E[pf_soff(g, i, g.n + 1, k)] = TR(0);
         if (DIM == 3) {
-            if (k == g.l) E[pf_soff(g, i, j, 0)] = 0.f;
-            if (k == 1) E[pf_soff(g, i, j, g.l + 1)] = 0.f;
+            if (k == g.l) E[pf_soff(g, i, j, 0)] = TR(0);
+            if (k == 1) E[pf_soff(g, i, j, g.l + 1)] = TR(0);
         }
     }
     return fabs(r / ap);

@@ -27,10 +27,11 @@ inline bool desc_valid(const pf_desc *d, std::string &why) {
         return false;
     }
     if (d->precision != PF_PREC_FP64 && d->precision != PF_PREC_MIXED) { why = "precision must be FP64 or MIXED"; return false; }
-    if (d->inner_iters < 0 || d->max_cycles < 0 || !(d->residual_tol >= 0)) {
-        why = "inner_iters, max_cycles >= 0 and residual_tol >= 0";
+    if (d->inner_iters < 0 || d->max_cycles < 0 || !(d->residual_tol >= 0) || !(d->inner_tol >= 0)) {
+        why = "inner_iters, max_cycles >= 0 and residual_tol, inner_tol >= 0";
         return false;
     }
+    if (d->solver != PF_SOLVER_SOR && d->solver != PF_SOLVER_MG) { why = "solver must be SOR or MG"; return false; }
     if (!(d->dx > 0) || !(d->dy > 0) || (d->dim == 3 && !(d->dz > 0)) || !(d->dt > 0) || !(d->density != 0)) {
         why = "dx, dy, dz, dt must be > 0 and density != 0";
         return false;

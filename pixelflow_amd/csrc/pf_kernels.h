@@ -83,7 +83,15 @@ void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepAr
                      unsigned long long *slots, bool fast);
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
                   const float *R, const double *phi, unsigned long long *err_slot);
+void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
+                     const double *PHIs, const double *BB, const double *phi, double *R, double *E,
+                     unsigned long long *slots, bool fast);
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P);
+void launch_update(hipStream_t st, const PfGeom &g, const double *E, double *P);
+void launch_update_track(hipStream_t st, const PfGeom &g, const float *E, double *P, unsigned long long *slots,
+                         int nslots);
+void launch_update_track(hipStream_t st, const PfGeom &g, const double *E, double *P, unsigned long long *slots,
+                         int nslots);
 void launch_ghost_refresh32(hipStream_t st, const PfGeom &g, int do_y, int do_z, float *E);
 void launch_split32(hipStream_t st, const PfGeom &g, const double *src, float *dst);   // blocks of one colour-sweep launch (= error slots)
 void launch_reduce_err(hipStream_t st, unsigned long long *slots, int nslots, unsigned long long *step_err);

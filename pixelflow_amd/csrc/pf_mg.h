@@ -1,0 +1,140 @@
+// pf_mg.h -- multigrid-preconditioned BiCGStab for the pressure correction A e = r.
+//
+// The reference solves A p = bb with a fixed number of red/black SOR iterations and has no
+// convergence exit (ibm_3d_uniform_omp_cpu.f90:460-611).  SOR at omega = 1.7 contracts
+// the smooth error by about 1 - 2e-4 per iteration on a 512-cell extent, so a residual
+// tolerance of 1e-8 is out of its reach on the large grids.  This solver reaches it: the
+// FP64 outer loop (r = bb - A p in FP64, p += e) stays as in the SOR defect correction, and
+// the correction equation A e = r is solved to a relative tolerance by BiCGStab,
+// right-preconditioned with one multigrid V-cycle per application:
+//
+// * level 0 is the case's own grid in the colour-split layout; its smoother is the red/black
+//   colour sweep of the solve (omega = 1), FP32 for the mixed precision and FP64 otherwise;
+// * coarse levels aggregate 2x2x2 cells (2x2 in 2D) with the Galerkin operator of piecewise
+//   constant interpolation, P^T A P, built once per porosity field: the coarse coupling
+//   across a face is the sum of the fine couplings that cross it, so the solid cells' 1e-6
+//   faces and the fluid's faces keep their weights.  Rows the boundary rules decouple (the
+//   outlet rows: all six couplings zero, p set by its own row) are left out of the coarse
+//   space, so a cell next to them sees them as fixed values on every level;
+// * coarse levels are dense arrays with one ghost layer; periodic ghosts are copies (written
+//   with the update for even extents, by a refresh pass for odd ones, so every update of a
+//   pass reads values from before the pass across a wrap, as the reference's p_old does);
+// * the coarsest level (at most MG_COARSEST_CELLS cells) is solved by red/black sweeps in
+//   LDS inside one block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "pf_kernels.h"
+#include "pf_layout.h"
+
+namespace pf {
+
+constexpr int MG_COARSEST_CELLS = 512;   // cells of the coarsest level (one block, LDS)
+constexpr int MG_MAX_LEVELS = 16;
+constexpr int MG_NPART = 1024;           // blocks of the reduction kernels (partial sums)
+
+// A coarse level: interior extents m x n x l (l = 1 in 2D) with one ghost layer around
+// (in z only in 3D); cell (I,J,K), 0-based, at (I+1) + PX*((J+1) + Y*(K+KG)).
+struct MgGeom {
+    int dim;
+    int m, n, l;
+    int PX, Y, Z, KG;
+    int64_t N;           // PX * Y * Z
+    int per;             // y (and z in 3D) periodic
+    int fuse_y, fuse_z;  // periodic with an even extent: mirrors written by the updates
+    int ref_y, ref_z;    // periodic with an odd extent: a refresh pass after each update
+};
+
+PF_HD int64_t mg_off(const MgGeom &g, int I, int J, int K) {
+    return (int64_t)(I + 1) + (int64_t)g.PX * ((int64_t)(J + 1) + (int64_t)g.Y * (int64_t)(K + g.KG));
+}
+
+inline MgGeom mg_make_geom(int dim, int m, int n, int l, int per) {
+    MgGeom g;
+    g.dim = dim;
+    g.m = m;
+    g.n = n;
+    g.l = (dim == 3) ? l : 1;
+    g.PX = m + 2;
+    g.Y = n + 2;
+    g.KG = (dim == 3) ? 1 : 0;
+    g.Z = (dim == 3) ? g.l + 2 : 1;
+    g.N = (int64_t)g.PX * g.Y * g.Z;
+    g.per = per;
+    g.fuse_y = per && (n % 2 == 0);
+    g.ref_y = per && (n % 2 == 1);
+    g.fuse_z = per && dim == 3 && (g.l % 2 == 0);
+    g.ref_z = per && dim == 3 && (g.l % 2 == 1);
+    return g;
+}
+
+// The coarse extent of a fine extent f (aggregates of two, the last one single when f is
+// odd); z is not coarsened in 2D.
+inline int mg_coarse_extent(int f) { return (f + 1) / 2; }
+
+template <typename T>
+struct MgLevel {
+    MgGeom g;
+    T *e = nullptr, *r = nullptr;                // correction and right-hand side
+    T *ap = nullptr, *cE = nullptr, *cW = nullptr, *cN = nullptr, *cS = nullptr, *cT = nullptr, *cB = nullptr;
+    uint8_t *keep = nullptr;                     // 1: the row takes part in the next coarser level
+};
+
+// What the solver reads of the case it serves.
+struct MgCase {
+    PfGeom g;
+    StepArgs a;
+    const double *phi = nullptr;     // natural FP64 porosity (the air rows' fluid test)
+    const double *PHIs = nullptr;    // colour-split FP64 porosity
+    const float *PHI32 = nullptr;    // colour-split FP32 porosity (FP32 solver)
+    bool fast = false;               // FP64 faces by the checked constant-divisor division
+    int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings
+};
+
+// Tuning of the V-cycle (fixed; the same for every case).
+struct MgParams {
+    int nu1 = 2, nu2 = 2;            // red/black sweeps before and after the coarse correction
+    double alpha = 1.6;              // over-correction of the piecewise-constant interpolation
+    int coarsest_sweeps = 60;        // red/black sweeps of the coarsest level
+};
+
+// The solver of one case in precision T (float: mixed precision; double: FP64).
+template <typename T>
+struct MgSolver {
+    MgCase c;
+    MgParams prm;
+    std::vector<MgLevel<T>> lev;     // lev[0]: the case's grid (no coarse arrays); lev[1..]: coarse
+    // level-0 vectors, colour-split (2 * HS each): x = the solution e, r = the residual (holds
+    // the right-hand side b on entry), the rest BiCGStab's
+    T *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
+      *y = nullptr, *z = nullptr;
+    double *scal = nullptr;          // device scalars (see pf_mg.hip)
+    double *part = nullptr;          // MG_NPART x 4 partial sums
+    double *h_scal = nullptr;        // pinned host copy of the scalars the host tests
+    void *mem = nullptr;             // the one device allocation holding all of the above
+    int64_t bytes = 0;
+    bool built = false;              // coarse operators match the current porosity
+};
+
+// Device bytes of a solver for grid g (one allocation).
+int64_t mg_bytes(const PfGeom &g, int bc, bool fp32);
+// Allocate (hipMalloc, hipHostMalloc) and carve the arrays; false on failure with *why set.
+template <typename T>
+bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why);
+template <typename T>
+void mg_destroy(MgSolver<T> &S);
+// Build the coarse operators from the case's porosity (after every porosity change).
+template <typename T>
+void mg_setup(MgSolver<T> &S, hipStream_t st);
+// Solve A e = r to ||r_k|| <= rtol ||r_0|| (or maxit BiCGStab iterations) from e = 0, with
+// S.r holding r_0 on entry; e is left in S.x with its periodic ghost copies.  *applications
+// receives the number of V-cycles applied, *relres ||r_k|| / ||r_0|| (recursive residual).
+// Returns hipSuccess or the first HIP error.
+template <typename T>
+hipError_t mg_solve(MgSolver<T> &S, hipStream_t st, double rtol, int maxit, int *applications, double *relres);
+
+}  // namespace pf

@@ -1,0 +1,1006 @@
+// pf_mg.hip -- multigrid-preconditioned BiCGStab for A e = r (see pf_mg.h).
+//
+// Kernels, by level:
+//   level 0 (the case's grid, colour-split T arrays):  the red/black colour sweeps of the
+//     solve with omega = 1 (k_sor32 / k_sor_phi) smooth; k_mg_restrict0 sums the residual
+//     of the kept cells of each 2x2x2 aggregate into level 1 (and clears level 1's e);
+//     k_mg_prolong0 adds alpha times the aggregate's correction back; k_mg_apply0 is A y
+//     with the dot products BiCGStab needs in the same pass;
+//   coarse levels (dense, ghosted):  k_mg_smooth (one colour of a red/black sweep),
+//     k_mg_restrict, k_mg_prolong, k_mg_refresh (odd periodic extents), k_mg_coarsest (the
+//     last level, solved in LDS by one block);
+//   set-up:  k_mg_gal0 / k_mg_gal (Galerkin sums P^T A P).
+// BiCGStab's scalars stay on the device (k_bicg_scalar*), the host reads one word per
+// iteration to test convergence.  Reductions are fixed-order trees, so a solve is
+// deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "pf_cells.h"
+#include "pf_device.h"
+#include "pf_kernels.h"
+#include "pf_mg.h"
+
+namespace pf {
+
+template <typename T>
+struct MgK {
+    T cdx, cdy, cdz;   // FP32 solver: face scale dt / d^2 as the FP32 sweep computes it
+};
+
+// ------------------------------------------------------------------------------------
+// Level 0: the operator of the colour sweep of precision T at fine cell (i, j, k).
+// float: the FP32 sweep's faces (sor32_value); double: the reference's faces (sor_value).
+// Returns ap; c receives the six couplings after the boundary rules.
+// ------------------------------------------------------------------------------------
+template <typename T, int DIM, bool FAST>
+__device__ __forceinline__ T op0(const PfGeom &g, const StepArgs &a, const MgK<T> &K, const T *__restrict__ F_,
+                                 const double *__restrict__ phi, int i, int j, int k, CoefT<T> &c) {
+    const auto F = gmem(F_);
+    const T fc = F[pf_soff(g, i, j, k)];
+    const T fE = F[pf_soff(g, i + 1, j, k)], fW = F[pf_soff(g, i - 1, j, k)];
+    const T fN = F[pf_soff(g, i, j + 1, k)], fS = F[pf_soff(g, i, j - 1, k)];
+    T fT = T(0), fB = T(0);
+    if (DIM == 3) {
+        fT = F[pf_soff(g, i, j, k + 1)];
+        fB = F[pf_soff(g, i, j, k - 1)];
+    }
+    T ap;
+    if constexpr (std::is_same<T, float>::value) {
+        c.ae = face32(fE, fc, K.cdx);
+        c.aw = face32(fc, fW, K.cdx);
+        c.an = face32(fN, fc, K.cdy);
+        c.as = face32(fc, fS, K.cdy);
+        c.at = 0.f;
+        c.ab = 0.f;
+        if (DIM == 3) {
+            c.at = face32(fT, fc, K.cdz);
+            c.ab = face32(fc, fB, K.cdz);
+        }
+        ap = -(c.ae + c.aw + c.an + c.as + c.at + c.ab);
+    } else {
+        c.ae = face_any<FAST>(fE, fc, a.dt, a.dx, a.rdx);
+        c.aw = face_any<FAST>(fc, fW, a.dt, a.dx, a.rdx);
+        c.an = face_any<FAST>(fN, fc, a.dt, a.dy, a.rdy);
+        c.as = face_any<FAST>(fc, fS, a.dt, a.dy, a.rdy);
+        c.at = 0.;
+        c.ab = 0.;
+        if (DIM == 3) {
+            c.at = face_any<FAST>(fT, fc, a.dt, a.dz, a.rdz);
+            c.ab = face_any<FAST>(fc, fB, a.dt, a.dz, a.rdz);
+            ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
+        } else {
+            ap = (((-c.ae) - c.aw) - c.an) - c.as;
+        }
+    }
+    if (is_edge_row(g, a, i, j, k)) row_rules<false, T>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    return ap;
+}
+
+// A row that the boundary rules decouple (all six couplings zero: the outlet rows) takes
+// no part in the coarse space.
+__device__ __forceinline__ bool dec0(const PfGeom &g, const StepArgs &a, int i, int j, int k, const double *phi) {
+    if (!is_edge_row(g, a, i, j, k)) return false;
+    if (a.bc == PF_BC_UNIFORM) return i == g.m;
+    CoefT<float> c{1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+    row_rules<false, float>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    return c.ae == 0.f && c.aw == 0.f && c.an == 0.f && c.as == 0.f && c.at == 0.f && c.ab == 0.f;
+}
+
+// (A y)(i, j, k) at level 0.
+template <typename T, int DIM, bool FAST>
+__device__ __forceinline__ T apply0_at(const PfGeom &g, const StepArgs &a, const MgK<T> &K, const T *F,
+                                       const double *phi, const T *__restrict__ y_, int i, int j, int k) {
+    CoefT<T> c;
+    const T ap = op0<T, DIM, FAST>(g, a, K, F, phi, i, j, k, c);
+    const auto y = gmem(y_);
+    T s = ap * y[pf_soff(g, i, j, k)];
+    s = s + c.ae * y[pf_soff(g, i + 1, j, k)];
+    s = s + c.aw * y[pf_soff(g, i - 1, j, k)];
+    s = s + c.an * y[pf_soff(g, i, j + 1, k)];
+    s = s + c.as * y[pf_soff(g, i, j - 1, k)];
+    if (DIM == 3) {
+        s = s + c.at * y[pf_soff(g, i, j, k + 1)];
+        s = s + c.ab * y[pf_soff(g, i, j, k - 1)];
+    }
+    return s;
+}
+
+// Store v at coarse cell (I,J,K) and at the periodic ghost slots that mirror it.
+template <typename T>
+__device__ __forceinline__ void mg_store(const MgGeom &g, T *__restrict__ e, int I, int J, int K, T v, bool my,
+                                         bool mz) {
+    e[mg_off(g, I, J, K)] = v;
+    if (my) {
+        if (J == 0) e[mg_off(g, I, g.n, K)] = v;
+        if (J == g.n - 1) e[mg_off(g, I, -1, K)] = v;
+    }
+    if (mz && g.dim == 3) {
+        if (K == 0) e[mg_off(g, I, J, g.l)] = v;
+        if (K == g.l - 1) e[mg_off(g, I, J, -1)] = v;
+    }
+}
+
+__device__ __forceinline__ void mg_cell(const MgGeom &g, int64_t q, int &I, int &J, int &K) {
+    I = (int)(q % g.m);
+    const int64_t t = q / g.m;
+    J = (int)(t % g.n);
+    K = (int)(t / g.n);
+}
+
+// ------------------------------------------------------------------------------------
+// Level 0 -> 1: r1 = sum over the kept cells of each aggregate of (rhs - A e); e1 = 0.
+// ------------------------------------------------------------------------------------
+template <typename T, int DIM, bool FAST>
+__global__ __launch_bounds__(256) void k_mg_restrict0(PfGeom g, StepArgs a, MgK<T> K, const T *F,
+                                                      const double *phi, const T *__restrict__ rhs,
+                                                      const T *__restrict__ e, MgGeom c1, T *__restrict__ r1,
+                                                      T *__restrict__ e1) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)c1.m * c1.n * c1.l) return;
+    int I, J, Kc;
+    mg_cell(c1, q, I, J, Kc);
+    T sum = T(0);
+#pragma unroll
+    for (int dz = 0; dz < (DIM == 3 ? 2 : 1); ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int i = 2 * I + dx + 1, j = 2 * J + dy + 1, k = (DIM == 3) ? 2 * Kc + dz + 1 : 0;
+                if (i > g.m || j > g.n || (DIM == 3 && k > g.l)) continue;
+                if (dec0(g, a, i, j, k, phi)) continue;
+                sum = sum + (gmem(rhs)[pf_soff(g, i, j, k)] - apply0_at<T, DIM, FAST>(g, a, K, F, phi, e, i, j, k));
+            }
+    r1[mg_off(c1, I, J, Kc)] = sum;
+    mg_store(c1, e1, I, J, Kc, T(0), c1.per, c1.per);
+}
+
+// Level 1 -> 0: e += alpha * e1(parent) at every kept fine cell, with the periodic ghost
+// copies of even extents (odd extents: the refresh pass follows).
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_mg_prolong0(PfGeom g, StepArgs a, const double *phi, MgGeom c1,
+                                                     const T *__restrict__ e1, T *__restrict__ e, T alpha, int fuse_y,
+                                                     int fuse_z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (DIM == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > g.m || j > g.n) return;
+    if (dec0(g, a, i, j, k, phi)) return;
+    const T ec = e1[mg_off(c1, (i - 1) >> 1, (j - 1) >> 1, (DIM == 3) ? (k - 1) >> 1 : 0)];
+    const int64_t so = pf_soff(g, i, j, k);
+    const T v = e[so] + alpha * ec;
+    e[so] = v;
+    if (fuse_y) {
+        if (j == g.n) e[pf_soff(g, i, 0, k)] = v;
+        if (j == 1) e[pf_soff(g, i, g.n + 1, k)] = v;
+    }
+    if (DIM == 3 && fuse_z) {
+        if (k == g.l) e[pf_soff(g, i, j, 0)] = v;
+        if (k == 1) e[pf_soff(g, i, j, g.l + 1)] = v;
+    }
+}
+
+// out = A y at level 0; MODE 0: partial (w, out); MODE 1: partials (out, w) and (out, out).
+template <typename T, int DIM, bool FAST, int MODE>
+__global__ __launch_bounds__(256) void k_mg_apply0(PfGeom g, StepArgs a, MgK<T> K, const T *F, const double *phi,
+                                                   const T *__restrict__ y, T *__restrict__ out,
+                                                   const T *__restrict__ w, double *__restrict__ part) {
+    const int rows = g.n * (DIM == 3 ? g.l : 1);
+    double d0 = 0., d1 = 0.;
+    for (int row = (int)blockIdx.x * 4 + (int)threadIdx.y; row < rows; row += (int)gridDim.x * 4) {
+        const int j = row % g.n + 1;
+        const int k = (DIM == 3) ? row / g.n + 1 : 0;
+        for (int i = (int)threadIdx.x + 1; i <= g.m; i += 64) {
+            const T Ay = apply0_at<T, DIM, FAST>(g, a, K, F, phi, y, i, j, k);
+            const int64_t so = pf_soff(g, i, j, k);
+            out[so] = Ay;
+            const double wv = (double)w[so];
+            if (MODE == 0) {
+                d0 += wv * (double)Ay;
+            } else {
+                d0 += (double)Ay * wv;
+                d1 += (double)Ay * (double)Ay;
+            }
+        }
+    }
+    __shared__ double red[2][256];
+    const int tid = (int)(threadIdx.x + threadIdx.y * blockDim.x);
+    red[0][tid] = d0;
+    red[1][tid] = d1;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            red[0][tid] += red[0][tid + off];
+            red[1][tid] += red[1][tid + off];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[(int64_t)blockIdx.x * 4 + 0] = red[0][0];
+        part[(int64_t)blockIdx.x * 4 + 1] = red[1][0];
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Coarse levels.
+// ------------------------------------------------------------------------------------
+template <typename T>
+struct MgOp {
+    const T *ap, *cE, *cW, *cN, *cS, *cT, *cB;
+};
+
+template <typename T>
+__device__ __forceinline__ T mg_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, int64_t o) {
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    T s = A.ap[o] * e[o];
+    s = s + A.cE[o] * e[o + 1];
+    s = s + A.cW[o] * e[o - 1];
+    s = s + A.cN[o] * e[o + sy];
+    s = s + A.cS[o] * e[o - sy];
+    if (g.dim == 3) {
+        s = s + A.cT[o] * e[o + sz];
+        s = s + A.cB[o] * e[o - sz];
+    }
+    return s;
+}
+
+// One colour of a red/black Gauss-Seidel sweep (omega = 1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_smooth(MgGeom g, MgOp<T> A, const T *__restrict__ r, T *__restrict__ e,
+                                                   int colour) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)g.m * g.n * g.l) return;
+    int I, J, K;
+    mg_cell(g, q, I, J, K);
+    if (((I + J + K) & 1) != colour) return;
+    const int64_t o = mg_off(g, I, J, K);
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    T s = r[o];
+    s = s - A.cE[o] * e[o + 1];
+    s = s - A.cW[o] * e[o - 1];
+    s = s - A.cN[o] * e[o + sy];
+    s = s - A.cS[o] * e[o - sy];
+    if (g.dim == 3) {
+        s = s - A.cT[o] * e[o + sz];
+        s = s - A.cB[o] * e[o - sz];
+    }
+    mg_store(g, e, I, J, K, s / A.ap[o], g.fuse_y, g.fuse_z);
+}
+
+// Periodic ghosts of odd extents from their source cells.
+template <typename T>
+__global__ void k_mg_refresh(MgGeom g, T *__restrict__ e) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ny = (int64_t)g.m * g.l, nz = (int64_t)g.m * g.n;
+    if (g.ref_y && q < ny) {
+        const int I = (int)(q % g.m), K = (int)(q / g.m);
+        e[mg_off(g, I, -1, K)] = e[mg_off(g, I, g.n - 1, K)];
+        e[mg_off(g, I, g.n, K)] = e[mg_off(g, I, 0, K)];
+    }
+    if (g.ref_z && g.dim == 3 && q < nz) {
+        const int I = (int)(q % g.m), J = (int)(q / g.m);
+        e[mg_off(g, I, J, -1)] = e[mg_off(g, I, J, g.l - 1)];
+        e[mg_off(g, I, J, g.l)] = e[mg_off(g, I, J, 0)];
+    }
+}
+
+// Level f -> c: rc = sum over the kept cells of each aggregate of (r - A e); ec = 0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_restrict(MgGeom f, MgOp<T> A, const uint8_t *__restrict__ keep,
+                                                     const T *__restrict__ r, const T *__restrict__ e, MgGeom c,
+                                                     T *__restrict__ rc, T *__restrict__ ec) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)c.m * c.n * c.l) return;
+    int I, J, K;
+    mg_cell(c, q, I, J, K);
+    T sum = T(0);
+    for (int dz = 0; dz < (f.dim == 3 ? 2 : 1); ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int i = 2 * I + dx, j = 2 * J + dy, k = (f.dim == 3) ? 2 * K + dz : 0;
+                if (i >= f.m || j >= f.n || k >= f.l) continue;
+                const int64_t o = mg_off(f, i, j, k);
+                if (!keep[o]) continue;
+                sum = sum + (r[o] - mg_ax(f, A, e, o));
+            }
+    rc[mg_off(c, I, J, K)] = sum;
+    mg_store(c, ec, I, J, K, T(0), c.per, c.per);
+}
+
+// Level c -> f: e += alpha * ec(parent) at every kept cell of f.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong(MgGeom f, const uint8_t *__restrict__ keep, MgGeom c,
+                                                    const T *__restrict__ ec, T *__restrict__ e, T alpha) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)f.m * f.n * f.l) return;
+    int I, J, K;
+    mg_cell(f, q, I, J, K);
+    const int64_t o = mg_off(f, I, J, K);
+    if (!keep[o]) return;
+    const T v = e[o] + alpha * ec[mg_off(c, I >> 1, J >> 1, (f.dim == 3) ? K >> 1 : 0)];
+    mg_store(f, e, I, J, K, v, f.fuse_y, f.fuse_z);
+}
+
+// The coarsest level: `sweeps` red/black Gauss-Seidel sweeps from e = 0 in LDS, one block
+// of 256 threads, each owning at most MG_CPT cells (their r and couplings in registers).
+constexpr int MG_CPT = 4;
+constexpr int MG_LDS = 6144;   // ghosted cells of the coarsest level (checked on the host)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const T *__restrict__ r,
+                                                     T *__restrict__ e, int sweeps) {
+    __shared__ T se[MG_LDS];
+    const int tid = (int)threadIdx.x;
+    const int cells = g.m * g.n * g.l;
+    for (int q = tid; q < (int)g.N; q += 256) se[q] = T(0);
+    int64_t o[MG_CPT];
+    int col[MG_CPT], I[MG_CPT], J[MG_CPT], K[MG_CPT];
+    T rr[MG_CPT], ap[MG_CPT], cE[MG_CPT], cW[MG_CPT], cN[MG_CPT], cS[MG_CPT], cT[MG_CPT], cB[MG_CPT];
+#pragma unroll
+    for (int u = 0; u < MG_CPT; ++u) {
+        const int q = tid + 256 * u;
+        col[u] = -1;
+        if (q < cells) {
+            mg_cell(g, q, I[u], J[u], K[u]);
+            o[u] = mg_off(g, I[u], J[u], K[u]);
+            col[u] = (I[u] + J[u] + K[u]) & 1;
+            rr[u] = r[o[u]];
+            ap[u] = A.ap[o[u]];
+            cE[u] = A.cE[o[u]];
+            cW[u] = A.cW[o[u]];
+            cN[u] = A.cN[o[u]];
+            cS[u] = A.cS[o[u]];
+            cT[u] = (g.dim == 3) ? A.cT[o[u]] : T(0);
+            cB[u] = (g.dim == 3) ? A.cB[o[u]] : T(0);
+        }
+    }
+    __syncthreads();
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+            for (int u = 0; u < MG_CPT; ++u) {
+                if (col[u] != c) continue;
+                const int64_t x = o[u];
+                T s = rr[u];
+                s = s - cE[u] * se[x + 1];
+                s = s - cW[u] * se[x - 1];
+                s = s - cN[u] * se[x + sy];
+                s = s - cS[u] * se[x - sy];
+                if (g.dim == 3) {
+                    s = s - cT[u] * se[x + sz];
+                    s = s - cB[u] * se[x - sz];
+                }
+                const T v = s / ap[u];
+                se[x] = v;
+                if (g.fuse_y) {
+                    if (J[u] == 0) se[mg_off(g, I[u], g.n, K[u])] = v;
+                    if (J[u] == g.n - 1) se[mg_off(g, I[u], -1, K[u])] = v;
+                }
+                if (g.fuse_z && g.dim == 3) {
+                    if (K[u] == 0) se[mg_off(g, I[u], J[u], g.l)] = v;
+                    if (K[u] == g.l - 1) se[mg_off(g, I[u], J[u], -1)] = v;
+                }
+            }
+            __syncthreads();
+            if (g.ref_y || g.ref_z) {
+                for (int q = tid; q < g.m * (g.l > g.n ? g.l : g.n); q += 256) {
+                    const int a0 = q % g.m, b0 = q / g.m;
+                    if (g.ref_y && b0 < g.l) {
+                        se[mg_off(g, a0, -1, b0)] = se[mg_off(g, a0, g.n - 1, b0)];
+                        se[mg_off(g, a0, g.n, b0)] = se[mg_off(g, a0, 0, b0)];
+                    }
+                    if (g.ref_z && g.dim == 3 && b0 < g.n) {
+                        se[mg_off(g, a0, b0, -1)] = se[mg_off(g, a0, b0, g.l - 1)];
+                        se[mg_off(g, a0, b0, g.l)] = se[mg_off(g, a0, b0, 0)];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int q = tid; q < (int)g.N; q += 256) e[q] = se[q];
+}
+
+// ------------------------------------------------------------------------------------
+// Set-up: Galerkin coarse operators.  For each kept child of an aggregate its diagonal and
+// its couplings to kept cells of the same aggregate go to the aggregate's diagonal, and its
+// couplings to kept cells of the neighbouring aggregate in a direction go to that
+// direction's coupling; couplings to rows left out of the coarse space are dropped.
+// ------------------------------------------------------------------------------------
+template <typename T>
+struct MgOut {
+    T *ap, *cE, *cW, *cN, *cS, *cT, *cB;
+    uint8_t *keep;
+};
+
+template <typename T>
+__device__ __forceinline__ void mg_write_row(const MgGeom &c, const MgOut<T> &out, int64_t o, bool any, T sap, T sE,
+                                             T sW, T sN, T sS, T sT, T sB) {
+    if (!any) {   // an aggregate of left-out rows only: a unit row with zero right-hand side
+        out.ap[o] = T(1);
+        out.cE[o] = out.cW[o] = out.cN[o] = out.cS[o] = out.cT[o] = out.cB[o] = T(0);
+        out.keep[o] = 0;
+        return;
+    }
+    const bool coupled = sE != T(0) || sW != T(0) || sN != T(0) || sS != T(0) || sT != T(0) || sB != T(0);
+    out.ap[o] = (sap != T(0)) ? sap : T(1);
+    out.cE[o] = sE;
+    out.cW[o] = sW;
+    out.cN[o] = sN;
+    out.cS[o] = sS;
+    out.cT[o] = sT;
+    out.cB[o] = sB;
+    out.keep[o] = coupled ? 1 : 0;
+}
+
+template <typename T, int DIM, bool FAST>
+__global__ __launch_bounds__(256) void k_mg_gal0(PfGeom g, StepArgs a, MgK<T> K, const T *F, const double *phi,
+                                                 MgGeom c, MgOut<T> out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)c.m * c.n * c.l) return;
+    int I, J, Kc;
+    mg_cell(c, q, I, J, Kc);
+    const bool per = (a.bc == PF_BC_UNIFORM);
+    bool any = false;
+    T sap = T(0), sE = T(0), sW = T(0), sN = T(0), sS = T(0), sT = T(0), sB = T(0);
+    for (int dz = 0; dz < (DIM == 3 ? 2 : 1); ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int i = 2 * I + dx + 1, j = 2 * J + dy + 1, k = (DIM == 3) ? 2 * Kc + dz + 1 : 0;
+                if (i > g.m || j > g.n || (DIM == 3 && k > g.l)) continue;
+                if (dec0(g, a, i, j, k, phi)) continue;
+                any = true;
+                CoefT<T> cc;
+                sap = sap + op0<T, DIM, FAST>(g, a, K, F, phi, i, j, k, cc);
+                const T vals[6] = {cc.ae, cc.aw, cc.an, cc.as, cc.at, cc.ab};
+                const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+                for (int d = 0; d < (DIM == 3 ? 6 : 4); ++d) {
+                    const T v = vals[d];
+                    if (v == T(0)) continue;
+                    const int ni = i + di[d];
+                    int nj = j + dj[d], nk = k + dk[d];
+                    if (per) {
+                        if (nj == 0) nj = g.n; else if (nj == g.n + 1) nj = 1;
+                        if (DIM == 3) { if (nk == 0) nk = g.l; else if (nk == g.l + 1) nk = 1; }
+                    }
+                    if (ni < 1 || ni > g.m || nj < 1 || nj > g.n || (DIM == 3 && (nk < 1 || nk > g.l))) continue;
+                    if (dec0(g, a, ni, nj, nk, phi)) continue;
+                    const bool same = ((ni - 1) >> 1) == I && ((nj - 1) >> 1) == J &&
+                                      (DIM != 3 || ((nk - 1) >> 1) == Kc);
+                    if (same) sap = sap + v;
+                    else if (d == 0) sE = sE + v;
+                    else if (d == 1) sW = sW + v;
+                    else if (d == 2) sN = sN + v;
+                    else if (d == 3) sS = sS + v;
+                    else if (d == 4) sT = sT + v;
+                    else sB = sB + v;
+                }
+            }
+    mg_write_row(c, out, mg_off(c, I, J, Kc), any, sap, sE, sW, sN, sS, sT, sB);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_gal(MgGeom f, MgOp<T> A, const uint8_t *__restrict__ keep, MgGeom c,
+                                                MgOut<T> out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)c.m * c.n * c.l) return;
+    int I, J, K;
+    mg_cell(c, q, I, J, K);
+    bool any = false;
+    T sap = T(0), sE = T(0), sW = T(0), sN = T(0), sS = T(0), sT = T(0), sB = T(0);
+    for (int dz = 0; dz < (f.dim == 3 ? 2 : 1); ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int i = 2 * I + dx, j = 2 * J + dy, k = (f.dim == 3) ? 2 * K + dz : 0;
+                if (i >= f.m || j >= f.n || k >= f.l) continue;
+                const int64_t o = mg_off(f, i, j, k);
+                if (!keep[o]) continue;
+                any = true;
+                sap = sap + A.ap[o];
+                const T vals[6] = {A.cE[o], A.cW[o], A.cN[o], A.cS[o], A.cT[o], A.cB[o]};
+                const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, 1, -1};
+                for (int d = 0; d < (f.dim == 3 ? 6 : 4); ++d) {
+                    const T v = vals[d];
+                    if (v == T(0)) continue;
+                    const int ni = i + di[d];
+                    int nj = j + dj[d], nk = k + dk[d];
+                    if (f.per) {
+                        if (nj < 0) nj = f.n - 1; else if (nj >= f.n) nj = 0;
+                        if (f.dim == 3) { if (nk < 0) nk = f.l - 1; else if (nk >= f.l) nk = 0; }
+                    }
+                    if (ni < 0 || ni >= f.m || nj < 0 || nj >= f.n || nk < 0 || nk >= f.l) continue;
+                    if (!keep[mg_off(f, ni, nj, nk)]) continue;
+                    const bool same = (ni >> 1) == I && (nj >> 1) == J && (f.dim != 3 || (nk >> 1) == K);
+                    if (same) sap = sap + v;
+                    else if (d == 0) sE = sE + v;
+                    else if (d == 1) sW = sW + v;
+                    else if (d == 2) sN = sN + v;
+                    else if (d == 3) sS = sS + v;
+                    else if (d == 4) sT = sT + v;
+                    else sB = sB + v;
+                }
+            }
+    mg_write_row(c, out, mg_off(c, I, J, K), any, sap, sE, sW, sN, sS, sT, sB);
+}
+
+// ------------------------------------------------------------------------------------
+// BiCGStab vector kernels over every slot of the colour-split arrays (ghost and padding
+// slots of r, rh, p, v, s, t are zero from allocation on and stay zero, so sums over all
+// slots are sums over the interior cells), and the scalar updates.
+// Scalars: S[0] rho, S[1] alpha, S[2] omega, S[3] beta, S[4] |b|^2, S[5] |r|^2, S[6] stop.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void block_sum2(double a0, double a1, double *part) {
+    __shared__ double red[2][256];
+    const int tid = (int)threadIdx.x;
+    red[0][tid] = a0;
+    red[1][tid] = a1;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            red[0][tid] += red[0][tid + off];
+            red[1][tid] += red[1][tid + off];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[(int64_t)blockIdx.x * 4 + 0] = red[0][0];
+        part[(int64_t)blockIdx.x * 4 + 1] = red[1][0];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_init(int64_t n, const T *__restrict__ r, T *__restrict__ rh,
+                                                   T *__restrict__ p, T *__restrict__ v, T *__restrict__ x,
+                                                   double *part) {
+    double d = 0.;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        const T b = r[q];
+        rh[q] = b;
+        p[q] = T(0);
+        v[q] = T(0);
+        x[q] = T(0);
+        d += (double)b * (double)b;
+    }
+    block_sum2(d, 0., part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_p(int64_t n, const T *__restrict__ r, T *__restrict__ p,
+                                                const T *__restrict__ v, const double *S) {
+    const T beta = (T)S[3], omega = (T)S[2];
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256)
+        p[q] = r[q] + beta * (p[q] - omega * v[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_s(int64_t n, const T *__restrict__ r, const T *__restrict__ v,
+                                                T *__restrict__ s, const double *S) {
+    const T alpha = (T)S[1];
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256)
+        s[q] = r[q] - alpha * v[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, T *__restrict__ x, const T *__restrict__ y,
+                                                 const T *__restrict__ z, T *__restrict__ r, const T *__restrict__ s,
+                                                 const T *__restrict__ t, const T *__restrict__ rh, const double *S,
+                                                 double *part) {
+    const T alpha = (T)S[1], omega = (T)S[2];
+    double d0 = 0., d1 = 0.;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        x[q] = x[q] + alpha * y[q] + omega * z[q];
+        const T rn = s[q] - omega * t[q];
+        r[q] = rn;
+        d0 += (double)rh[q] * (double)rn;
+        d1 += (double)rn * (double)rn;
+    }
+    block_sum2(d0, d1, part);
+}
+
+__device__ __forceinline__ void sum_parts(const double *part, int nb, double &a0, double &a1) {
+    __shared__ double red[2][256];
+    const int tid = (int)threadIdx.x;
+    double s0 = 0., s1 = 0.;
+    for (int b = tid; b < nb; b += 256) {
+        s0 += part[(int64_t)b * 4 + 0];
+        s1 += part[(int64_t)b * 4 + 1];
+    }
+    red[0][tid] = s0;
+    red[1][tid] = s1;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            red[0][tid] += red[0][tid + off];
+            red[1][tid] += red[1][tid + off];
+        }
+        __syncthreads();
+    }
+    a0 = red[0][0];
+    a1 = red[1][0];
+}
+
+__device__ __forceinline__ bool finite_nonzero(double x) { return x != 0. && fabs(x) <= 1.7976931348623157e308; }
+
+// stage 0: after k_bicg_init; 1: alpha after A y; 2: omega after A z; 3: beta after x, r.
+__global__ __launch_bounds__(256) void k_bicg_scalar(const double *part, int nb, double *S, int stage) {
+    double a0, a1;
+    sum_parts(part, nb, a0, a1);
+    if (threadIdx.x != 0) return;
+    if (stage == 0) {
+        S[0] = a0;
+        S[4] = a0;
+        S[5] = a0;
+        S[1] = 1.;
+        S[2] = 1.;
+        S[3] = 0.;
+        S[6] = (a0 == 0. || !(a0 <= 1.7976931348623157e308)) ? 1. : 0.;
+    } else if (stage == 1) {
+        if (finite_nonzero(a0)) {
+            S[1] = S[0] / a0;
+        } else {
+            S[1] = 0.;
+            S[6] = 1.;
+        }
+    } else if (stage == 2) {
+        S[2] = (finite_nonzero(a1) && a0 <= 1.7976931348623157e308 && a0 >= -1.7976931348623157e308) ? a0 / a1 : 0.;
+    } else {
+        const double rho = S[0], omega = S[2];
+        if (!finite_nonzero(omega) || !finite_nonzero(rho) || !(fabs(a0) <= 1.7976931348623157e308)) {
+            S[3] = 0.;
+            S[6] = 1.;
+        } else {
+            S[3] = (a0 / rho) * (S[1] / omega);
+        }
+        S[0] = a0;
+        S[5] = a1;
+    }
+}
+
+__global__ void k_mg_zero_t(float *p, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) p[q] = 0.f;
+}
+__global__ void k_mg_zero_t(double *p, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) p[q] = 0.;
+}
+
+// ------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+static inline unsigned flat_blocks(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < MG_NPART ? (b < 1 ? 1 : b) : MG_NPART);
+}
+
+static std::vector<MgGeom> mg_levels(const PfGeom &g, int bc) {
+    const int per = (bc == PF_BC_UNIFORM);
+    std::vector<MgGeom> L;
+    L.push_back(mg_make_geom(g.dim, g.m, g.n, g.l, per));
+    for (;;) {
+        const MgGeom &f = L.back();
+        const int64_t cells = (int64_t)f.m * f.n * f.l;
+        if (L.size() > 1 && cells <= MG_COARSEST_CELLS) break;
+        const int mc = mg_coarse_extent(f.m), nc = mg_coarse_extent(f.n);
+        const int lc = (g.dim == 3) ? mg_coarse_extent(f.l) : 1;
+        if (mc == f.m && nc == f.n && lc == f.l) break;
+        if ((int)L.size() == MG_MAX_LEVELS) break;
+        L.push_back(mg_make_geom(g.dim, mc, nc, lc, per));
+    }
+    return L;
+}
+
+static inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+static int64_t mg_bytes_t(const PfGeom &g, int bc, int64_t tsz) {
+    const std::vector<MgGeom> L = mg_levels(g, bc);
+    int64_t b = 9 * align256(2 * g.HS * tsz);
+    for (size_t q = 1; q < L.size(); ++q) b += 9 * align256(L[q].N * tsz) + align256(L[q].N);
+    b += align256(16 * 8) + align256((int64_t)MG_NPART * 4 * 8);
+    return b;
+}
+
+int64_t mg_bytes(const PfGeom &g, int bc, bool fp32) { return mg_bytes_t(g, bc, fp32 ? 4 : 8); }
+
+template <typename T>
+bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why) {
+    S.c = c;
+    const PfGeom &g = c.g;
+    const std::vector<MgGeom> L = mg_levels(g, c.a.bc);
+    if (L.size() > 1 && L.back().N > MG_LDS) {
+        if (why) *why = "multigrid: coarsest level too large for the LDS solve";
+        return false;
+    }
+    if (L.size() > 1 && (int64_t)L.back().m * L.back().n * L.back().l > 256 * MG_CPT) {
+        if (why) *why = "multigrid: coarsest level has more cells than one block holds";
+        return false;
+    }
+    S.bytes = mg_bytes_t(g, c.a.bc, sizeof(T));
+    if (hipMalloc(&S.mem, (size_t)S.bytes) != hipSuccess) {
+        S.mem = nullptr;
+        if (why) *why = "multigrid: hipMalloc of " + std::to_string(S.bytes) + " bytes failed";
+        return false;
+    }
+    if (hipHostMalloc((void **)&S.h_scal, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        S.h_scal = nullptr;
+        (void)hipFree(S.mem);
+        S.mem = nullptr;
+        if (why) *why = "multigrid: hipHostMalloc failed";
+        return false;
+    }
+    char *p = (char *)S.mem;
+    auto take = [&](int64_t bytes) {
+        char *q = p;
+        p += align256(bytes);
+        return (void *)q;
+    };
+    const int64_t n0 = 2 * g.HS * (int64_t)sizeof(T);
+    T **vecs[9] = {&S.x, &S.r, &S.rh, &S.p, &S.v, &S.s, &S.t, &S.y, &S.z};
+    for (T **v : vecs) *v = (T *)take(n0);
+    S.lev.assign(L.size(), MgLevel<T>{});
+    S.lev[0].g = L[0];
+    for (size_t q = 1; q < L.size(); ++q) {
+        MgLevel<T> &lv = S.lev[q];
+        lv.g = L[q];
+        const int64_t nb = L[q].N * (int64_t)sizeof(T);
+        T **arr[9] = {&lv.e, &lv.r, &lv.ap, &lv.cE, &lv.cW, &lv.cN, &lv.cS, &lv.cT, &lv.cB};
+        for (T **a : arr) *a = (T *)take(nb);
+        lv.keep = (uint8_t *)take(L[q].N);
+    }
+    S.scal = (double *)take(16 * 8);
+    S.part = (double *)take((int64_t)MG_NPART * 4 * 8);
+    if (hipMemset(S.mem, 0, (size_t)S.bytes) != hipSuccess) {
+        mg_destroy(S);
+        if (why) *why = "multigrid: hipMemset failed";
+        return false;
+    }
+    S.built = false;
+    return true;
+}
+
+template <typename T>
+void mg_destroy(MgSolver<T> &S) {
+    if (S.mem) (void)hipFree(S.mem);
+    if (S.h_scal) (void)hipHostFree(S.h_scal);
+    S.mem = nullptr;
+    S.h_scal = nullptr;
+    S.lev.clear();
+    S.built = false;
+}
+
+template <typename T>
+static MgK<T> mg_k(const StepArgs &a) {
+    MgK<T> K;
+    K.cdx = (T)(float)(a.dt / (a.dx * a.dx));
+    K.cdy = (T)(float)(a.dt / (a.dy * a.dy));
+    K.cdz = (T)(float)(a.dt / (a.dz * a.dz));
+    return K;
+}
+
+template <typename T>
+static const T *fine_phi(const MgSolver<T> &S);
+template <>
+const float *fine_phi<float>(const MgSolver<float> &S) { return S.c.PHI32; }
+template <>
+const double *fine_phi<double>(const MgSolver<double> &S) { return S.c.PHIs; }
+
+template <typename T>
+static MgOp<T> op_of(const MgLevel<T> &L) {
+    return MgOp<T>{L.ap, L.cE, L.cW, L.cN, L.cS, L.cT, L.cB};
+}
+
+template <typename T>
+void mg_setup(MgSolver<T> &S, hipStream_t st) {
+    const PfGeom &g = S.c.g;
+    const MgK<T> K = mg_k<T>(S.c.a);
+    const T *F = fine_phi(S);
+    for (size_t q = 1; q < S.lev.size(); ++q) {
+        MgLevel<T> &c = S.lev[q];
+        const MgOut<T> out{c.ap, c.cE, c.cW, c.cN, c.cS, c.cT, c.cB, c.keep};
+        const unsigned nb = blocks_for((int64_t)c.g.m * c.g.n * c.g.l);
+        if (q == 1) {
+#define PF_GAL0(D, F_) hipLaunchKernelGGL((k_mg_gal0<T, D, F_>), dim3(nb), dim3(256), 0, st, g, S.c.a, K, F, S.c.phi, c.g, out)
+            if (g.dim == 3) { if (S.c.fast) PF_GAL0(3, true); else PF_GAL0(3, false); }
+            else { if (S.c.fast) PF_GAL0(2, true); else PF_GAL0(2, false); }
+#undef PF_GAL0
+        } else {
+            const MgLevel<T> &f = S.lev[q - 1];
+            hipLaunchKernelGGL((k_mg_gal<T>), dim3(nb), dim3(256), 0, st, f.g, op_of(f), (const uint8_t *)f.keep, c.g,
+                               out);
+        }
+    }
+    S.built = true;
+}
+
+// One red/black sweep colour of level 0 with omega = 1 (the solve's own colour-sweep kernels).
+static void sweep0(const MgSolver<float> &S, hipStream_t st, const StepArgs &a1, int colour, float *e,
+                   const float *rhs) {
+    const PfGeom &g = S.c.g;
+    const MgGeom &l0 = S.lev[0].g;
+    SweepArgs sw{};
+    sw.colour = colour;
+    sw.fuse_y = l0.fuse_y;
+    sw.fuse_z = l0.fuse_z;
+    sw.track_err = 0;
+    sw.xcd_remap = S.c.remap32;
+    sw.kb = S.c.kb32;
+    sw.xw = S.c.xw;
+    launch_sor32(st, g, a1, sw, e, S.c.PHI32, rhs, S.c.phi, nullptr);
+    if (l0.ref_y || l0.ref_z) launch_ghost_refresh32(st, g, l0.ref_y, l0.ref_z, e);
+}
+static void sweep0(const MgSolver<double> &S, hipStream_t st, const StepArgs &a1, int colour, double *e,
+                   const double *rhs) {
+    const PfGeom &g = S.c.g;
+    const MgGeom &l0 = S.lev[0].g;
+    SweepArgs sw{};
+    sw.colour = colour;
+    sw.fuse_y = l0.fuse_y;
+    sw.fuse_z = l0.fuse_z;
+    sw.track_err = 0;
+    sw.xcd_remap = S.c.remap64;
+    sw.kb = S.c.kb64;
+    sw.xw = S.c.xw;
+    launch_sor_phi(st, g, a1, sw, e, S.c.PHIs, rhs, S.c.phi, nullptr, S.c.fast);
+    if (l0.ref_y || l0.ref_z) launch_ghost_refresh(st, g, l0.ref_y, l0.ref_z, e);
+}
+static void refresh0(const MgSolver<float> &S, hipStream_t st, float *e) {
+    const MgGeom &l0 = S.lev[0].g;
+    if (l0.ref_y || l0.ref_z) launch_ghost_refresh32(st, S.c.g, l0.ref_y, l0.ref_z, e);
+}
+static void refresh0(const MgSolver<double> &S, hipStream_t st, double *e) {
+    const MgGeom &l0 = S.lev[0].g;
+    if (l0.ref_y || l0.ref_z) launch_ghost_refresh(st, S.c.g, l0.ref_y, l0.ref_z, e);
+}
+
+template <typename T>
+static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
+    MgLevel<T> &L = S.lev[q];
+    const MgGeom &g = L.g;
+    const int64_t cells = (int64_t)g.m * g.n * g.l;
+    if (q + 1 == S.lev.size()) {
+        hipLaunchKernelGGL((k_mg_coarsest<T>), dim3(1), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e,
+                           S.prm.coarsest_sweeps);
+        return;
+    }
+    const unsigned nb = blocks_for(cells);
+    const unsigned nbr = blocks_for((int64_t)g.m * (g.l > g.n ? g.l : g.n));
+    auto smooth = [&](int colour) {
+        hipLaunchKernelGGL((k_mg_smooth<T>), dim3(nb), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e, colour);
+        if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
+    };
+    for (int s = 0; s < S.prm.nu1; ++s) {
+        smooth(0);
+        smooth(1);
+    }
+    MgLevel<T> &C = S.lev[q + 1];
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks_for((int64_t)C.g.m * C.g.n * C.g.l)), dim3(256), 0, st, g,
+                       op_of(L), (const uint8_t *)L.keep, (const T *)L.r, (const T *)L.e, C.g, C.r, C.e);
+    coarse_cycle(S, st, q + 1);
+    hipLaunchKernelGGL((k_mg_prolong<T>), dim3(nb), dim3(256), 0, st, g, (const uint8_t *)L.keep, C.g,
+                       (const T *)C.e, L.e, (T)S.prm.alpha);
+    if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
+    for (int s = 0; s < S.prm.nu2; ++s) {
+        smooth(1);
+        smooth(0);
+    }
+}
+
+// e = M rhs: one V-cycle from e = 0.
+template <typename T>
+static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
+    const PfGeom &g = S.c.g;
+    StepArgs a1 = S.c.a;
+    a1.omega = 1.0;
+    a1.omega_c = 0.0;
+    hipLaunchKernelGGL(k_mg_zero_t, dim3(flat_blocks(2 * g.HS)), dim3(256), 0, st, e, 2 * g.HS);
+    for (int s = 0; s < S.prm.nu1; ++s) {
+        sweep0(S, st, a1, 0, e, rhs);
+        sweep0(S, st, a1, 1, e, rhs);
+    }
+    if (S.lev.size() > 1) {
+        MgLevel<T> &C = S.lev[1];
+        const MgK<T> K = mg_k<T>(S.c.a);
+        const T *F = fine_phi(S);
+        const unsigned nb = blocks_for((int64_t)C.g.m * C.g.n * C.g.l);
+#define PF_R0(D, F_) hipLaunchKernelGGL((k_mg_restrict0<T, D, F_>), dim3(nb), dim3(256), 0, st, g, S.c.a, K, F, S.c.phi, \
+                                        rhs, (const T *)e, C.g, C.r, C.e)
+        if (g.dim == 3) { if (S.c.fast) PF_R0(3, true); else PF_R0(3, false); }
+        else { if (S.c.fast) PF_R0(2, true); else PF_R0(2, false); }
+#undef PF_R0
+        coarse_cycle(S, st, 1);
+        const MgGeom &l0 = S.lev[0].g;
+        dim3 b(64, 4, 1);
+        dim3 gr((unsigned)((g.m + 63) / 64), (unsigned)((g.n + 3) / 4), (unsigned)(g.dim == 3 ? g.l : 1));
+        if (g.dim == 3)
+            hipLaunchKernelGGL((k_mg_prolong0<T, 3>), gr, b, 0, st, g, S.c.a, S.c.phi, C.g, (const T *)C.e, e,
+                               (T)S.prm.alpha, l0.fuse_y, l0.fuse_z);
+        else
+            hipLaunchKernelGGL((k_mg_prolong0<T, 2>), gr, b, 0, st, g, S.c.a, S.c.phi, C.g, (const T *)C.e, e,
+                               (T)S.prm.alpha, l0.fuse_y, l0.fuse_z);
+        refresh0(S, st, e);
+    }
+    for (int s = 0; s < S.prm.nu2; ++s) {
+        sweep0(S, st, a1, 1, e, rhs);
+        sweep0(S, st, a1, 0, e, rhs);
+    }
+}
+
+template <typename T>
+static unsigned apply_blocks(const PfGeom &g) {
+    const int rows = g.n * (g.dim == 3 ? g.l : 1);
+    const int b = (rows + 3) / 4;
+    return (unsigned)(b < MG_NPART ? b : MG_NPART);
+}
+
+template <typename T, int MODE>
+static void apply0(MgSolver<T> &S, hipStream_t st, const T *y, T *out, const T *w) {
+    const PfGeom &g = S.c.g;
+    const MgK<T> K = mg_k<T>(S.c.a);
+    const T *F = fine_phi(S);
+    const dim3 b(64, 4, 1), gr(apply_blocks<T>(g));
+#define PF_AP(D, F_) hipLaunchKernelGGL((k_mg_apply0<T, D, F_, MODE>), gr, b, 0, st, g, S.c.a, K, F, S.c.phi, y, out, \
+                                        w, S.part)
+    if (g.dim == 3) { if (S.c.fast) PF_AP(3, true); else PF_AP(3, false); }
+    else { if (S.c.fast) PF_AP(2, true); else PF_AP(2, false); }
+#undef PF_AP
+}
+
+template <typename T>
+hipError_t mg_solve(MgSolver<T> &S, hipStream_t st, double rtol, int maxit, int *applications, double *relres) {
+    *applications = 0;
+    *relres = 0.;
+    if (!S.built) mg_setup(S, st);
+    const PfGeom &g = S.c.g;
+    const int64_t n = 2 * g.HS;
+    const unsigned fb = flat_blocks(n);
+    const int nap = (int)apply_blocks<T>(g);
+    hipLaunchKernelGGL((k_bicg_init<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.rh, S.p, S.v, S.x, S.part);
+    hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, (int)fb, S.scal, 0);
+    hipError_t e = hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const double b2 = S.h_scal[4];
+    if (S.h_scal[6] != 0. || !(b2 > 0.)) {
+        *relres = (b2 == 0.) ? 0. : NAN;
+        return hipGetLastError();
+    }
+    for (int it = 0; it < maxit; ++it) {
+        hipLaunchKernelGGL((k_bicg_p<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.p, (const T *)S.v,
+                           (const double *)S.scal);
+        vcycle(S, st, (const T *)S.p, S.y);
+        apply0<T, 0>(S, st, S.y, S.v, S.rh);
+        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 1);
+        hipLaunchKernelGGL((k_bicg_s<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, (const T *)S.v, S.s,
+                           (const double *)S.scal);
+        vcycle(S, st, (const T *)S.s, S.z);
+        apply0<T, 1>(S, st, S.z, S.t, S.s);
+        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 2);
+        hipLaunchKernelGGL((k_bicg_xr<T>), dim3(fb), dim3(256), 0, st, n, S.x, (const T *)S.y, (const T *)S.z, S.r,
+                           (const T *)S.s, (const T *)S.t, (const T *)S.rh, (const double *)S.scal, S.part);
+        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, (int)fb, S.scal, 3);
+        *applications += 2;
+        e = hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        const double rel = std::sqrt(S.h_scal[5] / b2);
+        *relres = rel;
+        if (!(rel > rtol) || S.h_scal[6] != 0.) break;   // converged, breakdown, or non-finite
+    }
+    return hipGetLastError();
+}
+
+template bool mg_create<float>(MgSolver<float> &, const MgCase &, std::string *);
+template bool mg_create<double>(MgSolver<double> &, const MgCase &, std::string *);
+template void mg_destroy<float>(MgSolver<float> &);
+template void mg_destroy<double>(MgSolver<double> &);
+template void mg_setup<float>(MgSolver<float> &, hipStream_t);
+template void mg_setup<double>(MgSolver<double> &, hipStream_t);
+template hipError_t mg_solve<float>(MgSolver<float> &, hipStream_t, double, int, int *, double *);
+template hipError_t mg_solve<double>(MgSolver<double> &, hipStream_t, double, int, int *, double *);
+
+}  // namespace pf

@@ -25,14 +25,14 @@ namespace pf {
 // wave max of |r / ap| into the wave's slot.
 // ------------------------------------------------------------------------------------
 
-template <int DIM, bool FAST>
+template <int DIM, bool FAST, typename TR>
 __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ P,
                                                   const double *__restrict__ PHIs, const double *__restrict__ BB,
-                                                  const double *__restrict__ phi, float *__restrict__ R,
-                                                  float *__restrict__ E, unsigned long long *slots) {
+                                                  const double *__restrict__ phi, TR *__restrict__ R,
+                                                  TR *__restrict__ E, unsigned long long *slots) {
     const CellIdx q = sweep_cell<DIM>(g, s);
     double res = 0.;
-    if (q.active) res = residual_cell<DIM, FAST>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E);
+    if (q.active) res = residual_cell<DIM, FAST, TR>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E);
     if (s.track_err) wave_max_to_slot(res, slots);
 }
 
@@ -75,9 +75,24 @@ __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s
 
 // p += e over every slot of both colours: interior cells and the periodic ghost slots
 // (which hold e of their source cell, so the ghost stays equal to its source).
-__global__ void k_update(const float *__restrict__ E, double *__restrict__ P, int64_t n) {
+template <typename TE>
+__global__ void k_update(const TE *__restrict__ E, double *__restrict__ P, int64_t n) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
         P[t] = P[t] + (double)E[t];
+}
+
+// p += e as k_update, with the max |e| of the update (NaN kept) folded into the slots of
+// the block's waves: the step's diagnostic when a solve reports no sweep error.
+template <typename TE>
+__global__ __launch_bounds__(256) void k_update_track(const TE *__restrict__ E, double *__restrict__ P, int64_t n,
+                                                      unsigned long long *slots) {
+    double mx = 0.;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const double e = (double)E[t];
+        P[t] = P[t] + e;
+        mx = max_keep_nan(mx, fabs(e));
+    }
+    wave_max_to_slot(mx, slots);
 }
 
 __global__ void k_ghost_refresh32(PfGeom g, int do_y, int do_z, float *__restrict__ E) {
@@ -113,15 +128,26 @@ static inline dim3 sweep_grid(const PfGeom &g, SweepArgs &s, dim3 b) {
     return dim3(gx * gy * gz);
 }
 
-void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
-                     const double *PHIs, const double *BB, const double *phi, float *R, float *E,
-                     unsigned long long *slots, bool fast) {
+template <typename TR>
+static void residual_any(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
+                         const double *PHIs, const double *BB, const double *phi, TR *R, TR *E,
+                         unsigned long long *slots, bool fast) {
     dim3 b(64, 4, 1);
     dim3 gr = sweep_grid(g, s, b);
-#define PF_RES(D, F) hipLaunchKernelGGL((k_residual<D, F>), gr, b, 0, st, g, a, s, P, PHIs, BB, phi, R, E, slots)
+#define PF_RES(D, F) hipLaunchKernelGGL((k_residual<D, F, TR>), gr, b, 0, st, g, a, s, P, PHIs, BB, phi, R, E, slots)
     if (g.dim == 3) { if (fast) PF_RES(3, true); else PF_RES(3, false); }
     else { if (fast) PF_RES(2, true); else PF_RES(2, false); }
 #undef PF_RES
+}
+void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
+                     const double *PHIs, const double *BB, const double *phi, float *R, float *E,
+                     unsigned long long *slots, bool fast) {
+    residual_any<float>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast);
+}
+void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
+                     const double *PHIs, const double *BB, const double *phi, double *R, double *E,
+                     unsigned long long *slots, bool fast) {
+    residual_any<double>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast);
 }
 
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
@@ -146,7 +172,26 @@ void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs 
 }
 
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P) {
-    hipLaunchKernelGGL(k_update, dim3(256 * 8), dim3(256), 0, st, E, P, 2 * g.HS);
+    hipLaunchKernelGGL(k_update<float>, dim3(256 * 8), dim3(256), 0, st, E, P, 2 * g.HS);
+}
+void launch_update(hipStream_t st, const PfGeom &g, const double *E, double *P) {
+    hipLaunchKernelGGL(k_update<double>, dim3(256 * 8), dim3(256), 0, st, E, P, 2 * g.HS);
+}
+// nslots: the slots available (four per block are used)
+template <typename TE>
+static void update_track_any(hipStream_t st, const PfGeom &g, const TE *E, double *P, unsigned long long *slots,
+                             int nslots) {
+    int nb = nslots / 4;
+    nb = nb < 1 ? 1 : (nb > 2048 ? 2048 : nb);
+    hipLaunchKernelGGL(k_update_track<TE>, dim3((unsigned)nb), dim3(256), 0, st, E, P, 2 * g.HS, slots);
+}
+void launch_update_track(hipStream_t st, const PfGeom &g, const float *E, double *P, unsigned long long *slots,
+                         int nslots) {
+    update_track_any<float>(st, g, E, P, slots, nslots);
+}
+void launch_update_track(hipStream_t st, const PfGeom &g, const double *E, double *P, unsigned long long *slots,
+                         int nslots) {
+    update_track_any<double>(st, g, E, P, slots, nslots);
 }
 
 void launch_ghost_refresh32(hipStream_t st, const PfGeom &g, int do_y, int do_z, float *E) {

@@ -15,7 +15,8 @@ module pf_binding
   integer(c_int), parameter :: PF_PREC_FP64 = 0, PF_PREC_MIXED = 1
   integer(c_int), parameter :: PF_FIELD_U = 0, PF_FIELD_V = 1, PF_FIELD_W = 2, PF_FIELD_P = 3, &
                                PF_FIELD_POROSITY = 4
-  integer(c_int), parameter :: PF_ABI_VERSION_EXPECTED = 2
+  integer(c_int), parameter :: PF_SOLVER_SOR = 0, PF_SOLVER_MG = 1
+  integer(c_int), parameter :: PF_ABI_VERSION_EXPECTED = 3
 
   type, bind(C) :: pf_desc
     integer(c_int32_t) :: dim = 0, m = 0, n = 0, l = 1
@@ -23,11 +24,13 @@ module pf_binding
     integer(c_int32_t) :: face(6) = 0
     integer(c_int32_t) :: precision = 0, inner_iters = 0, max_cycles = 0, report_residual = 0
     integer(c_int32_t) :: inlet_porosity = 0
+    integer(c_int32_t) :: solver = 0
     real(c_double) :: dx = 0, dy = 0, dz = 0, dt = 0
     real(c_double) :: xnue = 0, xlambda = 0, density = 0, thickness = 0
     real(c_double) :: inlet_velocity = 0, outlet_pressure = 0, aoa = 0
     real(c_double) :: omega = 0
     real(c_double) :: residual_tol = 0
+    real(c_double) :: inner_tol = 0
   end type pf_desc
 
   type, bind(C) :: pf_stats

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(PKG, "_build", "libpixelflow.so")
 
 PF_BC_UNIFORM, PF_BC_AIR = 0, 1
 PF_PREC_FP64, PF_PREC_MIXED = 0, 1
+PF_SOLVER_SOR, PF_SOLVER_MG = 0, 1
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "porosity": 4}
 PF_EDIVERGED = -5
 ERRORS = {-1: "PF_EINVAL", -2: "PF_EHIP", -3: "PF_ENOMEM", -4: "PF_ESTATE", -5: "PF_EDIVERGED"}
@@ -33,11 +34,12 @@ class PfDesc(ctypes.Structure):
         ("bc", ctypes.c_int32), ("nonslip", ctypes.c_int32), ("iter_max", ctypes.c_int32),
         ("face", ctypes.c_int32 * 6), ("precision", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
         ("max_cycles", ctypes.c_int32), ("report_residual", ctypes.c_int32), ("inlet_porosity", ctypes.c_int32),
+        ("solver", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
         ("xnue", ctypes.c_double), ("xlambda", ctypes.c_double), ("density", ctypes.c_double),
         ("thickness", ctypes.c_double), ("inlet_velocity", ctypes.c_double),
         ("outlet_pressure", ctypes.c_double), ("aoa", ctypes.c_double), ("omega", ctypes.c_double),
-        ("residual_tol", ctypes.c_double),
+        ("residual_tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
     ]
 
 
@@ -140,7 +142,8 @@ def lib() -> ctypes.CDLL:
 
 
 def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0, max_cycles: int = 0,
-              residual_tol: float = 0.0, report_residual: bool = False) -> PfDesc:
+              residual_tol: float = 0.0, report_residual: bool = False, solver: int = PF_SOLVER_SOR,
+              inner_tol: float = 0.0) -> PfDesc:
     d = PfDesc()
     d.dim, d.m, d.n, d.l = p.dim, p.m, p.n, (p.l if p.dim == 3 else 1)
     d.bc = PF_BC_AIR if p.bc == "air" else PF_BC_UNIFORM
@@ -158,6 +161,8 @@ def make_desc(p: CaseParams, precision: int = PF_PREC_FP64, inner_iters: int = 0
     d.inlet_velocity, d.outlet_pressure, d.aoa = p.inlet_velocity, p.outlet_pressure, p.aoa
     d.omega = p.relux_factor
     d.residual_tol = residual_tol
+    d.solver = int(solver)
+    d.inner_tol = float(inner_tol)
     return d
 
 

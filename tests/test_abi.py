@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     L = solver.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.pf_abi_version() == 2
+    assert L.pf_abi_version() == 3
 
 
 @pytest.mark.skipif(not os.path.exists(solver.LIB_PATH), reason="HIP library not built")
