@@ -15,7 +15,8 @@ from pixelflow_amd import cases
 
 pytestmark = pytest.mark.gpu
 
-FULL = [n for n in golden_names() if not n.startswith("g3d_ref_")]
+# the full-length benchmark fixtures have their own tests (tests/test_gpu_longrun.py)
+FULL = [n for n in golden_names() if not n.startswith("g3d_ref_") and n not in ("g2d_c1_2000", "g3d_c2_500")]
 
 
 def _solver(p, phi):
