@@ -231,6 +231,8 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         pf::MgCase mc;
         mc.g = g;
         mc.a = c->a;
+        mc.P = c->P;
+        mc.BB = c->BB;
         mc.phi = c->phi;
         mc.PHIs = c->PHIs;
         mc.PHI32 = c->PHI32;
@@ -528,43 +530,22 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
     return cycles;
 }
 
-// Tolerance-mode solve with the multigrid-preconditioned inner solver (pf_mg.h): before
-// every cycle the FP64 residual r = bb - A p of the reference operator (k_residual, stored
-// in precision T as the inner right-hand side) and its max|r/ap|; then A e = r to inner_tol
-// by BiCGStab with V-cycles; then p += e in FP64, its max |e| kept as the step's diagnostic.
+// Tolerance-mode solve with the multigrid-preconditioned inner solver (pf_mg.h): cycles of
+// the FP64 residual of the reference operator, an inner BiCGStab solve with V-cycles to
+// inner_tol, and the FP64 update p += e, until max|r/ap| <= residual_tol.  The step's
+// diagnostic (max |e| of its updates) is left in the solver's err_slots.
 template <typename T>
 static int enqueue_mg(pf_ctx *c, pf::MgSolver<T> &S, double *res, int *iters) {
-    const PfGeom &g = c->g;
-    const bool fast = (c->sor_variant == 2);
-    const int maxc = c->d.max_cycles > 0 ? c->d.max_cycles : 50;
-    const double rtol = c->d.inner_tol > 0 ? c->d.inner_tol : 1e-3;
-    const int maxit = c->d.inner_iters > 0 ? c->d.inner_iters : 30;
-    int cycles = 0;
-    *iters = 0;
-    *res = -1.0;
-    for (int cyc = 0; cyc <= maxc; ++cyc) {
-        for (int half = 0; half < 2; ++half) {
-            pf::SweepArgs s{};
-            s.colour = half;
-            s.track_err = 1;
-            s.xcd_remap = c->xcd_remap;
-            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, S.r, (T *)nullptr, c->res_part,
-                                fast);
-        }
-        pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
-        unsigned long long bits = 0;
-        PF_TRY_HIP(c, hipMemcpyAsync(&bits, c->res_val, sizeof(bits), hipMemcpyDeviceToHost, c->st));
-        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
-        std::memcpy(res, &bits, sizeof(double));
-        if (*res <= c->d.residual_tol || cyc == maxc || !std::isfinite(*res)) break;
-        int apps = 0;
-        double rel = 0.;
-        PF_TRY_HIP(c, pf::mg_solve(S, c->st, rtol, maxit, &apps, &rel));
-        *iters += apps;
-        pf::launch_update_track(c->st, g, S.x, c->P, c->err_part, c->nparts);
-        ++cycles;
-    }
-    return cycles;
+    pf::MgControl ctl;
+    ctl.tol = c->d.residual_tol;
+    ctl.max_cycles = c->d.max_cycles > 0 ? c->d.max_cycles : 50;
+    ctl.rtol = c->d.inner_tol > 0 ? c->d.inner_tol : 1e-3;
+    ctl.maxit = c->d.inner_iters > 0 ? c->d.inner_iters : 30;
+    std::vector<pf::MgStepStats> out;
+    PF_TRY_HIP(c, pf::mg_step_solve<T>({&S}, {c->st}, ctl, out));
+    *res = out[0].residual;
+    *iters = out[0].vcycles;
+    return out[0].cycles;
 }
 
 // FP64 solve to a residual tolerance: cycles of inner_iters reference SOR iterations with an
@@ -648,7 +629,9 @@ extern "C" int pf_step(pf_ctx *c, int nsteps, pf_stats *stats) {
         } else {
             enqueue_sor(c, c->err_part, c->d.iter_max);
         }
-        pf::launch_reduce_err(c->st, c->err_part, c->nparts, c->err + s);
+        if (c->mg32) pf::launch_reduce_err(c->st, c->mg32->err_slots, c->mg32->nslots, c->err + s);
+        else if (c->mg64) pf::launch_reduce_err(c->st, c->mg64->err_slots, c->mg64->nslots, c->err + s);
+        else pf::launch_reduce_err(c->st, c->err_part, c->nparts, c->err + s);
         if (c->timing) PF_TRY_HIP(c, hipEventRecord(c->evs[(size_t)(2 * s + 1)], c->st));
         pf::launch_project(c->st, g, c->a, c->P, c->u, c->v, c->w);
         pf::launch_boundary(c->st, g, c->a, c->phi, c->u, c->v, c->w, c->P);

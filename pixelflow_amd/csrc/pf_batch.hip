@@ -31,6 +31,7 @@
 #include "pf_device.h"
 #include "pf_host.h"
 #include "pf_kernels.h"
+#include "pf_mg.h"
 
 namespace pf {
 
@@ -361,6 +362,10 @@ struct pf_batch {
         int64_t off = 0, bytes = 0;
         BatchCase bc{};
         int64_t steps = 0;
+        // tolerance mode with PF_SOLVER_MG: the case's solver, its arrays inside the
+        // case's arena range (FP32 for mixed precision, FP64 otherwise)
+        pf::MgSolver<float> *mg32 = nullptr;
+        pf::MgSolver<double> *mg64 = nullptr;
     };
     std::vector<Slot> slots;
     std::vector<int> order;         // compact case index -> slot
@@ -399,6 +404,9 @@ struct pf_batch {
     pf_batch_config cfg{};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // PF_SOLVER_MG: per-step statistics of the last pf_batch_step, kept on the host
+    std::vector<double> mg_res;
+    std::vector<int> mg_cyc, mg_vc;
     std::string msg;
 };
 
@@ -429,6 +437,8 @@ static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
 // Bytes of one case inside a batch arena (velocity pairs, porosity, div, p/bb/porosity in
 // colour-split FP64, FP32 e/r/porosity for mixed precision, the interior output buffer).
+static bool mg_mode(const pf_desc &d) { return d.residual_tol > 0 && d.solver == PF_SOLVER_MG; }
+
 static int64_t case_bytes(const pf_desc &d) {
     const PfGeom g = pf_make_geom(d.dim, d.m, d.n, d.l);
     const int nat = (d.dim == 3) ? 8 : 6;
@@ -437,7 +447,13 @@ static int64_t case_bytes(const pf_desc &d) {
     b += 3 * align256(8 * 2 * g.HS);
     if (d.precision == PF_PREC_MIXED) b += 3 * align256(4 * 2 * g.HS);
     b += align256(8 * (int64_t)d.m * d.n * (d.dim == 3 ? d.l : 1));
+    if (mg_mode(d)) b += align256(pf::mg_bytes(g, d.bc, d.precision == PF_PREC_MIXED));
     return b;
+}
+
+static void slot_release_mg(pf_batch::Slot &S) {
+    if (S.mg32) { pf::mg_destroy(*S.mg32); delete S.mg32; S.mg32 = nullptr; }
+    if (S.mg64) { pf::mg_destroy(*S.mg64); delete S.mg64; S.mg64 = nullptr; }
 }
 
 extern "C" int64_t pf_batch_case_bytes(const pf_desc *d) {
@@ -448,7 +464,8 @@ extern "C" int64_t pf_batch_case_bytes(const pf_desc *d) {
 
 static bool same_kind(const pf_desc &a, const pf_desc &b) {
     return a.dim == b.dim && a.bc == b.bc && a.precision == b.precision && a.iter_max == b.iter_max &&
-           a.inner_iters == b.inner_iters && a.max_cycles == b.max_cycles && a.residual_tol == b.residual_tol;
+           a.inner_iters == b.inner_iters && a.max_cycles == b.max_cycles && a.residual_tol == b.residual_tol &&
+           a.solver == b.solver && a.inner_tol == b.inner_tol;
 }
 
 static void drop_graphs(pf_batch *b) {
@@ -536,6 +553,9 @@ extern "C" int pf_batch_create(const pf_desc *kind, int max_cases, int64_t arena
 extern "C" int pf_batch_destroy(pf_batch *b) {
     if (!b) return PF_EINVAL;
     if (b->st) (void)hipStreamSynchronize(b->st);
+    for (int q = 0; q < 2; ++q)
+        if (b->side[q]) (void)hipStreamSynchronize(b->side[q]);
+    for (auto &s : b->slots) slot_release_mg(s);
     drop_graphs(b);
     void *ptrs[] = {b->arena, b->d_cases, b->d_nat_prefix, b->d_swp_prefix, b->d_nat_tile, b->d_swp_tile,
                     b->d_face_tile[0], b->d_face_tile[1], b->d_face_tile[2],
@@ -787,6 +807,7 @@ extern "C" int pf_batch_add(pf_batch *b, const pf_desc *d, const double *raw, do
         bc.PHI32 = takeF(2 * g.HS);
     }
     bc.dout = takeD((int64_t)g.m * g.n * g.l);
+    char *mg_mem = mg_mode(*d) ? p : nullptr;   // the solver's arrays, carved below
     const bool periodic = d->bc == PF_BC_UNIFORM;
     bc.fuse_y = periodic && (g.n % 2 == 0);
     bc.fuse_z = periodic && g.dim == 3 && (g.l % 2 == 0);
@@ -810,6 +831,39 @@ extern "C" int pf_batch_add(pf_batch *b, const pf_desc *d, const double *raw, do
     (void)hipFree(bad);
     PFB_TRY(b, e);
     bc.fast = (hbad == 0);
+    if (mg_mem) {
+        // the single-case solve's launch settings for this grid (pf_create)
+        const int64_t cells = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
+        pf::MgCase mc;
+        mc.g = g;
+        mc.a = bc.a;
+        mc.P = bc.P;
+        mc.BB = bc.BB;
+        mc.phi = bc.phi;
+        mc.PHIs = bc.PHIs;
+        mc.PHI32 = bc.PHI32;
+        mc.fast = bc.fast;
+        mc.kb32 = (cells * 12 <= ((int64_t)128 << 20)) ? 2 : 1;
+        mc.kb64 = 1;
+        mc.xw = pf::sweep_wave_width(g);
+        mc.remap32 = 1;
+        mc.remap64 = (cells * 24 > ((int64_t)512 << 20)) ? 0 : 1;
+        std::string mwhy;
+        bool ok;
+        if (d->precision == PF_PREC_MIXED) {
+            S.mg32 = new (std::nothrow) pf::MgSolver<float>();
+            ok = S.mg32 && pf::mg_create(*S.mg32, mc, &mwhy, mg_mem);
+        } else {
+            S.mg64 = new (std::nothrow) pf::MgSolver<double>();
+            ok = S.mg64 && pf::mg_create(*S.mg64, mc, &mwhy, mg_mem);
+        }
+        if (!ok) {
+            slot_release_mg(S);
+            arena_free(b, off, bytes);
+            b->msg = mwhy.empty() ? std::string("multigrid solver set-up failed") : mwhy;
+            return PF_ENOMEM;
+        }
+    }
     // initial_conditions + boundary into the buffers the next step reads as "old"
     const int q = 1 - b->parity;
     double u0, v0;
@@ -840,6 +894,9 @@ extern "C" int pf_batch_remove(pf_batch *b, int slot) {
     if (!b || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return PF_EINVAL;
     PFB_TRY(b, hipStreamSynchronize(b->st));
     pf_batch::Slot &S = b->slots[(size_t)slot];
+    for (int q = 0; q < 2; ++q)
+        if (b->side[q]) PFB_TRY(b, hipStreamSynchronize(b->side[q]));
+    slot_release_mg(S);
     arena_free(b, S.off, S.bytes);
     S.used = false;
     b->dirty = true;
@@ -1147,6 +1204,48 @@ static int step_tolerance_eager(pf_batch *b, int p) {
     return PF_OK;
 }
 
+// Tolerance mode with PF_SOLVER_MG, step `s` of the call: the batched stages before and
+// after the solve, and in between every case's multigrid solve (pf_mg.h), the cases dealt
+// to the batch's stream and its two side streams so that one case's host round trips
+// overlap the others' work.  Per-case statistics go to the host-side history.
+template <typename T>
+static int step_mg_t(pf_batch *b, int p, int s, pf::MgSolver<T> *pf_batch::Slot::*member) {
+    enqueue_pre(b, p);
+    std::vector<pf::MgSolver<T> *> runs;
+    std::vector<int> slots;
+    for (int slot : b->order) {
+        pf::MgSolver<T> *S = b->slots[(size_t)slot].*member;
+        if (!S) continue;
+        runs.push_back(S);
+        slots.push_back(slot);
+    }
+    std::vector<hipStream_t> streams = {b->st, b->side[0], b->side[1]};
+    PFB_TRY(b, hipEventRecord(b->ev_fork, b->st));
+    for (int q = 0; q < 2; ++q) PFB_TRY(b, hipStreamWaitEvent(b->side[q], b->ev_fork, 0));
+    pf::MgControl ctl;
+    ctl.tol = b->kind.residual_tol;
+    ctl.max_cycles = b->kind.max_cycles > 0 ? b->kind.max_cycles : 50;
+    ctl.rtol = b->kind.inner_tol > 0 ? b->kind.inner_tol : 1e-3;
+    ctl.maxit = b->kind.inner_iters > 0 ? b->kind.inner_iters : 30;
+    std::vector<pf::MgStepStats> out;
+    PFB_TRY(b, pf::mg_step_solve<T>(runs, streams, ctl, out));
+    for (size_t q = 0; q < runs.size(); ++q) {
+        const int slot = slots[q];
+        pf::launch_reduce_err(streams[q % streams.size()], runs[q]->err_slots, runs[q]->nslots,
+                              b->d_err_hist + (int64_t)s * b->max_cases + slot);
+        b->mg_res[(size_t)s * b->max_cases + slot] = out[q].residual;
+        b->mg_cyc[(size_t)s * b->max_cases + slot] = out[q].cycles;
+        b->mg_vc[(size_t)s * b->max_cases + slot] = out[q].vcycles;
+    }
+    for (int q = 0; q < 2; ++q) {
+        PFB_TRY(b, hipEventRecord(b->ev_join[q], b->side[q]));
+        PFB_TRY(b, hipStreamWaitEvent(b->st, b->ev_join[q], 0));
+    }
+    enqueue_post(b, p);
+    hipLaunchKernelGGL(pf::k_inc_int, dim3(1), dim3(64), 0, b->st, b->d_step);
+    return PF_OK;
+}
+
 extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
     if (!b || nsteps < 0) return PF_EINVAL;
     if (nsteps == 0) return PF_OK;
@@ -1156,11 +1255,22 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
     if ((rc = ensure_hist(b, nsteps)) != PF_OK) return rc;
     const int nc = (int)b->order.size();
     const bool tol = tol_mode(b);
-    const bool graph = b->cfg.use_graph && (!tol || tol_max_cycles(b) <= kTolGraphMaxCycles);
+    const bool mg = mg_mode(b->kind);
+    const bool graph = !mg && b->cfg.use_graph && (!tol || tol_max_cycles(b) <= kTolGraphMaxCycles);
+    if (mg) {
+        const size_t n = (size_t)nsteps * b->max_cases;
+        b->mg_res.assign(n, -1.0);
+        b->mg_cyc.assign(n, 0);
+        b->mg_vc.assign(n, 0);
+    }
     hipLaunchKernelGGL(pf::k_set_int, dim3(1), dim3(64), 0, b->st, b->d_step, 0);
     for (int s = 0; s < nsteps; ++s) {
         const int p = b->parity;
-        if (graph) {
+        if (mg) {
+            rc = (b->kind.precision == PF_PREC_MIXED) ? step_mg_t<float>(b, p, s, &pf_batch::Slot::mg32)
+                                                      : step_mg_t<double>(b, p, s, &pf_batch::Slot::mg64);
+            if (rc != PF_OK) return rc;
+        } else if (graph) {
             if (!b->gexec[p] && (rc = capture(b, p)) != PF_OK) return rc;
             PFB_TRY(b, hipGraphLaunch(b->gexec[p], b->st));
         } else if (tol) {
@@ -1174,9 +1284,17 @@ extern "C" int pf_batch_step(pf_batch *b, int nsteps, pf_stats *stats) {
     std::vector<unsigned long long> eh((size_t)nsteps * b->max_cases), rh((size_t)nsteps * b->max_cases);
     std::vector<int> ih, ch;
     PFB_TRY_SYNC(b, hipMemcpyAsync(eh.data(), b->d_err_hist, eh.size() * 8, hipMemcpyDeviceToHost, b->st));
-    if (tol || b->kind.report_residual)
+    if (!mg && (tol || b->kind.report_residual))
         PFB_TRY_SYNC(b, hipMemcpyAsync(rh.data(), b->d_res_hist, rh.size() * 8, hipMemcpyDeviceToHost, b->st));
-    if (tol) {
+    if (mg) {
+        ih.assign(eh.size(), 0);
+        ch.assign(eh.size(), 0);
+        for (size_t q = 0; q < eh.size(); ++q) {
+            std::memcpy(&rh[q], &b->mg_res[q], 8);
+            ih[q] = b->mg_vc[q];
+            ch[q] = b->mg_cyc[q];
+        }
+    } else if (tol) {
         ih.resize(eh.size());
         ch.resize(eh.size());
         PFB_TRY_SYNC(b, hipMemcpyAsync(ih.data(), b->d_it_hist, ih.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));

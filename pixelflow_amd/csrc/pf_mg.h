@@ -88,6 +88,8 @@ struct MgLevel {
 struct MgCase {
     PfGeom g;
     StepArgs a;
+    double *P = nullptr;             // colour-split FP64 pressure (updated in place)
+    const double *BB = nullptr;      // colour-split FP64 right-hand side
     const double *phi = nullptr;     // natural FP64 porosity (the air rows' fluid test)
     const double *PHIs = nullptr;    // colour-split FP64 porosity
     const float *PHI32 = nullptr;    // colour-split FP32 porosity (FP32 solver)
@@ -114,27 +116,55 @@ struct MgSolver {
       *y = nullptr, *z = nullptr;
     double *scal = nullptr;          // device scalars (see pf_mg.hip)
     double *part = nullptr;          // MG_NPART x 4 partial sums
+    // the outer loop's reductions: per-wave max |r/ap| and max |e| (nslots each), the
+    // reduced residual
+    unsigned long long *res_slots = nullptr, *err_slots = nullptr, *res_val = nullptr;
+    int nslots = 0;
     double *h_scal = nullptr;        // pinned host copy of the scalars the host tests
-    void *mem = nullptr;             // the one device allocation holding all of the above
+    hipEvent_t ev = nullptr;         // marks the end of the work the host waits for next
+    void *mem = nullptr;             // the device memory holding all of the above
+    bool own_mem = false;            // mem came from hipMalloc (else from the caller)
     int64_t bytes = 0;
     bool built = false;              // coarse operators match the current porosity
+    // state of the current step's solve (mg_step_solve)
+    int state = 0, cyc = 0, it = 0, vcycles = 0;
+    double res = -1., b2 = 0.;
 };
 
-// Device bytes of a solver for grid g (one allocation).
+// Controls of the pressure solve of one step.
+struct MgControl {
+    double tol = 1e-8;               // stop when the FP64 max|r/ap| <= tol
+    int max_cycles = 50;             // outer (FP64 residual + update) cycles
+    double rtol = 1e-3;              // relative residual of each inner solve
+    int maxit = 30;                  // BiCGStab iterations per inner solve
+};
+
+struct MgStepStats {
+    double residual;                 // FP64 max|r/ap| of the final pressure
+    int cycles;                      // outer cycles run
+    int vcycles;                     // V-cycles applied
+};
+
+// Device bytes of a solver for grid g.
 int64_t mg_bytes(const PfGeom &g, int bc, bool fp32);
-// Allocate (hipMalloc, hipHostMalloc) and carve the arrays; false on failure with *why set.
+// Carve the arrays from `mem` (mg_bytes of them, zeroed by the caller), or allocate them
+// (hipMalloc) when mem is null; false on failure with *why set.
 template <typename T>
-bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why);
+bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem = nullptr);
 template <typename T>
 void mg_destroy(MgSolver<T> &S);
 // Build the coarse operators from the case's porosity (after every porosity change).
 template <typename T>
 void mg_setup(MgSolver<T> &S, hipStream_t st);
-// Solve A e = r to ||r_k|| <= rtol ||r_0|| (or maxit BiCGStab iterations) from e = 0, with
-// S.r holding r_0 on entry; e is left in S.x with its periodic ghost copies.  *applications
-// receives the number of V-cycles applied, *relres ||r_k|| / ||r_0|| (recursive residual).
-// Returns hipSuccess or the first HIP error.
+// The pressure solve of one step for every solver of `runs` together, solver q on
+// streams[q % streams.size()]: per solver, cycles of
+//   r = bb - A p (FP64, the reference's operator) and max|r/ap|, stop when <= ctl.tol;
+//   A e = r from e = 0 by BiCGStab with a V-cycle per application, to ctl.rtol;
+//   p += e (FP64), max |e| into err_slots;
+// the host waits for one solver at a time while the others' work runs.  out[q] receives
+// solver q's statistics.  Returns hipSuccess or the first HIP error.
 template <typename T>
-hipError_t mg_solve(MgSolver<T> &S, hipStream_t st, double rtol, int maxit, int *applications, double *relres);
+hipError_t mg_step_solve(const std::vector<MgSolver<T> *> &runs, const std::vector<hipStream_t> &streams,
+                         const MgControl &ctl, std::vector<MgStepStats> &out);
 
 }  // namespace pf

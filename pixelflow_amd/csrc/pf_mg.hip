@@ -697,18 +697,21 @@ static std::vector<MgGeom> mg_levels(const PfGeom &g, int bc) {
 
 static inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
+static int mg_nslots(const PfGeom &g) { return sor_blocks(g); }
+
 static int64_t mg_bytes_t(const PfGeom &g, int bc, int64_t tsz) {
     const std::vector<MgGeom> L = mg_levels(g, bc);
     int64_t b = 9 * align256(2 * g.HS * tsz);
     for (size_t q = 1; q < L.size(); ++q) b += 9 * align256(L[q].N * tsz) + align256(L[q].N);
     b += align256(16 * 8) + align256((int64_t)MG_NPART * 4 * 8);
+    b += 2 * align256((int64_t)mg_nslots(g) * 8) + align256(8);
     return b;
 }
 
 int64_t mg_bytes(const PfGeom &g, int bc, bool fp32) { return mg_bytes_t(g, bc, fp32 ? 4 : 8); }
 
 template <typename T>
-bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why) {
+bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
     S.c = c;
     const PfGeom &g = c.g;
     const std::vector<MgGeom> L = mg_levels(g, c.a.bc);
@@ -721,16 +724,31 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why) {
         return false;
     }
     S.bytes = mg_bytes_t(g, c.a.bc, sizeof(T));
-    if (hipMalloc(&S.mem, (size_t)S.bytes) != hipSuccess) {
-        S.mem = nullptr;
-        if (why) *why = "multigrid: hipMalloc of " + std::to_string(S.bytes) + " bytes failed";
-        return false;
+    S.own_mem = (mem == nullptr);
+    if (S.own_mem) {
+        if (hipMalloc(&S.mem, (size_t)S.bytes) != hipSuccess) {
+            S.mem = nullptr;
+            if (why) *why = "multigrid: hipMalloc of " + std::to_string(S.bytes) + " bytes failed";
+            return false;
+        }
+        if (hipMemset(S.mem, 0, (size_t)S.bytes) != hipSuccess) {
+            mg_destroy(S);
+            if (why) *why = "multigrid: hipMemset failed";
+            return false;
+        }
+    } else {
+        S.mem = mem;
     }
     if (hipHostMalloc((void **)&S.h_scal, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         S.h_scal = nullptr;
-        (void)hipFree(S.mem);
-        S.mem = nullptr;
+        mg_destroy(S);
         if (why) *why = "multigrid: hipHostMalloc failed";
+        return false;
+    }
+    if (hipEventCreateWithFlags(&S.ev, hipEventDisableTiming) != hipSuccess) {
+        S.ev = nullptr;
+        mg_destroy(S);
+        if (why) *why = "multigrid: hipEventCreate failed";
         return false;
     }
     char *p = (char *)S.mem;
@@ -754,21 +772,22 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why) {
     }
     S.scal = (double *)take(16 * 8);
     S.part = (double *)take((int64_t)MG_NPART * 4 * 8);
-    if (hipMemset(S.mem, 0, (size_t)S.bytes) != hipSuccess) {
-        mg_destroy(S);
-        if (why) *why = "multigrid: hipMemset failed";
-        return false;
-    }
+    S.nslots = mg_nslots(g);
+    S.res_slots = (unsigned long long *)take((int64_t)S.nslots * 8);
+    S.err_slots = (unsigned long long *)take((int64_t)S.nslots * 8);
+    S.res_val = (unsigned long long *)take(8);
     S.built = false;
     return true;
 }
 
 template <typename T>
 void mg_destroy(MgSolver<T> &S) {
-    if (S.mem) (void)hipFree(S.mem);
+    if (S.mem && S.own_mem) (void)hipFree(S.mem);
     if (S.h_scal) (void)hipHostFree(S.h_scal);
+    if (S.ev) (void)hipEventDestroy(S.ev);
     S.mem = nullptr;
     S.h_scal = nullptr;
+    S.ev = nullptr;
     S.lev.clear();
     S.built = false;
 }
@@ -950,57 +969,139 @@ static void apply0(MgSolver<T> &S, hipStream_t st, const T *y, T *out, const T *
 #undef PF_AP
 }
 
+// ------------------------------------------------------------------------------------
+// The step's solve as a state machine per solver.  After each enqueued stage the solver's
+// event is recorded; the host waits for one solver's event at a time, reads what that
+// stage copied to pinned memory and enqueues the next stage, so several solvers' work is
+// in flight on their streams while the host deals with one.
+// ------------------------------------------------------------------------------------
+enum MgState { MG_RESIDUAL = 0, MG_INIT = 1, MG_ITER = 2, MG_DONE = 3 };
+
+// The FP64 residual r = bb - A p of the reference operator into S.r (precision T) and
+// max|r/ap| into h_scal[8]; the colour launches of the mixed path's residual kernel.
 template <typename T>
-hipError_t mg_solve(MgSolver<T> &S, hipStream_t st, double rtol, int maxit, int *applications, double *relres) {
-    *applications = 0;
-    *relres = 0.;
-    if (!S.built) mg_setup(S, st);
-    const PfGeom &g = S.c.g;
-    const int64_t n = 2 * g.HS;
+static void enqueue_residual(MgSolver<T> &S, hipStream_t st) {
+    const MgCase &c = S.c;
+    for (int half = 0; half < 2; ++half) {
+        SweepArgs sw{};
+        sw.colour = half;
+        sw.track_err = 1;
+        sw.xcd_remap = c.remap32;
+        launch_residual(st, c.g, c.a, sw, c.P, c.PHIs, c.BB, c.phi, S.r, (T *)nullptr, S.res_slots, c.fast);
+    }
+    launch_reduce_err(st, S.res_slots, S.nslots, S.res_val);
+    (void)hipMemcpyAsync(S.h_scal + 8, S.res_val, 8, hipMemcpyDeviceToHost, st);
+}
+
+template <typename T>
+static void enqueue_init(MgSolver<T> &S, hipStream_t st) {
+    const int64_t n = 2 * S.c.g.HS;
     const unsigned fb = flat_blocks(n);
-    const int nap = (int)apply_blocks<T>(g);
     hipLaunchKernelGGL((k_bicg_init<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.rh, S.p, S.v, S.x, S.part);
     hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, (int)fb, S.scal, 0);
-    hipError_t e = hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    const double b2 = S.h_scal[4];
-    if (S.h_scal[6] != 0. || !(b2 > 0.)) {
-        *relres = (b2 == 0.) ? 0. : NAN;
-        return hipGetLastError();
+    (void)hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
+}
+
+template <typename T>
+static void enqueue_iteration(MgSolver<T> &S, hipStream_t st) {
+    const int64_t n = 2 * S.c.g.HS;
+    const unsigned fb = flat_blocks(n);
+    const int nap = (int)apply_blocks<T>(S.c.g);
+    hipLaunchKernelGGL((k_bicg_p<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.p, (const T *)S.v,
+                       (const double *)S.scal);
+    vcycle(S, st, (const T *)S.p, S.y);
+    apply0<T, 0>(S, st, S.y, S.v, S.rh);
+    hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 1);
+    hipLaunchKernelGGL((k_bicg_s<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, (const T *)S.v, S.s,
+                       (const double *)S.scal);
+    vcycle(S, st, (const T *)S.s, S.z);
+    apply0<T, 1>(S, st, S.z, S.t, S.s);
+    hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 2);
+    hipLaunchKernelGGL((k_bicg_xr<T>), dim3(fb), dim3(256), 0, st, n, S.x, (const T *)S.y, (const T *)S.z, S.r,
+                       (const T *)S.s, (const T *)S.t, (const T *)S.rh, (const double *)S.scal, S.part);
+    hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, (int)fb, S.scal, 3);
+    (void)hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
+    S.vcycles += 2;
+}
+
+// p += e and the start of the next cycle.
+template <typename T>
+static void enqueue_update_and_residual(MgSolver<T> &S, hipStream_t st) {
+    launch_update_track(st, S.c.g, S.x, S.c.P, S.err_slots, S.nslots);
+    ++S.cyc;
+    enqueue_residual(S, st);
+    S.state = MG_RESIDUAL;
+}
+
+template <typename T>
+hipError_t mg_step_solve(const std::vector<MgSolver<T> *> &runs, const std::vector<hipStream_t> &streams,
+                         const MgControl &ctl, std::vector<MgStepStats> &out) {
+    const size_t nr = runs.size(), ns = streams.size();
+    out.assign(nr, MgStepStats{-1., 0, 0});
+    if (nr == 0 || ns == 0) return hipSuccess;
+    hipError_t err = hipSuccess;
+    for (size_t q = 0; q < nr; ++q) {
+        MgSolver<T> &S = *runs[q];
+        hipStream_t st = streams[q % ns];
+        if (!S.built) mg_setup(S, st);
+        S.state = MG_RESIDUAL;
+        S.cyc = 0;
+        S.it = 0;
+        S.vcycles = 0;
+        S.res = -1.;
+        enqueue_residual(S, st);
+        if ((err = hipEventRecord(S.ev, st)) != hipSuccess) return err;
     }
-    for (int it = 0; it < maxit; ++it) {
-        hipLaunchKernelGGL((k_bicg_p<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.p, (const T *)S.v,
-                           (const double *)S.scal);
-        vcycle(S, st, (const T *)S.p, S.y);
-        apply0<T, 0>(S, st, S.y, S.v, S.rh);
-        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 1);
-        hipLaunchKernelGGL((k_bicg_s<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, (const T *)S.v, S.s,
-                           (const double *)S.scal);
-        vcycle(S, st, (const T *)S.s, S.z);
-        apply0<T, 1>(S, st, S.z, S.t, S.s);
-        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 2);
-        hipLaunchKernelGGL((k_bicg_xr<T>), dim3(fb), dim3(256), 0, st, n, S.x, (const T *)S.y, (const T *)S.z, S.r,
-                           (const T *)S.s, (const T *)S.t, (const T *)S.rh, (const double *)S.scal, S.part);
-        hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, (int)fb, S.scal, 3);
-        *applications += 2;
-        e = hipMemcpyAsync(S.h_scal, S.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return e;
-        const double rel = std::sqrt(S.h_scal[5] / b2);
-        *relres = rel;
-        if (!(rel > rtol) || S.h_scal[6] != 0.) break;   // converged, breakdown, or non-finite
+    size_t active = nr;
+    while (active > 0) {
+        for (size_t q = 0; q < nr; ++q) {
+            MgSolver<T> &S = *runs[q];
+            if (S.state == MG_DONE) continue;
+            hipStream_t st = streams[q % ns];
+            if ((err = hipEventSynchronize(S.ev)) != hipSuccess) return err;
+            if (S.state == MG_RESIDUAL) {
+                unsigned long long bits;
+                std::memcpy(&bits, S.h_scal + 8, 8);
+                std::memcpy(&S.res, &bits, 8);
+                // a non-finite residual never converges: stop at once (the caller reports it)
+                if (S.res <= ctl.tol || S.cyc >= ctl.max_cycles || !std::isfinite(S.res)) {
+                    S.state = MG_DONE;
+                    out[q] = MgStepStats{S.res, S.cyc, S.vcycles};
+                    --active;
+                    continue;
+                }
+                enqueue_init(S, st);
+                S.state = MG_INIT;
+            } else if (S.state == MG_INIT) {
+                S.b2 = S.h_scal[4];
+                if (S.h_scal[6] != 0. || !(S.b2 > 0.)) {
+                    enqueue_update_and_residual(S, st);   // e = 0: the residual is re-read
+                } else {
+                    S.it = 0;
+                    enqueue_iteration(S, st);
+                    S.state = MG_ITER;
+                }
+            } else {   // MG_ITER
+                ++S.it;
+                const double rel = std::sqrt(S.h_scal[5] / S.b2);
+                if (!(rel > ctl.rtol) || S.h_scal[6] != 0. || S.it >= ctl.maxit) enqueue_update_and_residual(S, st);
+                else enqueue_iteration(S, st);
+            }
+            if ((err = hipEventRecord(S.ev, st)) != hipSuccess) return err;
+        }
     }
     return hipGetLastError();
 }
 
-template bool mg_create<float>(MgSolver<float> &, const MgCase &, std::string *);
-template bool mg_create<double>(MgSolver<double> &, const MgCase &, std::string *);
+template bool mg_create<float>(MgSolver<float> &, const MgCase &, std::string *, void *);
+template bool mg_create<double>(MgSolver<double> &, const MgCase &, std::string *, void *);
 template void mg_destroy<float>(MgSolver<float> &);
 template void mg_destroy<double>(MgSolver<double> &);
 template void mg_setup<float>(MgSolver<float> &, hipStream_t);
 template void mg_setup<double>(MgSolver<double> &, hipStream_t);
-template hipError_t mg_solve<float>(MgSolver<float> &, hipStream_t, double, int, int *, double *);
-template hipError_t mg_solve<double>(MgSolver<double> &, hipStream_t, double, int, int *, double *);
+template hipError_t mg_step_solve<float>(const std::vector<MgSolver<float> *> &, const std::vector<hipStream_t> &,
+                                         const MgControl &, std::vector<MgStepStats> &);
+template hipError_t mg_step_solve<double>(const std::vector<MgSolver<double> *> &, const std::vector<hipStream_t> &,
+                                          const MgControl &, std::vector<MgStepStats> &);
 
 }  // namespace pf

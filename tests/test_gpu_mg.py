@@ -113,3 +113,51 @@ def test_mg_is_deterministic():
         np.testing.assert_array_equal(a[f], b[f])
     np.testing.assert_array_equal(sa["residual"], sb["residual"])
     np.testing.assert_array_equal(sa["sor_iters"], sb["sor_iters"])
+
+
+def _rooms():
+    return [cases.room_case(seed=s, steps=5, iter_max=100, size=sz)
+            for s, sz in ((3, (24, 22, 20)), (7, (32, 32, 32)), (11, (40, 28, 24)))]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_mg_matches_each_case_alone(precision):
+    """Rooms of three sizes in one batch, MG tolerance mode: every case gives the bits of its
+    single-case run, and every step of every case reaches the rule."""
+    from pixelflow_amd import solver as S
+    prec = S.PF_PREC_MIXED if precision == "mixed" else S.PF_PREC_FP64
+    rooms = _rooms()
+    kw = _mg(prec)
+    with S.Batch(rooms[0][0], max_cases=4, arena_bytes=1 << 28, **kw) as b:
+        slots = [b.add(p, phi) for p, phi in rooms]
+        st = b.step(5)
+        got = [{f: b.interior(s, f) for f in ("u", "v", "w", "p")} for s in slots]
+    for (p, phi), s, g in zip(rooms, slots, got):
+        ref, rst, _ = _run(p, phi, 5, **kw)
+        assert np.all(st["residual"][:, s] <= TOL), st["residual"][:, s]
+        for f in g:
+            np.testing.assert_array_equal(g[f], ref[f], err_msg=f"slot {s} {p.m}x{p.n}x{p.l} {f}")
+        np.testing.assert_array_equal(st["residual"][:, s], rst["residual"])
+        np.testing.assert_array_equal(st["sor_iters"][:, s], rst["sor_iters"])
+        np.testing.assert_array_equal(st["sor_error"][:, s], rst["sor_error"])
+
+
+def test_server_mg_requests_match_alone():
+    """Served rooms in MG tolerance mode, submitted while others run: each result equals the
+    case alone, and the last step of each reached the rule."""
+    from pixelflow_amd import solver as S
+    rooms = _rooms()
+    kw = _mg(S.PF_PREC_MIXED)
+    with S.NativeServer(rooms[0][0], max_cases=2, arena_bytes=1 << 28, chunk=2, **kw) as srv:
+        ids = [srv.submit(p, phi, 5) for p, phi in rooms]
+        for rid in ids:
+            assert srv.wait(rid, 120.0) == S.SERVE_DONE, srv.error(rid)
+        res = []
+        for rid, (p, phi) in zip(ids, rooms):
+            state, done, its, last = srv.poll(rid)
+            assert done == 5 and last["residual"] <= TOL
+            res.append({f: srv.interior(rid, f) for f in ("u", "v", "w", "p")})
+    for (p, phi), g in zip(rooms, res):
+        ref, _, _ = _run(p, phi, 5, **kw)
+        for f in g:
+            np.testing.assert_array_equal(g[f], ref[f], err_msg=f"{p.m}x{p.n}x{p.l} {f}")
