@@ -29,6 +29,14 @@ module pf_case_io
     character(len=50) :: output_folder, csv_file
     integer :: iter_max
     real(rk) :: relux_factor
+    ! optional group &gpu_control (not in the reference's controlDict; defaults = the
+    ! reference's fixed-count FP64 solve)
+    logical :: gpu_group = .false.
+    character(len=16) :: precision = 'fp64'      ! 'fp64' or 'mixed'
+    character(len=16) :: solver = 'sor'          ! 'sor' or 'multigrid' (tolerance mode)
+    real(rk) :: residual_tol = 0.0_rk            ! > 0: tolerance mode, max|r/ap| <= residual_tol
+    integer :: inner_iters = 0, max_cycles = 0
+    real(rk) :: inner_tol = 0.0_rk
   end type pf_settings
 
 contains
@@ -42,7 +50,9 @@ contains
     write(*,*) '# --- TIME: ', trim(stamp)
   end subroutine print_time_stamp
 
-  ! The seven namelist groups, read in order, then echoed.
+  ! The seven namelist groups, read in order, then echoed; then the optional eighth group
+  ! &gpu_control, read only when the file holds it (an existing controlDict parses as
+  ! before and its run is unchanged).
   subroutine read_control_dict(s)
     type(pf_settings), intent(out) :: s
     real(rk) :: xnue, xlambda, density, width, height, depth, time
@@ -53,7 +63,11 @@ contains
     character(len=50) :: output_folder, csv_file
     integer :: iter_max
     real(rk) :: relux_factor
-    integer :: unit
+    integer :: unit, ios
+    character(len=16) :: precision, solver
+    real(rk) :: residual_tol, inner_tol
+    integer :: inner_iters, max_cycles
+    namelist /gpu_control/ precision, solver, residual_tol, inner_iters, max_cycles, inner_tol
     namelist /physical/ xnue, xlambda, density, width, height, depth, time
     namelist /physical/ inlet_velocity, outlet_pressure, AoA
     namelist /file_control/ istep_out
@@ -71,7 +85,35 @@ contains
     read(unit, nml=calculation_method)
     read(unit, nml=directory_control)
     read(unit, nml=solver_control)
+    precision = 'fp64'
+    solver = 'sor'
+    residual_tol = 0.0_rk
+    inner_iters = 0
+    max_cycles = 0
+    inner_tol = 0.0_rk
+    read(unit, nml=gpu_control, iostat=ios)
+    if (ios > 0) then
+      write(*,*) 'controlDict: the &gpu_control group could not be read'
+      error stop 1
+    end if
     close(unit)
+    s%gpu_group = (ios == 0)
+    if (s%gpu_group) then
+      if (precision /= 'fp64' .and. precision /= 'mixed') then
+        write(*,*) 'controlDict: &gpu_control precision must be fp64 or mixed'
+        error stop 1
+      end if
+      if (solver /= 'sor' .and. solver /= 'multigrid') then
+        write(*,*) 'controlDict: &gpu_control solver must be sor or multigrid'
+        error stop 1
+      end if
+      s%precision = precision
+      s%solver = solver
+      s%residual_tol = residual_tol
+      s%inner_iters = inner_iters
+      s%max_cycles = max_cycles
+      s%inner_tol = inner_tol
+    end if
 
     s%xnue = xnue; s%xlambda = xlambda; s%density = density
     s%width = width; s%height = height; s%depth = depth; s%time = time
@@ -108,6 +150,14 @@ contains
     write(*,*) '# --- Solver information'
     write(*,*) '# SOR max iteration steps =', iter_max
     write(*,*) '# SOR reluxation factor =', relux_factor
+    if (s%gpu_group) then
+      write(*,*) '#'
+      write(*,*) '# --- GPU control'
+      write(*,*) '# precision = ', trim(s%precision)
+      write(*,*) '# solver = ', trim(s%solver)
+      write(*,*) '# residual_tol =', s%residual_tol
+      write(*,*) '# inner_iters =', s%inner_iters, ' max_cycles =', s%max_cycles, ' inner_tol =', s%inner_tol
+    end if
   end subroutine read_control_dict
 
   ! Porosity CSV -> raw interior values, i fastest (the row's own i, j, k place each value).

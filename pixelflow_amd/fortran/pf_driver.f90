@@ -69,6 +69,12 @@ contains
     if (s%nonslip) d%nonslip = 1
     d%iter_max = s%iter_max
     d%precision = PF_PREC_FP64
+    if (s%precision == 'mixed') d%precision = PF_PREC_MIXED
+    if (s%solver == 'multigrid') d%solver = PF_SOLVER_MG
+    d%residual_tol = s%residual_tol
+    d%inner_iters = s%inner_iters
+    d%max_cycles = s%max_cycles
+    d%inner_tol = s%inner_tol
     if (variant == '2d_backstep') d%inlet_porosity = 1
     d%dx = dx
     d%dy = dy
@@ -126,7 +132,14 @@ contains
         istep = istep + 1
         time = istep * dt
         write(*,*) '--- time_steps= ', istep, ' --  time = ', time
-        write(*,*) 'SOR iteration no.', s%iter_max, '-- p error:', stats(q)%sor_error
+        if (s%residual_tol > 0.0_rk) then
+          ! tolerance mode: the iterations the step ran (V-cycles for the multigrid solve)
+          ! and the FP64 residual it reached
+          write(*,*) 'SOR iteration no.', stats(q)%sor_iters, '-- p error:', stats(q)%sor_error, &
+                     '-- residual:', stats(q)%residual
+        else
+          write(*,*) 'SOR iteration no.', s%iter_max, '-- p error:', stats(q)%sor_error
+        end if
       end do
       if (drag) call force_log()
       if (mod(istep, s%istep_out) == 0) call snapshot(istep)
