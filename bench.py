@@ -20,7 +20,10 @@ sweeps), projection and boundary update, all resident in HBM.  The line printed:
               the CPU restatement (oracle/pf_oracle.c) beside it; the restatement alone
               when the reference build is absent; and `single_thread`, the restatement
               on one thread for one step
-Further sections: mixed precision on the same workload and on the 512x256x256 boxes,
+Further sections: mixed precision on the same workload; the 512x256x256 boxes solved to
+the residual rule max|r/ap| <= 1e-8 every step (multigrid-preconditioned, FP64 and mixed,
+time to solution per step) with the HBM roofline of the FP64 colour sweep at that size; the
+2D cylinder benchmark case (256x128, 2000 steps) on the GPU and on the reference's build;
 the batched 2D workload, the served 3D rooms and the same rooms resident together.
 """
 from __future__ import annotations
@@ -56,9 +59,53 @@ def measured_traffic(kernel: str, workload: str):
     return d.get("bytes_per_launch")
 
 
-def cpu_baseline_reference(p, phi, steps: int, threads: int):
-    """Time the reference's own OpenMP build (oracle/_ref/fp64/ref_ibm_3d_uniform, compiled
-    from its sources with -fdefault-real-8) on the same case for steps + 1 steps.  Its
+def omp_env(threads: int):
+    """OpenMP settings of the CPU baseline: `threads` threads, one per CPU of the process's
+    affinity mask, packed.  (Places of whole cores made the reference's flang/libomp build
+    12x slower on the GPU host, 7.9 against 0.6 s per 3D step, and 1.4x slower in this
+    container; places of single CPUs were the fastest binding in both.)"""
+    return dict(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="threads")
+
+
+def host_cpus():
+    """CPUs this process may use (affinity mask, bounded by a cgroup CPU quota if one is
+    set), and a description of the host's processors."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    usable = min(aff, quota) if quota else aff
+    model, sockets, cores = None, set(), set()
+    try:
+        phys = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                    sockets.add(v)
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    return {"usable": usable, "affinity": aff, "cgroup_quota_cpus": quota, "model": model,
+            "sockets": len(sockets) or None, "physical_cores": len(cores) or None, "logical": os.cpu_count()}
+
+
+def cpu_baseline_reference(p, phi, steps: int, threads: int, main: str = "ref_ibm_3d_uniform"):
+    """Time the reference's own OpenMP build (oracle/_ref/fp64/<main>, compiled from its
+    sources with -fdefault-real-8) on the same case for steps + 1 steps.  Its
     output goes to a pseudo-terminal, so the Fortran runtime writes each log line as it
     happens; the time between the first and the last per-step 'SOR iteration' line is
     `steps` whole steps, without the CSV read before them or the output files after (the
@@ -69,10 +116,10 @@ def cpu_baseline_reference(p, phi, steps: int, threads: int):
     import select
     import subprocess
     import tempfile
-    exe = os.path.join(ROOT, "oracle", "_ref", "fp64", "ref_ibm_3d_uniform")
+    exe = os.path.join(ROOT, "oracle", "_ref", "fp64", main)
     if not os.access(exe, os.X_OK):
         return None
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    env = dict(os.environ, **omp_env(threads))
     q = dataclasses.replace(p, istep_max=steps + 1, time=p.dt * (steps + 1), istep_out=10 ** 6)
     stamps = []
     with tempfile.TemporaryDirectory() as root:
@@ -115,7 +162,7 @@ def cpu_baseline_reference(p, phi, steps: int, threads: int):
 def cpu_baseline(p, phi, steps: int, threads: int):
     """Time the oracle (CPU restatement of the reference) for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.update(omp_env(threads))
     import pyoracle  # noqa: E402  (test infrastructure: the baseline leg only)
     o = pyoracle.OracleRun(p, phi)
     o.step(1)                      # warm the pages and the thread pool
@@ -201,45 +248,77 @@ def _timed_run(s, steps: int, warmup: int):
     return wall, sweep_us, st
 
 
-def mixed_large(steps: int, warmup: int, inner: int):
+def tolerance_large(steps: int, warmup: int, tol: float):
     """The 512x256x256 boxes workload (40 seeded boxes, sides 4-32 cells, tanh voxel
     interface, U = 1), whose fields exceed the 256 MB last-level cache, in its stable form:
     cell Reynolds number 1 (xnue = U dx, dt from the scheme's stability limits) and boxes
-    clear of the inlet, the outlet region and the periodic faces.  The explicit step
-    diverges within 10 steps at the specified xnue = 5e-4 with boxes placed anywhere, and
-    within 50 with either change alone.  FP64 and mixed precision are timed on the same
-    case, fixed count (iter_max = 100), and the fields after warmup + steps compared."""
-    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    clear of the inlet, the outlet region and the periodic faces (as specified, with binary
+    porosity and xnue = 5e-4, the explicit step goes non-finite after a few steps).  Every
+    step solves the pressure to max|r/ap| <= tol: the multigrid-preconditioned solve in
+    FP64 and in mixed precision (FP32 inner solve, FP64 residual and update), timed per
+    step (time to solution), their fields compared after warmup + steps.  The fixed-count
+    FP64 path (iter_max = 100 SOR iterations, the reference's schedule, which leaves the
+    residual far above tol) is timed beside them; its colour sweep gives the HBM roofline
+    at a size the cache cannot hold."""
+    from pixelflow_amd.solver import PF_PREC_FP64, PF_PREC_MIXED, PF_SOLVER_MG, Solver
     p, phi = cases.boxes_case(steps=warmup + steps, cell_re=1.0, clear_faces=True)
-    log(f"[bench] mixed vs FP64 on boxes3d_{p.m}x{p.n}x{p.l}: {steps} steps, inner={inner}")
-    a = Solver(p, phi)
-    wa, ua, sa = _timed_run(a, steps, warmup)
-    b = Solver(p, phi, precision=PF_PREC_MIXED, inner_iters=inner, report_residual=True)
-    wb, ub, sb = _timed_run(b, steps, warmup)
-    errs = {}
-    for f in ("u", "v", "w", "p"):
-        x, y = a.interior(f), b.interior(f)
-        errs[f] = float(np.linalg.norm(y - x) / max(np.linalg.norm(x), 1e-300))
-    da, db = a.divergence(), b.divergence()
-    ddiv = float(np.max(np.abs(da - db)) / max(np.max(np.abs(da)), 1e-300))
-    a.close()
-    b.close()
-    cells = p.cells
-    va, vb = cells * steps / wa, cells * steps / wb
-    return {
-        "workload": f"boxes3d_{p.m}x{p.n}x{p.l}", "xnue": p.xnue, "dt": p.dt, "cell_reynolds": p.dx / p.xnue,
-        "steps": steps, "warmup": warmup, "iter_max": p.iter_max,
-        "inner_iters": inner, "mode": "fixed-count",
-        "fp64": {"value": va, "unit": "cell-steps/s", "ms_per_step": wa * 1e3 / steps,
-                 "sor_cell_updates_per_s": va * p.iter_max, "avg_colour_sweep_us": ua,
-                 "sweep_roofline_frac": 3 * 8 * cells / (ua * 1e-6) / 1e9 / HBM_PEAK_GBS},
-        "mixed": {"value": vb, "unit": "cell-steps/s", "ms_per_step": wb * 1e3 / steps,
-                  "sor_cell_updates_per_s": vb * p.iter_max, "avg_colour_sweep_us": ub,
-                  "sweep_roofline_frac": 3 * 4 * cells / (ub * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                  "final_residual": float(sb["residual"][-1])},
-        "speedup_vs_fp64": vb / va,
-        "field_rel_l2_vs_fp64": errs, "divergence_max_diff_rel_vs_fp64": ddiv,
-    }
+    log(f"[bench] boxes3d_{p.m}x{p.n}x{p.l}: {steps} steps to max|r/ap| <= {tol:g}, FP64 and mixed multigrid")
+    mg = dict(residual_tol=tol, solver=PF_SOLVER_MG, max_cycles=30)
+    res = {"workload": f"boxes3d_{p.m}x{p.n}x{p.l}", "xnue": p.xnue, "dt": p.dt, "cell_reynolds": p.dx / p.xnue,
+           "steps": steps, "warmup": warmup, "residual_tol": tol}
+    fields = {}
+    for label, prec in (("fp64", PF_PREC_FP64), ("mixed", PF_PREC_MIXED)):
+        with Solver(p, phi, precision=prec, **mg) as s:
+            wall, _, st = _timed_run(s, steps, warmup)
+            fields[label] = {f: s.interior(f) for f in ("u", "v", "w", "p")}
+            fields[label]["div"] = s.divergence()
+        res[label] = {"ms_per_step": wall * 1e3 / steps, "cell_steps_per_s": p.cells * steps / wall,
+                      "max_residual": float(np.max(st["residual"])),
+                      "vcycles_per_step": float(np.mean(st["sor_iters"])),
+                      "cycles_per_step": float(np.mean(st["cycles"]))}
+    res["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    a, b = fields["fp64"], fields["mixed"]
+    res["field_rel_l2_vs_fp64"] = {f: float(np.linalg.norm(b[f] - a[f]) / max(np.linalg.norm(a[f]), 1e-300))
+                                   for f in ("u", "v", "w", "p")}
+    res["divergence_max_diff_rel_vs_fp64"] = float(np.max(np.abs(b["div"] - a["div"])) /
+                                                   max(np.max(np.abs(a["div"])), 1e-300))
+    with Solver(p, phi) as s:
+        wall, sweep_us, _ = _timed_run(s, steps, warmup)
+    res["fixed_count_fp64"] = {"ms_per_step": wall * 1e3 / steps, "cell_steps_per_s": p.cells * steps / wall,
+                               "iter_max": p.iter_max, "avg_colour_sweep_us": sweep_us}
+    bytes_ = 3 * 8 * p.cells
+    achieved = bytes_ / (sweep_us * 1e-6) / 1e9
+    res["sweep_roofline"] = {"bound": "hbm", "kernel": "k_sor_phi (red/black colour sweep, FP64)",
+                             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS,
+                             "traffic": measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"),
+                             "bytes_per_launch_algorithmic": bytes_, "avg_launch_us": sweep_us}
+    return res
+
+
+def c1_cylinder(threads: int, cpu_steps: int):
+    """The 2D benchmark case in full: 256x128 cylinder, 2000 steps, FP64, on the GPU (bit
+    for bit the reference) and on the reference's OpenMP build on the host cores."""
+    from pixelflow_amd.solver import Solver
+    p, phi = cases.cylinder_case(steps=2000)
+    with Solver(p, phi) as s:
+        s.step(5)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(p.istep_max)
+        s.sync()
+        wall = time.perf_counter() - t0
+    out = {"workload": f"cylinder2d_{p.m}x{p.n}", "steps": p.istep_max, "iter_max": p.iter_max,
+           "gpu": {"cell_steps_per_s": p.cells * p.istep_max / wall, "wall_s": wall,
+                   "ms_per_step": wall * 1e3 / p.istep_max}}
+    if cpu_steps > 0:
+        log(f"[bench] C1 on the reference's OpenMP build: {cpu_steps} steps on {threads} threads")
+        ref = cpu_baseline_reference(p, phi, cpu_steps, threads, main="ref_ibm_2d_uniform")
+        if ref is not None:
+            out["cpu_reference"] = {"cell_steps_per_s": ref[0], "wall_s": ref[1], "steps": cpu_steps,
+                                    "threads": threads}
+            out["gpu_vs_cpu_reference"] = out["gpu"]["cell_steps_per_s"] / ref[0]
+    return out
 
 
 def main(argv=None):
@@ -248,7 +327,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (capped at 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every CPU this process may use")
     ap.add_argument("--cpu-steps-1t", type=int, default=1,
                     help="steps of the C restatement timed on one thread (0 = skip)")
     ap.add_argument("--cpu-ref-steps", type=int, default=5,
@@ -259,9 +338,16 @@ def main(argv=None):
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the mixed-precision comparison on the same workload (0 = skip)")
     ap.add_argument("--mixed-inner", type=int, default=0, help="FP32 iterations per defect-correction cycle")
-    ap.add_argument("--large-steps", type=int, default=10,
+    ap.add_argument("--large-steps", type=int, default=8,
                     help="steps of the FP64/mixed comparison on the 512x256x256 boxes workload (0 = skip)")
-    ap.add_argument("--large-inner", type=int, default=25, help="FP32 iterations per cycle on that workload")
+    ap.add_argument("--tol", type=float, default=1e-9,
+                    help="residual rule max|r/ap| <= tol of the 512x256x256 boxes section (<= 1e-8)")
+    ap.add_argument("--serve-tol", type=float, default=1e-10,
+                    help="residual rule of the served rooms and the rooms batch (<= 1e-8; tighter than the "
+                         "boxes' because a closed room's pressure level is fixed only through small outlets)")
+    ap.add_argument("--c1-cpu-steps", type=int, default=2000,
+                    help="steps of the 2D benchmark case on the reference's build (0 = skip the CPU leg)")
+    ap.add_argument("--c1", type=int, default=1, help="run the 2D benchmark case section (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=512,
                     help="cases of the batched 2D workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -353,9 +439,15 @@ def main(argv=None):
     }
     if args.mixed_steps > 0:
         res["mixed"] = mixed_comparison(p, phi, args.mixed_steps, args.warmup, args.mixed_inner, value)
+    cpus = host_cpus()
+    threads = args.cpu_threads or cpus["usable"]
     if args.large_steps > 0:
-        res["mixed_large"] = mixed_large(args.large_steps, 2, args.large_inner)
-        log(f"[bench] boxes 512x256x256: mixed {res['mixed_large']['speedup_vs_fp64']:.2f}x FP64")
+        res["tolerance_large"] = tolerance_large(args.large_steps, 2, args.tol)
+        res["roofline_hbm"] = res["tolerance_large"]["sweep_roofline"]
+        log(f"[bench] boxes 512x256x256 to the rule: mixed {res['tolerance_large']['speedup_vs_fp64']:.2f}x FP64; "
+            f"FP64 sweep at {res['roofline_hbm']['frac']:.2f} of HBM peak")
+    if args.c1:
+        res["c1"] = c1_cylinder(threads, args.c1_cpu_steps)
     if args.batch_cases > 0:
         from pixelflow_amd import workloads
         res["batch"] = workloads.batched_2d(count=args.batch_cases, steps=args.batch_steps, log=log)
@@ -364,15 +456,14 @@ def main(argv=None):
     if args.serve_requests > 0:
         from pixelflow_amd import workloads
         res["serve"] = workloads.served_rooms(n_requests=args.serve_requests, rate_per_s=args.serve_rate,
-                                              steps=args.serve_steps, log=log)
+                                              steps=args.serve_steps, tol=args.serve_tol, log=log)
         log(f"[bench] serve: p50 {res['serve']['p50_time_to_solution_s']:.2f} s, "
             f"p99 {res['serve']['p99_time_to_solution_s']:.2f} s")
     if args.rooms_batch > 0:
         from pixelflow_amd import workloads
-        res["rooms_batch"] = workloads.rooms_batch(n_requests=args.rooms_batch, log=log)
+        res["rooms_batch"] = workloads.rooms_batch(n_requests=args.rooms_batch, tol=args.serve_tol, log=log)
         log(f"[bench] rooms batch: mixed {res['rooms_batch']['batch_mixed_vs_alone_fp64']:.2f}x the rooms alone in FP64")
     if args.cpu_steps > 0:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps on {threads} threads")
         v, dt = cpu_baseline(p, phi, args.cpu_steps, threads)
         port = {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
@@ -383,12 +474,13 @@ def main(argv=None):
             ref = cpu_baseline_reference(p, phi, args.cpu_ref_steps, threads)
         if ref is not None:
             res["cpu_baseline"] = {"value": ref[0], "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+                                   "host": cpus, "omp": omp_env(threads),
                                    "sample": f"{args.cpu_ref_steps} steps of the same {workload} workload on the "
                                              f"reference's OpenMP build, FP64 ({ref[1]:.1f} s, from its per-step log "
                                              f"lines; CSV read and output files excluded)",
                                    "port": port}
         else:
-            res["cpu_baseline"] = port
+            res["cpu_baseline"] = dict(port, host=cpus, omp=omp_env(threads))
         if args.cpu_steps_1t > 0:
             log(f"[bench] CPU baseline: oracle, {args.cpu_steps_1t} step(s) on one thread")
             one = cpu_baseline_one_thread(args.m, args.n, args.l, args.cpu_steps_1t)

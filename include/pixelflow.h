@@ -227,6 +227,9 @@ int pf_serve_submit(pf_server *s, const pf_desc *desc, const double *raw_interio
 /* state (pf_serve_state), steps taken so far, SOR iterations over those steps, the
  * statistics of its latest step (pointers other than state may be NULL). */
 int pf_serve_poll(pf_server *s, int64_t id, int *state, int64_t *steps_done, int64_t *sor_iters, pf_stats *last);
+/* Per field, the maximum of the request's per-step statistics over the steps run so far
+ * (the worst residual shows whether every step met a tolerance; a NaN is kept). */
+int pf_serve_worst(pf_server *s, int64_t id, pf_stats *worst);
 /* Block until the request is done or failed, or timeout_s (< 0: no limit) passes. */
 int pf_serve_wait(pf_server *s, int64_t id, double timeout_s, int *state);
 int pf_serve_times(pf_server *s, int64_t id, double *t_submit, double *t_admit, double *t_finish);

@@ -468,6 +468,15 @@ static bool same_kind(const pf_desc &a, const pf_desc &b) {
            a.solver == b.solver && a.inner_tol == b.inner_tol;
 }
 
+// Wait for every stream of the batch (its own and the two side streams the sweep groups
+// and the multigrid solves run on) before tables, histories or graphs are replaced.
+static hipError_t drain(pf_batch *b) {
+    hipError_t e = b->st ? hipStreamSynchronize(b->st) : hipSuccess;
+    for (int q = 0; q < 2 && e == hipSuccess; ++q)
+        if (b->side[q]) e = hipStreamSynchronize(b->side[q]);
+    return e;
+}
+
 static void drop_graphs(pf_batch *b) {
     for (int p = 0; p < 2; ++p) {
         if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
@@ -615,6 +624,7 @@ static int tile_kb(const pf_batch *b) {
 }
 
 static int rebuild_tables(pf_batch *b) {
+    PFB_TRY(b, drain(b));   // launches in flight read the tables replaced below
     b->order.clear();
     for (int s = 0; s < b->max_cases; ++s)
         if (b->slots[(size_t)s].used) b->order.push_back(s);
@@ -894,8 +904,7 @@ extern "C" int pf_batch_remove(pf_batch *b, int slot) {
     if (!b || slot < 0 || slot >= b->max_cases || !b->slots[(size_t)slot].used) return PF_EINVAL;
     PFB_TRY(b, hipStreamSynchronize(b->st));
     pf_batch::Slot &S = b->slots[(size_t)slot];
-    for (int q = 0; q < 2; ++q)
-        if (b->side[q]) PFB_TRY(b, hipStreamSynchronize(b->side[q]));
+    PFB_TRY(b, drain(b));
     slot_release_mg(S);
     arena_free(b, S.off, S.bytes);
     S.used = false;
@@ -1108,7 +1117,7 @@ static void enqueue_step_fixed(pf_batch *b, int p) {
 
 static int ensure_hist(pf_batch *b, int rows) {
     if (b->hist_rows >= rows) return PF_OK;
-    PFB_TRY(b, hipStreamSynchronize(b->st));
+    PFB_TRY(b, drain(b));
     if (b->d_err_hist) (void)hipFree(b->d_err_hist);
     if (b->d_res_hist) (void)hipFree(b->d_res_hist);
     if (b->d_it_hist) (void)hipFree(b->d_it_hist);
@@ -1372,7 +1381,7 @@ extern "C" int pf_batch_set_config(pf_batch *b, const pf_batch_config *cfg) {
     if (!b || !cfg || cfg->persistent_blocks < 0 || cfg->inner_iters < 0) return PF_EINVAL;
     const int tx = cfg->tile_x == 0 ? (b->kind.dim == 3 ? 32 : 64) : cfg->tile_x;
     if (tx != 32 && tx != 64 && tx != 128) { b->msg = "tile_x must be 32, 64 or 128"; return PF_EINVAL; }
-    PFB_TRY(b, hipStreamSynchronize(b->st));
+    PFB_TRY(b, drain(b));
     const int kb_before = tile_kb(b);
     b->cfg = *cfg;
     b->cfg.tile_x = tx;

@@ -47,6 +47,7 @@ struct Request {
     int64_t sor_iters = 0;       // SOR iterations over all its steps
     double t_submit = 0, t_admit = -1, t_finish = -1;
     pf_stats last{};
+    pf_stats worst{};             // per field, the maximum over its steps (NaN kept)
     std::map<int, std::vector<double>> out;   // field -> (m+2)(n+2)(l+2) host array
     std::string error;
 };
@@ -183,6 +184,14 @@ void worker_loop(pf_server *s) {
                 }
                 for (int q = 0; q < n; ++q) r.sor_iters += st[(size_t)q * s->max_cases + kv.first].sor_iters;
                 r.last = st[(size_t)(n - 1) * s->max_cases + kv.first];
+                for (int q = 0; q < n; ++q) {
+                    const pf_stats &x = st[(size_t)q * s->max_cases + kv.first];
+                    auto worse = [](double a, double b) { return (b > a || b != b) ? b : a; };
+                    r.worst.sor_error = worse(r.worst.sor_error, x.sor_error);
+                    r.worst.residual = worse(r.worst.residual, x.residual);
+                    r.worst.sor_iters = std::max(r.worst.sor_iters, x.sor_iters);
+                    r.worst.cycles = std::max(r.worst.cycles, x.cycles);
+                }
                 if (!finite) diverged.push_back(kv.second);
                 else if (r.steps_done >= r.nsteps) finished.push_back(kv.second);
             }
@@ -293,6 +302,15 @@ extern "C" int pf_serve_poll(pf_server *s, int64_t id, int *state, int64_t *step
     if (steps_done) *steps_done = it->second->steps_done;
     if (sor_iters) *sor_iters = it->second->sor_iters;
     if (last) *last = it->second->last;
+    return PF_OK;
+}
+
+extern "C" int pf_serve_worst(pf_server *s, int64_t id, pf_stats *worst) {
+    if (!s || !worst) return PF_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return PF_EINVAL;
+    *worst = it->second->worst;
     return PF_OK;
 }
 

@@ -67,7 +67,7 @@ EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set
            "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
            "pf_batch_divergence", "pf_batch_set_config", "pf_batch_get_config", "pf_batch_sync", "pf_batch_case_steps",
            "pf_device_memory", "pf_serve_create", "pf_serve_destroy", "pf_serve_batch", "pf_serve_submit",
-           "pf_serve_poll", "pf_serve_wait", "pf_serve_times", "pf_serve_result", "pf_serve_release",
+           "pf_serve_poll", "pf_serve_worst", "pf_serve_wait", "pf_serve_times", "pf_serve_result", "pf_serve_release",
            "pf_serve_counters", "pf_serve_error")
 
 
@@ -129,6 +129,7 @@ def lib() -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_int64)]
     L.pf_serve_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(PfStats)]
+    L.pf_serve_worst.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(PfStats)]
     L.pf_serve_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
     L.pf_serve_times.argtypes = [vp, ctypes.c_int64, dp, dp, dp]
     L.pf_serve_result.argtypes = [vp, ctypes.c_int64, ctypes.c_int, dp, i64, i64]
@@ -465,6 +466,12 @@ class NativeServer:
                                         ctypes.byref(last)), "pf_serve_poll", rid)
         return st.value, done.value, its.value, {"sor_error": last.sor_error, "residual": last.residual,
                                                  "sor_iters": last.sor_iters, "cycles": last.cycles}
+
+    def worst(self, rid: int) -> Dict[str, float]:
+        """Per field, the maximum of the request's per-step statistics so far."""
+        w = PfStats()
+        self._check(lib().pf_serve_worst(self._h, int(rid), ctypes.byref(w)), "pf_serve_worst", rid)
+        return {"sor_error": w.sor_error, "residual": w.residual, "sor_iters": w.sor_iters, "cycles": w.cycles}
 
     def wait(self, rid: int, timeout: float = -1.0) -> int:
         st = ctypes.c_int()

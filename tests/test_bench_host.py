@@ -43,3 +43,20 @@ def test_cpu_baseline_one_thread_runs_in_a_child_process():
     assert res is not None
     rate, seconds = res
     assert seconds > 0 and rate == pytest.approx(32 * 16 * 16 / seconds)
+
+
+@pytest.mark.skipif(not os.access(os.path.join(ROOT, "oracle", "_ref", "fp64", "ref_ibm_2d_uniform"), os.X_OK),
+                    reason="the reference's OpenMP build is not built here (make -C oracle ref)")
+def test_cpu_baseline_times_reference_2d_build():
+    """The 2D benchmark case's CPU leg uses the 2D executable of the reference build."""
+    import bench
+    from pixelflow_amd import cases
+    p, phi = cases.cylinder_case(m=64, n=32, steps=6, iter_max=20)
+    res = bench.cpu_baseline_reference(p, phi, 4, 2, main="ref_ibm_2d_uniform")
+    assert res is not None and res[1] > 0 and res[0] == pytest.approx(p.cells * 4 / res[1])
+
+
+def test_host_cpus_reports_usable_cpus():
+    import bench
+    h = bench.host_cpus()
+    assert 1 <= h["usable"] <= h["affinity"]
