@@ -101,13 +101,50 @@ def build_fortran() -> List[str]:
     return exes
 
 
+ASAN_DIR = os.path.join(OUT, "asan")
+ASAN_DRIVER = os.path.join(ASAN_DIR, "batch_lifecycle")
+# host AddressSanitizer only: each -fsanitize goes to the host compilation (the device code
+# is built as in the library)
+ASAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def build_asan_driver(jobs: int = 4) -> str:
+    """tests/native/batch_lifecycle.cpp linked with every library source, host code
+    instrumented with AddressSanitizer (a standalone executable: the sanitizer runtime is
+    linked in, nothing is preloaded)."""
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES] + [os.path.join(ROOT, "tests", "native", "batch_lifecycle.cpp")]
+    objs, procs = [], []
+    for src in srcs:
+        obj = os.path.join(ASAN_DIR, os.path.splitext(os.path.basename(src))[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") and "native" in src else []
+        cmd = [hipcc(), *HIPCC_FLAGS, *ASAN_HOST, "-I" + os.path.join(ROOT, "include"), *lang, "-c", src, "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+        if len(procs) >= jobs:
+            for p in procs:
+                if p.wait() != 0:
+                    raise subprocess.CalledProcessError(p.returncode, "hipcc")
+            procs = []
+    for p in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, "hipcc")
+    _run([hipcc(), f"--offload-arch={ARCH}", *ASAN_HOST, *objs, "-pthread", "-o", ASAN_DRIVER])
+    return ASAN_DRIVER
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib-only", action="store_true")
+    ap.add_argument("--asan", action="store_true",
+                    help="also build the host-AddressSanitizer lifecycle driver (tests/native)")
     args = ap.parse_args(argv)
     build_lib()
     if not args.lib_only:
         build_fortran()
+    if args.asan:
+        build_asan_driver()
     return 0
 
 

@@ -195,11 +195,23 @@ __global__ __launch_bounds__(256) void k_sor(PfGeom g, StepArgs a, SweepArgs s, 
 // Max of the partial slots of one step into that step's entry; the slots are cleared
 // for the next step.  One block.
 __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned long long *step_err) {
+    // eight independent loads in flight per thread before any slot is cleared (a sweep of
+    // the 256x128x128 grid leaves 32768 wave slots: four rounds instead of 32 dependent ones)
+    constexpr int U = 8;
     unsigned long long e = 0ull;
-    for (int t = threadIdx.x; t < nslots; t += blockDim.x) {
-        const unsigned long long v = slots[t];
-        e = (v > e) ? v : e;
-        slots[t] = 0ull;
+    for (int base = threadIdx.x; base < nslots; base += U * blockDim.x) {
+        unsigned long long v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = base + u * (int)blockDim.x;
+            v[u] = (t < nslots) ? slots[t] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = base + u * (int)blockDim.x;
+            e = (v[u] > e) ? v[u] : e;
+            if (t < nslots) slots[t] = 0ull;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o2 = __shfl_xor(e, off, 64);

@@ -97,9 +97,11 @@ struct MgCase {
     int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings
 };
 
-// Tuning of the V-cycle (fixed; the same for every case).
+// Tuning of the V-cycle, the same for every case (PF_MG_NU, PF_MG_NUC, PF_MG_ALPHA and
+// PF_MG_CS in the environment override it when a solver is created).
 struct MgParams {
-    int nu1 = 2, nu2 = 2;            // red/black sweeps before and after the coarse correction
+    int nu1 = 2, nu2 = 2;            // level 0: red/black sweeps before and after the coarse correction
+    int nuc = 2;                     // coarse levels: sweeps before and after
     double alpha = 1.6;              // over-correction of the piecewise-constant interpolation
     int coarsest_sweeps = 60;        // red/black sweeps of the coarsest level
 };

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -723,6 +724,13 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
         if (why) *why = "multigrid: coarsest level has more cells than one block holds";
         return false;
     }
+    {
+        const char *v;
+        if ((v = std::getenv("PF_MG_NU")) && *v) S.prm.nu1 = S.prm.nu2 = std::max(1, std::atoi(v));
+        if ((v = std::getenv("PF_MG_NUC")) && *v) S.prm.nuc = std::max(1, std::atoi(v));
+        if ((v = std::getenv("PF_MG_CS")) && *v) S.prm.coarsest_sweeps = std::max(1, std::atoi(v));
+        if ((v = std::getenv("PF_MG_ALPHA")) && *v) S.prm.alpha = std::atof(v);
+    }
     S.bytes = mg_bytes_t(g, c.a.bc, sizeof(T));
     S.own_mem = (mem == nullptr);
     if (S.own_mem) {
@@ -892,7 +900,7 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
         hipLaunchKernelGGL((k_mg_smooth<T>), dim3(nb), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e, colour);
         if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
     };
-    for (int s = 0; s < S.prm.nu1; ++s) {
+    for (int s = 0; s < S.prm.nuc; ++s) {
         smooth(0);
         smooth(1);
     }
@@ -903,7 +911,7 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
     hipLaunchKernelGGL((k_mg_prolong<T>), dim3(nb), dim3(256), 0, st, g, (const uint8_t *)L.keep, C.g,
                        (const T *)C.e, L.e, (T)S.prm.alpha);
     if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
-    for (int s = 0; s < S.prm.nu2; ++s) {
+    for (int s = 0; s < S.prm.nuc; ++s) {
         smooth(1);
         smooth(0);
     }

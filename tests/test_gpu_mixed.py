@@ -4,11 +4,15 @@ The FP64 path is bit-identical to the reference (test_gpu_parity.py), so it is t
 oracle here.
 
 * Tolerance mode (cycles until the FP64 residual max|r/ap| <= tol) against the FP64 path
-  run to the same stopping rule: relative L2 error <= 1e-6 in every field.
+  run to the same stopping rule: relative L2 error <= 1e-6 in every field, pressure
+  included -- with SOR inner sweeps on a small sphere, and with the multigrid solver on the
+  512x256x256 boxes at full size.
 * Fixed-count mode (the reference's iter_max, cycles of K FP32 iterations): velocities
   within 1e-6 (relative L2), the output_divergent diagnostic within 1e-6 of its own
-  magnitude; the pressure of an unconverged SOR iterate carries FP32 rounding that grows
-  with K (measured 1e-4 at K = 100, 1e-5 at K = 10 on the 2D cylinder), held to 2e-4.
+  magnitude.  The pressure of an unconverged SOR iterate is not a solution to compare:
+  after iter_max sweeps its residual is still 1e-3..1e-2, and FP32 rounding of the
+  correction moves it by up to 1e-4 (K = 100) or 1e-5 (K = 10) on the 2D cylinder, so
+  fixed-count p is held only to 2e-4.  The 1e-6 bar on p is met in tolerance mode.
   "Final divergence <= 1e-8" is the FP64 Poisson residual max|r/ap|, which tolerance mode
   drives below 1e-8 (checked in the last test).
 """
@@ -94,6 +98,28 @@ def test_mixed_tolerance_mode_matches_fp64_tolerance_mode():
             rel = np.linalg.norm(y - x) / np.linalg.norm(x)
             assert rel <= FIELD_TOL, (f, rel)
         print(f"[gpu] tolerance mode iterations fp64 {sa['sor_iters']} mixed {sb['sor_iters']}", flush=True)
+
+
+def test_mixed_tolerance_mode_multigrid_boxes_512x256x256():
+    """The 512x256x256 boxes (40 boxes, tanh voxel interface, cell Reynolds number 1) solved
+    every step to max|r/ap| <= 1e-9 by the multigrid-preconditioned solver, in FP64 and in
+    mixed precision (FP32 inner solve of the defect equation inside FP64 residual and
+    update): both reach the rule every step, and every field, pressure included, agrees to
+    1e-6 rel-L2."""
+    from pixelflow_amd import solver as S
+    p, phi = cases.boxes_case(steps=2, cell_re=1.0, clear_faces=True)
+    tol = 1e-9
+    kw = dict(residual_tol=tol, solver=S.PF_SOLVER_MG, max_cycles=30)
+    print("[gpu] multigrid tolerance mode, 512x256x256 boxes, FP64 and mixed, 2 steps", flush=True)
+    with S.Solver(p, phi, **kw) as a, S.Solver(p, phi, precision=S.PF_PREC_MIXED, **kw) as b:
+        sa = a.step(2)
+        sb = b.step(2)
+        assert np.all(sa["residual"] <= tol) and np.all(sb["residual"] <= tol), (sa["residual"], sb["residual"])
+        for f in ("u", "v", "w", "p"):
+            x, y = a.interior(f), b.interior(f)
+            rel = float(np.linalg.norm(y - x) / np.linalg.norm(x))
+            print(f"[gpu] 512x256x256 {f} rel-L2 {rel:.2e}", flush=True)
+            assert rel <= FIELD_TOL, (f, rel)
 
 
 def test_mixed_boxes_512x256x256_full_size():

@@ -141,3 +141,57 @@ def test_force_closed_forms():
     flat = pyoracle.OracleRun(p, np.ones((20, 40)))
     flat.p_[...] = 5.0
     assert np.all(flat.force() == 0.0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_force_3d_plane_wall_closed_forms(axis):
+    """The 3D branch (output.f90:1090-1165) on a plane interface normal to one axis,
+    porosity 0.5*tanh(s/T)+0.5 with T = thickness cells.  Since dphi/ds = (2/T) phi(1-phi),
+    the pressure term sums to -p times the wall's area (the integral of dphi/ds is the jump,
+    1), and the wall term to alpha*rho*nu*u_b * A * d_axis / (12 T d_b^2) (the integral of
+    (dphi/ds)^2 is 1/(3T)).  The cell spacings differ in x, y and z, so a swapped spacing or
+    a dropped factor fails by 10% or more; the tolerances are the Riemann sums' own error
+    over a 40-cell profile (1e-5 and 1.6e-4)."""
+    from pixelflow_amd import cases
+    vel, P, T = np.array([0.7, -0.4, 1.1]), 2.5, 1.5
+    ext = [10, 8, 6]
+    ext[axis] = 40
+    m, n, l = ext
+    p = cases.CaseParams(dim=3, m=m, n=n, l=l, width=1.0, height=0.6, depth=0.45, xnue=2e-3, density=1.3,
+                         thickness=T, time=1e-3, istep_max=1, iter_max=5)
+    prof = cases.tanh_profile(np.arange(ext[axis]) - (ext[axis] / 2 + 0.3), T)
+    shape = [1, 1, 1]
+    shape[2 - axis] = ext[axis]
+    o = pyoracle.OracleRun(p, np.broadcast_to(prof.reshape(shape), (l, n, m)).copy())
+    d = np.array([o.case.dx, o.case.dy, o.case.dz])
+    o.p_[...] = P
+    o.u[...], o.v[...], o.w[...] = vel
+    f = o.force()
+    area = np.prod([ext[a] * d[a] for a in range(3) if a != axis])
+    assert abs(f[axis] / (-P * area) - 1) < 5e-5
+    assert all(f[a] == 0.0 for a in range(3) if a != axis)      # no gradient across the other axes
+    wall = p.density * p.xnue * 32.0 * vel * area * d[axis] / (12 * T * d ** 2)
+    np.testing.assert_allclose(f[3:], wall, rtol=5e-4)
+
+
+def test_force_3d_sphere_symmetry_and_linearity():
+    """Constant pressure on a sphere centred in a box that is mirror-symmetric about the
+    centre in every axis: the pressure force vanishes in x, y and z (opposite faces cancel);
+    the wall term is linear in each velocity component and sees only its own one."""
+    from pixelflow_amd import cases
+    m, n, l = 24, 20, 16
+    p = cases.CaseParams(dim=3, m=m, n=n, l=l, width=1.0, height=0.8, depth=0.6, time=1e-3, istep_max=1, iter_max=5)
+    phi = cases.sphere_3d(m, n, l, 1.0, 0.8, 0.6, (0.5, 0.4, 0.3), 0.35)
+    o = pyoracle.OracleRun(p, phi)
+    o.p_[...] = 3.0
+    unit = []
+    for b in range(3):
+        vel = [0.0, 0.0, 0.0]
+        vel[b] = 1.0
+        o.u[...], o.v[...], o.w[...] = vel
+        f = o.force()
+        assert np.all(np.abs(f[:3]) < 1e-12)                  # |each term| is up to about 0.1
+        assert f[3 + b] > 0 and all(f[3 + a] == 0.0 for a in range(3) if a != b)
+        unit.append(f[3 + b])
+    o.u[...], o.v[...], o.w[...] = 2.0, -1.0, 0.5
+    np.testing.assert_allclose(o.force()[3:], np.array(unit) * [2.0, -1.0, 0.5], rtol=1e-14)
