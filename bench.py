@@ -37,6 +37,20 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+
+# The CPUs this process may use, read before anything binds a thread.  The in-process CPU
+# baseline binds OpenMP threads (OMP_PROC_BIND), which pins the calling thread to one CPU;
+# a child started from it afterwards would inherit that single CPU, so every child is given
+# this full set back.
+try:
+    PROCESS_CPUS = sorted(os.sched_getaffinity(0))
+except (AttributeError, OSError):
+    PROCESS_CPUS = None
+
+
+def _full_affinity():
+    if PROCESS_CPUS:
+        os.sched_setaffinity(0, PROCESS_CPUS)
 sys.path.insert(0, ROOT)
 
 from pixelflow_amd import cases  # noqa: E402
@@ -61,19 +75,16 @@ def measured_traffic(kernel: str, workload: str):
 
 def omp_env(threads: int):
     """OpenMP settings of the CPU baseline: `threads` threads, one per CPU of the process's
-    affinity mask, packed.  (Places of whole cores made the reference's flang/libomp build
-    12x slower on the GPU host, 7.9 against 0.6 s per 3D step, and 1.4x slower in this
-    container; places of single CPUs were the fastest binding in both.)"""
+    affinity mask, packed.  Measured on the GPU host's 16 CPUs with the reference's 3D build
+    (seconds per step): unbound 0.57, close/threads 0.58, spread/threads 0.55, close on an
+    explicit list of the 16 CPUs 0.54, close/cores 0.72."""
     return dict(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="threads")
 
 
 def host_cpus():
     """CPUs this process may use (affinity mask, bounded by a cgroup CPU quota if one is
     set), and a description of the host's processors."""
-    try:
-        aff = len(os.sched_getaffinity(0))
-    except (AttributeError, OSError):
-        aff = os.cpu_count() or 1
+    aff = len(PROCESS_CPUS) if PROCESS_CPUS else (os.cpu_count() or 1)
     quota = None
     try:
         with open("/sys/fs/cgroup/cpu.max") as f:
@@ -125,7 +136,8 @@ def cpu_baseline_reference(p, phi, steps: int, threads: int, main: str = "ref_ib
     with tempfile.TemporaryDirectory() as root:
         cases.write_case_dir(root, q, phi)
         master, slave = pty.openpty()
-        proc = subprocess.Popen([exe], cwd=root, stdin=subprocess.DEVNULL, stdout=slave, stderr=slave, env=env)
+        proc = subprocess.Popen([exe], cwd=root, stdin=subprocess.DEVNULL, stdout=slave, stderr=slave, env=env,
+                                preexec_fn=_full_affinity)
         os.close(slave)
         buf = b""
         deadline = time.time() + 600
@@ -186,7 +198,8 @@ def cpu_baseline_one_thread(m: int, n: int, l: int, steps: int):
             % (ROOT, os.path.join(ROOT, "oracle"), m, n, l, steps, steps))
     env = dict(os.environ, OMP_NUM_THREADS="1")
     try:
-        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300,
+                             preexec_fn=_full_affinity)
     except subprocess.TimeoutExpired:
         return None
     if out.returncode != 0:

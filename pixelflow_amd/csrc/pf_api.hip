@@ -48,6 +48,13 @@ struct pf_ctx {
     unsigned long long *res_part = nullptr;  // mixed: per-wave max |r/ap|
     unsigned long long *res_val = nullptr;   // mixed: reduced residual of the last evaluation
     float *E32 = nullptr, *R32 = nullptr, *PHI32 = nullptr;   // mixed: colour-split FP32 arrays
+    // FP64 sweeps: the colour-split porosity coded in 4 bytes per value (PHC), with the
+    // table of the values FP32 cannot hold (PHTAB, room for PHI_TAB_MAX); used when
+    // phi_coded (PF_PHI_CODE=1 in the environment turns it on: at 256x128x128 the coded
+    // sweep took 18.5 against 17.9 us)
+    float *PHC = nullptr;
+    double *PHTAB = nullptr;
+    bool phi_code_on = false, phi_coded = false;
     int err_cap = 0, nparts = 0;
     bool have_phi = false, have_init = false;
     bool timing = false;
@@ -65,6 +72,10 @@ struct pf_ctx {
 };
 
 static thread_local std::string g_create_error;
+
+// Distinct porosity values that FP32 cannot hold, at most, for the coded sweep (8 MB table;
+// a benchmark sphere has about 10^4, the 512x256x256 boxes a few hundred).
+static const size_t PHI_TAB_MAX = (size_t)1 << 20;
 
 // On a failure the context's stream is drained before the early return, so no asynchronous
 // copy from or into a host buffer of the calling function is still in flight afterwards.
@@ -98,7 +109,11 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     const int64_t mg = (d->residual_tol > 0 && d->solver == PF_SOLVER_MG)
                            ? pf::mg_bytes(g, d->bc, d->precision == PF_PREC_MIXED)
                            : 0;
-    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + 4096;
+    const char *pc = std::getenv("PF_PHI_CODE");
+    const bool code_on = pc && *pc && std::atoi(pc) != 0;
+    const int64_t coded =
+        (d->precision == PF_PREC_FP64 && code_on) ? (int64_t)4 * 2 * g.HS + (int64_t)8 * PHI_TAB_MAX : 0;
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + 4096;
 }
 
 static int alloc(pf_ctx *c, double **p, int64_t n) {
@@ -121,9 +136,10 @@ static void release(pf_ctx *c) {
     if (c->err_part) { (void)hipFree(c->err_part); c->err_part = nullptr; }
     if (c->res_part) { (void)hipFree(c->res_part); c->res_part = nullptr; }
     if (c->res_val) { (void)hipFree(c->res_val); c->res_val = nullptr; }
-    float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2};
+    float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2, &c->PHC};
     for (float **p : f32)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (c->PHTAB) { (void)hipFree(c->PHTAB); c->PHTAB = nullptr; }
     if (c->mg32) { pf::mg_destroy(*c->mg32); delete c->mg32; c->mg32 = nullptr; }
     if (c->mg64) { pf::mg_destroy(*c->mg64); delete c->mg64; c->mg64 = nullptr; }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
@@ -199,6 +215,17 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
             }
         }
     }
+    {
+        const char *pc = std::getenv("PF_PHI_CODE");
+        if (pc && *pc) c->phi_code_on = std::atoi(pc) != 0;
+    }
+    if (desc->precision == PF_PREC_FP64 && c->phi_code_on) {
+        if (hipMalloc((void **)&c->PHC, sizeof(float) * (size_t)(2 * g.HS)) != hipSuccess ||
+            hipMalloc((void **)&c->PHTAB, sizeof(double) * PHI_TAB_MAX) != hipSuccess) {
+            rc = fail(c, PF_ENOMEM, "hipMalloc of the coded porosity failed");
+            goto bad;
+        }
+    }
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         rc = fail(c, PF_EHIP, "stream synchronize failed after allocation");
         goto bad;
@@ -236,6 +263,8 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         mc.phi = c->phi;
         mc.PHIs = c->PHIs;
         mc.PHI32 = c->PHI32;
+        mc.PHC = c->PHC;
+        mc.PHTAB = c->PHTAB;
         mc.kb32 = c->sweep_kb32;
         mc.kb64 = c->sweep_kb;
         mc.xw = c->sweep_xw;
@@ -304,9 +333,27 @@ static int derive_from_porosity(pf_ctx *c) {
     PF_TRY_HIP(c, hipGetLastError());
     c->sor_variant = (hbad == 0) ? 2 : 1;
     if (c->sor_forced >= 0 && c->sor_forced <= 2 && !(c->sor_forced == 2 && hbad != 0)) c->sor_variant = c->sor_forced;
+    c->phi_coded = false;
+    if (c->PHC && c->PHTAB) {
+        std::vector<double> h((size_t)(2 * g.HS));
+        PF_TRY_HIP(c, hipMemcpyAsync(h.data(), c->PHIs, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        std::vector<float> code;
+        std::vector<double> tab;
+        if (pf::code_porosity(h, code, tab, PHI_TAB_MAX)) {
+            PF_TRY_HIP(c, hipMemcpy(c->PHC, code.data(), code.size() * sizeof(float), hipMemcpyHostToDevice));
+            if (!tab.empty())
+                PF_TRY_HIP(c, hipMemcpy(c->PHTAB, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+            c->phi_coded = true;
+        }
+    }
     // the multigrid operators follow the porosity (rebuilt at the next solve)
     if (c->mg32) { c->mg32->c.fast = (c->sor_variant == 2); c->mg32->built = false; }
-    if (c->mg64) { c->mg64->c.fast = (c->sor_variant == 2); c->mg64->built = false; }
+    if (c->mg64) {
+        c->mg64->c.fast = (c->sor_variant == 2);
+        c->mg64->c.coded = c->phi_coded;
+        c->mg64->built = false;
+    }
     c->have_phi = true;
     return PF_OK;
 }
@@ -441,7 +488,8 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
             if (c->sor_variant == 0)
                 pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
             else
-                pf::launch_sor_phi(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, slot, c->sor_variant == 2);
+                pf::launch_sor_phi(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, slot, c->sor_variant == 2,
+                                   c->phi_coded ? c->PHC : nullptr, c->PHTAB);
             if (ry || rz) pf::launch_ghost_refresh(c->st, g, ry, rz, c->P);
         }
     }
@@ -727,6 +775,8 @@ extern "C" int pf_last_sweep_time_us(const pf_ctx *c, double *avg_us, int64_t *l
 }
 
 extern "C" int pf_sor_kernel(const pf_ctx *c) { return c ? c->sor_variant : PF_EINVAL; }
+
+extern "C" int pf_sor_coded(const pf_ctx *c) { return c ? (c->phi_coded ? 1 : 0) : PF_EINVAL; }
 
 // Bandwidth calibration: copies `bytes` (rounded down to 16) device to device `iters` times
 // on a private stream and returns the average GB/s (read + write bytes per second).

@@ -396,6 +396,52 @@ __device__ __forceinline__ void sweep_gather(const PfGeom &g, int sh, int64_t o,
     }
 }
 
+// Porosity coded in 4 bytes, exactly: a value that FP32 holds exactly is stored as that
+// float; any other value as a quiet NaN whose low 22 bits index a table of the exact FP64
+// values.  The decoded double is the stored double, bit for bit.
+constexpr uint32_t PHI_CODE_INDEX_BITS = 22;
+__device__ __forceinline__ double phi_decode(float c, const double *__restrict__ tab) {
+    return (c == c) ? (double)c : gmem(tab)[__float_as_uint(c) & ((1u << PHI_CODE_INDEX_BITS) - 1u)];
+}
+
+// sweep_gather for the FP64 sweep with coded porosity (same loads, 4-byte porosity).
+template <int DIM>
+__device__ __forceinline__ void sweep_gather_coded(const PfGeom &g, int sh, int64_t o, const double *__restrict__ Xoth,
+                                                   const double *__restrict__ Xown, const float *__restrict__ Co,
+                                                   const float *__restrict__ Cx, const double *__restrict__ tab,
+                                                   const double *__restrict__ Ro, SweepIn<double> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    const auto X = gmem(Xoth);
+    const auto C = gmem(Cx);
+    x.c = gmem(Xown)[o];
+    x.w = X[o - 1 + sh];
+    x.e = X[o + sh];
+    x.s = X[o - HP];
+    x.n = X[o + HP];
+    const float cc = gmem(Co)[o], cw = C[o - 1 + sh], ce = C[o + sh], cs = C[o - HP], cn = C[o + HP];
+    x.rhs = gmem(Ro)[o];
+    float cb = 0.f, ct = 0.f;
+    if (DIM == 3) {
+        x.b = X[o - HPY];
+        x.t = X[o + HPY];
+        cb = C[o - HPY];
+        ct = C[o + HPY];
+    } else {
+        x.b = x.t = 0.;
+    }
+    x.fc = phi_decode(cc, tab);
+    x.fw = phi_decode(cw, tab);
+    x.fe = phi_decode(ce, tab);
+    x.fs = phi_decode(cs, tab);
+    x.fn = phi_decode(cn, tab);
+    if (DIM == 3) {
+        x.fb = phi_decode(cb, tab);
+        x.ft = phi_decode(ct, tab);
+    } else {
+        x.fb = x.ft = 0.;
+    }
+}
+
 // Store a new value at the cell and, with an even periodic extent, at the ghost slot that
 // mirrors it (same colour; no cell of the sweep reads it).
 template <int DIM, typename T>

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "pf_layout.h"
 
 #ifndef PF_BC_UNIFORM
@@ -47,9 +49,15 @@ void launch_rhs(hipStream_t st, const PfGeom &g, const StepArgs &a, const double
 void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                 const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
                 unsigned long long *err_slot);
+// PHC, tab: porosity coded in 4 bytes (phi_decode in pf_cells.h); nullptr reads PHIs.
 void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                     const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
-                    bool fast);
+                    bool fast, const float *PHC = nullptr, const double *tab = nullptr);
+// Code a colour-split FP64 porosity (2 * HS values) in 4 bytes per value: `code` receives
+// the floats, `tab` the distinct values FP32 cannot hold.  Returns false when there are
+// more of those than the code's index holds (`max_tab`), leaving the outputs unspecified.
+bool code_porosity(const std::vector<double> &phis, std::vector<float> &code, std::vector<double> &tab,
+                   size_t max_tab);
 void launch_faces_check_direct(hipStream_t st, const PfGeom &g, const double *phi, const StepArgs &a, int *bad);
 void launch_faces_check(hipStream_t st, const PfGeom &g, const double *phi, const double *AW, const double *AS,
                         const double *AB, const StepArgs &a, int *bad);

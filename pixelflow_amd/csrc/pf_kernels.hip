@@ -6,6 +6,11 @@
 // HBM-bandwidth bound stencils: one thread per cell, x-contiguous wave rows, no MFMA.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
 #include "pf_cells.h"
 #include "pf_device.h"
 #include "pf_fused.h"
@@ -231,11 +236,14 @@ __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned lon
 // porosity (helpers in pf_device.h).
 // One cell of the sweep (FP64, faces rebuilt from porosity); returns |p_new - p_old|.
 
-template <int DIM, bool FAST, int KB>
+// PT = double: porosity as stored FP64; PT = float: porosity coded in 4 bytes
+// (phi_decode, table `tab`), the same doubles for 4 fewer bytes per cell and launch.
+template <int DIM, bool FAST, int KB, typename PT>
 __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
-                                                 double *__restrict__ Pown, const double *__restrict__ PHIo,
-                                                 const double *__restrict__ PHIx, const double *__restrict__ BBo,
-                                                 const double *__restrict__ phi, unsigned long long *err_slot) {
+                                                 double *__restrict__ Pown, const PT *__restrict__ PHIo,
+                                                 const PT *__restrict__ PHIx, const double *__restrict__ BBo,
+                                                 const double *__restrict__ phi, unsigned long long *err_slot,
+                                                 const double *__restrict__ tab) {
     const SweepThread t = sweep_thread<DIM, KB>(g, s);
     double err = 0.;
     if (t.j <= g.n && t.i <= g.m) {
@@ -246,7 +254,10 @@ __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs
             // a plane past the last one loads plane l instead (in bounds) and is not updated
             const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
             o[q] = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
-            sweep_gather<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, BBo, x[q]);
+            if constexpr (std::is_same<PT, float>::value)
+                sweep_gather_coded<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, tab, BBo, x[q]);
+            else
+                sweep_gather<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, BBo, x[q]);
         }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
@@ -558,7 +569,7 @@ void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepA
 
 void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                     const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
-                    bool fast) {
+                    bool fast, const float *PHC, const double *tab) {
     const int c = s.colour;
     const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
     const int KB = (g.dim == 3) ? s.kb : 1;
@@ -568,9 +579,14 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
     s1.gx = sp.gx;
     s1.gy = sp.gy;
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
-#define PF_SOR_PHI(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
-                                               PHIs + own, PHIs + oth, BB + own, phi, err_slot)
-    if (g.dim == 3) {
+#define PF_SOR_PHI(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K, double>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
+                                               PHIs + own, PHIs + oth, BB + own, phi, err_slot, (const double *)nullptr)
+#define PF_SOR_PHC(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K, float>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
+                                               PHC + own, PHC + oth, BB + own, phi, err_slot, tab)
+    if (PHC && tab && KB == 1) {
+        if (g.dim == 3) { if (fast) PF_SOR_PHC(3, true, 1); else PF_SOR_PHC(3, false, 1); }
+        else { if (fast) PF_SOR_PHC(2, true, 1); else PF_SOR_PHC(2, false, 1); }
+    } else if (g.dim == 3) {
         if (fast) {
             if (KB == 1) PF_SOR_PHI(3, true, 1); else if (KB == 2) PF_SOR_PHI(3, true, 2); else PF_SOR_PHI(3, true, 4);
         } else {
@@ -580,6 +596,38 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
         if (fast) PF_SOR_PHI(2, true, 1); else PF_SOR_PHI(2, false, 1);
     }
 #undef PF_SOR_PHI
+#undef PF_SOR_PHC
+}
+
+bool code_porosity(const std::vector<double> &phis, std::vector<float> &code, std::vector<double> &tab,
+                   size_t max_tab) {
+    const size_t cap = std::min(max_tab, (size_t)1 << PHI_CODE_INDEX_BITS);
+    code.resize(phis.size());
+    tab.clear();
+    std::unordered_map<uint64_t, uint32_t> index;
+    for (size_t q = 0; q < phis.size(); ++q) {
+        const double v = phis[q];
+        const float f = (float)v;
+        if ((double)f == v) {   // exact in FP32 (a NaN never compares equal: it goes to the table)
+            code[q] = f;
+            continue;
+        }
+        uint64_t bits;
+        std::memcpy(&bits, &v, sizeof bits);
+        auto it = index.find(bits);
+        uint32_t id;
+        if (it == index.end()) {
+            if (tab.size() >= cap) return false;
+            id = (uint32_t)tab.size();
+            index.emplace(bits, id);
+            tab.push_back(v);
+        } else {
+            id = it->second;
+        }
+        const uint32_t cbits = 0x7FC00000u | id;   // quiet NaN, index in the low bits
+        std::memcpy(&code[q], &cbits, sizeof cbits);
+    }
+    return true;
 }
 
 void launch_faces_check_direct(hipStream_t st, const PfGeom &g, const double *phi, const StepArgs &a, int *bad) {

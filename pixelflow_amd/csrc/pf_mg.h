@@ -35,7 +35,7 @@ namespace pf {
 
 constexpr int MG_COARSEST_CELLS = 512;   // cells of the coarsest level (one block, LDS)
 constexpr int MG_MAX_LEVELS = 16;
-constexpr int MG_NPART = 1024;           // blocks of the reduction kernels (partial sums)
+constexpr int MG_NPART = 4096;           // blocks of the reduction kernels (partial sums)
 
 // A coarse level: interior extents m x n x l (l = 1 in 2D) with one ghost layer around
 // (in z only in 3D); cell (I,J,K), 0-based, at (I+1) + PX*((J+1) + Y*(K+KG)).
@@ -93,6 +93,9 @@ struct MgCase {
     const double *phi = nullptr;     // natural FP64 porosity (the air rows' fluid test)
     const double *PHIs = nullptr;    // colour-split FP64 porosity
     const float *PHI32 = nullptr;    // colour-split FP32 porosity (FP32 solver)
+    const float *PHC = nullptr;      // FP64 solver: the porosity coded in 4 bytes, and its table
+    const double *PHTAB = nullptr;   //   (used by the level-0 sweeps while `coded`)
+    bool coded = false;
     bool fast = false;               // FP64 faces by the checked constant-divisor division
     int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings
 };

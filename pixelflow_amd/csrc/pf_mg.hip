@@ -135,12 +135,14 @@ __device__ __forceinline__ void mg_cell(const MgGeom &g, int64_t q, int &I, int 
 
 // ------------------------------------------------------------------------------------
 // Level 0 -> 1: r1 = sum over the kept cells of each aggregate of (rhs - A e); e1 = 0.
+// Cells of colour `zero_colour` are left out: the pre-smoothing ends with a Gauss-Seidel
+// (omega = 1) sweep of that colour, after which their residual is zero but for rounding.
 // ------------------------------------------------------------------------------------
 template <typename T, int DIM, bool FAST>
 __global__ __launch_bounds__(256) void k_mg_restrict0(PfGeom g, StepArgs a, MgK<T> K, const T *F,
                                                       const double *phi, const T *__restrict__ rhs,
                                                       const T *__restrict__ e, MgGeom c1, T *__restrict__ r1,
-                                                      T *__restrict__ e1) {
+                                                      T *__restrict__ e1, int zero_colour) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)c1.m * c1.n * c1.l) return;
     int I, J, Kc;
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict0(PfGeom g, StepArgs a, MgK<
             for (int dx = 0; dx < 2; ++dx) {
                 const int i = 2 * I + dx + 1, j = 2 * J + dy + 1, k = (DIM == 3) ? 2 * Kc + dz + 1 : 0;
                 if (i > g.m || j > g.n || (DIM == 3 && k > g.l)) continue;
+                if (((i + j + k) & 1) == zero_colour) continue;
                 if (dec0(g, a, i, j, k, phi)) continue;
                 sum = sum + (gmem(rhs)[pf_soff(g, i, j, k)] - apply0_at<T, DIM, FAST>(g, a, K, F, phi, e, i, j, k));
             }
@@ -872,7 +875,8 @@ static void sweep0(const MgSolver<double> &S, hipStream_t st, const StepArgs &a1
     sw.xcd_remap = S.c.remap64;
     sw.kb = S.c.kb64;
     sw.xw = S.c.xw;
-    launch_sor_phi(st, g, a1, sw, e, S.c.PHIs, rhs, S.c.phi, nullptr, S.c.fast);
+    launch_sor_phi(st, g, a1, sw, e, S.c.PHIs, rhs, S.c.phi, nullptr, S.c.fast, S.c.coded ? S.c.PHC : nullptr,
+                   S.c.PHTAB);
     if (l0.ref_y || l0.ref_z) launch_ghost_refresh(st, g, l0.ref_y, l0.ref_z, e);
 }
 static void refresh0(const MgSolver<float> &S, hipStream_t st, float *e) {
@@ -934,8 +938,11 @@ static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
         const MgK<T> K = mg_k<T>(S.c.a);
         const T *F = fine_phi(S);
         const unsigned nb = blocks_for((int64_t)C.g.m * C.g.n * C.g.l);
+        // with an odd periodic extent a boundary cell's periodic neighbour has its own colour
+        // and is refreshed after the sweep, so its residual is not zero: all cells count
+        const int zero_colour = (S.prm.nu1 > 0 && !S.lev[0].g.ref_y && !S.lev[0].g.ref_z) ? 1 : -1;
 #define PF_R0(D, F_) hipLaunchKernelGGL((k_mg_restrict0<T, D, F_>), dim3(nb), dim3(256), 0, st, g, S.c.a, K, F, S.c.phi, \
-                                        rhs, (const T *)e, C.g, C.r, C.e)
+                                        rhs, (const T *)e, C.g, C.r, C.e, zero_colour)
         if (g.dim == 3) { if (S.c.fast) PF_R0(3, true); else PF_R0(3, false); }
         else { if (S.c.fast) PF_R0(2, true); else PF_R0(2, false); }
 #undef PF_R0

@@ -27,8 +27,8 @@ static void say(const char *what) {
     std::fflush(stdout);
 }
 
-// A 2D uniform-inflow case of m x n cells with a cylinder of porosity 0 (tanh interface),
-// cell Reynolds number 1 and a CFL-0.2 time step.
+// A 2D uniform-inflow case of m x n cells with a cylinder of porosity 0 (tanh interface,
+// diameter 0.3 n), cell Reynolds number 1 and a time step of 0.1 dx / U.
 static pf_desc desc_2d(int m, int n, int precision) {
     pf_desc d{};
     d.dim = 2;
@@ -43,7 +43,7 @@ static pf_desc desc_2d(int m, int n, int precision) {
     d.dx = 1.0 / (m - 1);
     d.dy = d.dx;
     d.dz = 1.0;
-    d.dt = 0.2 * d.dx;
+    d.dt = 0.1 * d.dx;
     d.xnue = d.dx;
     d.density = 1.0;
     d.thickness = 1.5;
@@ -74,7 +74,7 @@ static pf_desc desc_room(int s) {
 
 static std::vector<double> cylinder(int m, int n) {
     std::vector<double> phi((size_t)m * n);
-    const double cx = 0.3 * m, cy = 0.5 * n, r = 0.08 * m;
+    const double cx = 0.3 * m, cy = 0.5 * n, r = 0.15 * n;
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < m; ++i) {
             const double d = std::sqrt((i - cx) * (i - cx) + (j - cy) * (j - cy)) - r;
@@ -94,9 +94,27 @@ static int add_2d(pf_batch *b, int m, int n, int precision, double tol = 0.0, in
     return slot;
 }
 
-static void step(pf_batch *b, int nsteps) {
-    std::vector<pf_stats> st((size_t)nsteps * 256);
-    CHECK(pf_batch_step(b, nsteps, st.data()));
+// A step that reports PF_EDIVERGED still advanced every case (the autotuner times such
+// steps too); the cases whose statistics are not finite are listed once.
+static int g_diverged_calls = 0;
+
+static void step(pf_batch *b, int nsteps, int max_cases = 256) {
+    std::vector<pf_stats> st((size_t)nsteps * max_cases);
+    const int rc = pf_batch_step(b, nsteps, st.data());
+    if (rc == PF_EDIVERGED) {
+        if (g_diverged_calls++ == 0) {
+            for (int s = 0; s < nsteps; ++s)
+                for (int slot = 0; slot < max_cases; ++slot) {
+                    const pf_stats &o = st[(size_t)s * max_cases + slot];
+                    if (pf_batch_case_steps(b, slot) >= 0 && !std::isfinite(o.sor_error))
+                        std::printf("[lifecycle] step %d slot %d: sor_error %g residual %g\n", s, slot, o.sor_error,
+                                    o.residual);
+                }
+            std::fflush(stdout);
+        }
+    } else {
+        CHECK(rc);
+    }
     CHECK(pf_batch_sync(b));
 }
 
@@ -135,7 +153,7 @@ static void autotune_sequence(int precision) {
 static void rooms_sequence() {
     pf_desc kind = desc_room(24);
     pf_batch *b = nullptr;
-    CHECK(pf_batch_create(&kind, 16, (int64_t)1 << 30, &b));
+    CHECK(pf_batch_create(&kind, 16, (int64_t)1 << 30, &b));   // max_cases 16
     std::vector<int> slots;
     for (int s : {24, 28, 32, 20, 26}) {
         pf_desc d = desc_room(s);
@@ -150,10 +168,10 @@ static void rooms_sequence() {
     for (int pb : {0, 512}) {
         pf_batch_config cfg{pb, 1, 0, 0};
         CHECK(pf_batch_set_config(b, &cfg));
-        step(b, 2);
+        step(b, 2, 16);
     }
     CHECK(pf_batch_remove(b, slots[1]));
-    step(b, 2);
+    step(b, 2, 16);
     CHECK(pf_batch_destroy(b));
     say("3D rooms done");
 }
@@ -167,11 +185,11 @@ static void multigrid_sequence(int precision) {
     CHECK(pf_batch_create(&kind, 32, (int64_t)1 << 30, &b));
     std::vector<int> slots;
     for (int c = 0; c < 6; ++c) slots.push_back(add_2d(b, 48 + 16 * c, 24 + 8 * (c % 3), precision, 1e-9, PF_SOLVER_MG));
-    step(b, 2);
+    step(b, 2, 32);
     CHECK(pf_batch_remove(b, slots[2]));
     CHECK(pf_batch_remove(b, slots[4]));
     add_2d(b, 96, 40, precision, 1e-9, PF_SOLVER_MG);
-    step(b, 3);
+    step(b, 3, 32);
     CHECK(pf_batch_destroy(b));
     say(precision == PF_PREC_MIXED ? "multigrid batch (mixed) done" : "multigrid batch (fp64) done");
 }
@@ -207,6 +225,7 @@ int main() {
     multigrid_sequence(PF_PREC_MIXED);
     multigrid_sequence(PF_PREC_FP64);
     serve_sequence();
+    std::printf("[lifecycle] step calls reporting a diverged case: %d\n", g_diverged_calls);
     say("all sequences done");
     return 0;
 }

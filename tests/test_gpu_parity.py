@@ -175,6 +175,42 @@ def test_gpu_sweep_kernels_agree(name, variant, monkeypatch):
         np.testing.assert_array_equal(st["sor_error"], g["errs"])
 
 
+@pytest.mark.parametrize("coded", ["0", "1"])
+@pytest.mark.parametrize("name", ["g2d_weird_odd", "g3d_sph32", "g3d_odd_33_17_15", "g3d_air_room24", "g3d_weird",
+                                  "g3d_boxes"])
+def test_gpu_sweep_porosity_storage_matches_reference(name, coded, monkeypatch):
+    """The FP64 sweep with the porosity read as stored FP64 (PF_PHI_CODE=0, the default) and
+    coded in 4 bytes (PF_PHI_CODE=1) reproduces the reference identically; the binary-porosity boxes
+    hold the clamped 1e-6, which FP32 cannot hold exactly, and the tanh interfaces hold
+    values that need the table."""
+    monkeypatch.setenv("PF_PHI_CODE", coded)
+    p, g = load_golden(name)
+    with _solver(p, g["phi_raw"]) as s:
+        assert s.sor_coded() == (coded == "1")
+        st = s.step(p.istep_max)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), g[f], err_msg=f"{name}:{f}:coded{coded}")
+        np.testing.assert_array_equal(st["sor_error"], g["errs"])
+
+
+def test_gpu_porosity_code_falls_back_past_its_table(monkeypatch):
+    """A porosity field with more distinct non-FP32 values than the code's table holds
+    (2^20) is read as stored FP64: 120x96x100 cells of random porosity, checked against the
+    oracle for one step."""
+    import pyoracle
+    monkeypatch.setenv("PF_PHI_CODE", "1")
+    p = cases.CaseParams(dim=3, m=120, n=96, l=100, width=1.0, height=0.8, depth=0.83, time=1e-4, istep_max=1,
+                         iter_max=5)
+    phi = np.random.default_rng(7).uniform(0.2, 1.0, size=(p.l, p.n, p.m))
+    with _solver(p, phi) as s:
+        assert not s.sor_coded()
+        s.step(1)
+        o = pyoracle.OracleRun(p, phi)
+        o.step(1)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), o.interior(f), err_msg=f)
+
+
 def test_gpu_default_sweep_is_checked_fast_path():
     p, phi = cases.sphere_case(m=64, n=32, l=32, steps=1)
     with _solver(p, phi) as s:
