@@ -160,9 +160,10 @@ int pf_probe_copy(int64_t bytes, int iters, double *gbps);
 int pf_sor_fused(const pf_ctx *ctx);
 /* 1 if the FP64 colour sweeps read the porosity coded in 4 bytes per value (values FP32
  * holds exactly as floats, the others through a table of at most 2^20 distinct doubles:
- * the same doubles, 4 fewer bytes per cell and sweep; opt-in with PF_PHI_CODE=1 in the
- * environment), 0 if they read it as stored FP64 (the default, or too many distinct
- * values, or a precision other than FP64).  Same results.  No reference counterpart. */
+ * the same doubles, 4 fewer bytes per cell and sweep; the default past 512 MB of sweep
+ * arrays, PF_PHI_CODE=0/1 in the environment decides), 0 if they read it as stored FP64
+ * (smaller grids, too many distinct values, or a precision other than FP64).  Same
+ * results.  No reference counterpart. */
 int pf_sor_coded(const pf_ctx *ctx);
 int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
 

@@ -50,8 +50,7 @@ struct pf_ctx {
     float *E32 = nullptr, *R32 = nullptr, *PHI32 = nullptr;   // mixed: colour-split FP32 arrays
     // FP64 sweeps: the colour-split porosity coded in 4 bytes per value (PHC), with the
     // table of the values FP32 cannot hold (PHTAB, room for PHI_TAB_MAX); used when
-    // phi_coded (PF_PHI_CODE=1 in the environment turns it on: at 256x128x128 the coded
-    // sweep took 18.5 against 17.9 us)
+    // phi_coded (on past 512 MB of sweep arrays; PF_PHI_CODE=0/1 in the environment decides)
     float *PHC = nullptr;
     double *PHTAB = nullptr;
     bool phi_code_on = false, phi_coded = false;
@@ -110,7 +109,7 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
                            ? pf::mg_bytes(g, d->bc, d->precision == PF_PREC_MIXED)
                            : 0;
     const char *pc = std::getenv("PF_PHI_CODE");
-    const bool code_on = pc && *pc && std::atoi(pc) != 0;
+    const bool code_on = (pc && *pc) ? std::atoi(pc) != 0 : interior * 24 > ((int64_t)512 << 20);
     const int64_t coded =
         (d->precision == PF_PREC_FP64 && code_on) ? (int64_t)4 * 2 * g.HS + (int64_t)8 * PHI_TAB_MAX : 0;
     return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + 4096;
@@ -216,6 +215,11 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         }
     }
     {
+        // coded porosity where the FP64 sweep's arrays (24 B per cell) are well past the
+        // last-level cache: 512x256x256 179 against 184 us per sweep; at 256x128x128, inside
+        // the cache, it was slower (18.5 against 17.9 us)
+        const int64_t cells = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
+        c->phi_code_on = cells * 24 > ((int64_t)512 << 20);
         const char *pc = std::getenv("PF_PHI_CODE");
         if (pc && *pc) c->phi_code_on = std::atoi(pc) != 0;
     }

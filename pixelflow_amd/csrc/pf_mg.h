@@ -36,6 +36,7 @@ namespace pf {
 constexpr int MG_COARSEST_CELLS = 512;   // cells of the coarsest level (one block, LDS)
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_NPART = 4096;           // blocks of the reduction kernels (partial sums)
+constexpr int MG_TAIL_LEVELS = 8;        // levels the one-block tail of the V-cycle holds at most
 
 // A coarse level: interior extents m x n x l (l = 1 in 2D) with one ghost layer around
 // (in z only in 3D); cell (I,J,K), 0-based, at (I+1) + PX*((J+1) + Y*(K+KG)).
@@ -120,6 +121,9 @@ struct MgSolver {
     T *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
       *y = nullptr, *z = nullptr;
     double *scal = nullptr;          // device scalars (see pf_mg.hip)
+    T *cinv = nullptr;               // the coarsest level's inverse, transposed (n x n)
+    bool cinv_ok = false;            // cinv holds the inverse of the current operators
+    int tail_q = 0;                  // first level of the one-block tail (0: no tail)
     double *part = nullptr;          // MG_NPART x 4 partial sums
     // the outer loop's reductions: per-wave max |r/ap| and max |e| (nslots each), the
     // reduced residual

@@ -126,7 +126,7 @@ __device__ __forceinline__ void mg_store(const MgGeom &g, T *__restrict__ e, int
     }
 }
 
-__device__ __forceinline__ void mg_cell(const MgGeom &g, int64_t q, int &I, int &J, int &K) {
+PF_HD void mg_cell(const MgGeom &g, int64_t q, int &I, int &J, int &K) {
     I = (int)(q % g.m);
     const int64_t t = q / g.m;
     J = (int)(t % g.n);
@@ -409,6 +409,202 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
         }
     }
     for (int q = tid; q < (int)g.N; q += 256) e[q] = se[q];
+}
+
+// ------------------------------------------------------------------------------------
+// The tail of the V-cycle: every level from the first one of at most MG_TAIL_CELLS cells
+// down to the coarsest, in one block of MG_TAIL_THREADS threads.  Each launch of the coarse
+// levels is a phase here, separated by block barriers (global memory written by the block
+// is visible to the block after the barrier); each thread gathers the operands of up to
+// MG_TAIL_U cells before it stores any, so a phase costs about one memory round trip.  The
+// coarsest level is solved exactly by its inverse (coarsest_inv, transposed), computed on
+// the host when the operators are built.  The operations and their order are those of the
+// per-level launches.
+// ------------------------------------------------------------------------------------
+constexpr int MG_TAIL_THREADS = 1024;
+constexpr int MG_TAIL_U = 8;
+constexpr int64_t MG_TAIL_CELLS = (int64_t)MG_TAIL_THREADS * MG_TAIL_U;
+
+template <typename T>
+struct MgTailLevel {
+    MgGeom g;
+    MgOp<T> A;
+    const uint8_t *keep;
+    T *r, *e;
+};
+
+template <typename T>
+struct MgTailArgs {
+    MgTailLevel<T> L[MG_TAIL_LEVELS];
+    int nl;                 // levels in L; L[nl - 1] is the coarsest
+    int nuc;                // sweeps before and after the coarse correction
+    T alpha;
+    const T *cinv;          // coarsest inverse, transposed: cinv[j * n + i] = (A^-1)(i, j)
+};
+
+template <typename T>
+__device__ __forceinline__ void tail_refresh(const MgGeom &g, T *__restrict__ e) {
+    if (!(g.ref_y || g.ref_z)) return;
+    const int64_t ny = (int64_t)g.m * g.l, nz = (int64_t)g.m * g.n;
+    const int64_t nmax = ny > nz ? ny : nz;
+    for (int64_t q = threadIdx.x; q < nmax; q += MG_TAIL_THREADS) {
+        if (g.ref_y && q < ny) {
+            const int I = (int)(q % g.m), K = (int)(q / g.m);
+            e[mg_off(g, I, -1, K)] = e[mg_off(g, I, g.n - 1, K)];
+            e[mg_off(g, I, g.n, K)] = e[mg_off(g, I, 0, K)];
+        }
+        if (g.ref_z && g.dim == 3 && q < nz) {
+            const int I = (int)(q % g.m), J = (int)(q / g.m);
+            e[mg_off(g, I, J, -1)] = e[mg_off(g, I, J, g.l - 1)];
+            e[mg_off(g, I, J, g.l)] = e[mg_off(g, I, J, 0)];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour) {
+    const MgGeom &g = L.g;
+    const int64_t cells = (int64_t)g.m * g.n * g.l;
+    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+        T v[MG_TAIL_U];
+        int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
+        bool act[MG_TAIL_U];
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u) {
+            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            act[u] = false;
+            v[u] = T(0);
+            if (q >= cells) continue;
+            mg_cell(g, q, I[u], J[u], K[u]);
+            if (((I[u] + J[u] + K[u]) & 1) != colour) continue;
+            act[u] = true;
+            const int64_t o = mg_off(g, I[u], J[u], K[u]);
+            T s = L.r[o];
+            s = s - L.A.cE[o] * L.e[o + 1];
+            s = s - L.A.cW[o] * L.e[o - 1];
+            s = s - L.A.cN[o] * L.e[o + sy];
+            s = s - L.A.cS[o] * L.e[o - sy];
+            if (g.dim == 3) {
+                s = s - L.A.cT[o] * L.e[o + sz];
+                s = s - L.A.cB[o] * L.e[o - sz];
+            }
+            v[u] = s / L.A.ap[o];
+        }
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u)
+            if (act[u]) mg_store(g, L.e, I[u], J[u], K[u], v[u], g.fuse_y, g.fuse_z);
+    }
+    __syncthreads();
+    tail_refresh(g, L.e);
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_restrict(const MgTailLevel<T> &F, const MgTailLevel<T> &C) {
+    const MgGeom &f = F.g, &c = C.g;
+    const int64_t cells = (int64_t)c.m * c.n * c.l;
+    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+        T v[MG_TAIL_U];
+        int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u) {
+            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            v[u] = T(0);
+            if (q >= cells) continue;
+            mg_cell(c, q, I[u], J[u], K[u]);
+            T sum = T(0);
+            for (int dz = 0; dz < (f.dim == 3 ? 2 : 1); ++dz)
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int i = 2 * I[u] + dx, j = 2 * J[u] + dy, k = (f.dim == 3) ? 2 * K[u] + dz : 0;
+                        if (i >= f.m || j >= f.n || k >= f.l) continue;
+                        const int64_t o = mg_off(f, i, j, k);
+                        if (!F.keep[o]) continue;
+                        sum = sum + (F.r[o] - mg_ax(f, F.A, F.e, o));
+                    }
+            v[u] = sum;
+        }
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u) {
+            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            if (q >= cells) continue;
+            C.r[mg_off(c, I[u], J[u], K[u])] = v[u];
+            mg_store(c, C.e, I[u], J[u], K[u], T(0), c.per, c.per);
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_prolong(const MgTailLevel<T> &F, const MgTailLevel<T> &C, T alpha) {
+    const MgGeom &f = F.g, &c = C.g;
+    const int64_t cells = (int64_t)f.m * f.n * f.l;
+    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+        T v[MG_TAIL_U];
+        int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
+        bool act[MG_TAIL_U];
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u) {
+            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            act[u] = false;
+            v[u] = T(0);
+            if (q >= cells) continue;
+            mg_cell(f, q, I[u], J[u], K[u]);
+            const int64_t o = mg_off(f, I[u], J[u], K[u]);
+            if (!F.keep[o]) continue;
+            act[u] = true;
+            v[u] = F.e[o] + alpha * C.e[mg_off(c, I[u] >> 1, J[u] >> 1, (f.dim == 3) ? K[u] >> 1 : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < MG_TAIL_U; ++u)
+            if (act[u]) mg_store(f, F.e, I[u], J[u], K[u], v[u], f.fuse_y, f.fuse_z);
+    }
+    __syncthreads();
+    tail_refresh(f, F.e);
+}
+
+// e = A^-1 r on the coarsest level (n <= MG_COARSEST_CELLS cells).
+template <typename T>
+__device__ __forceinline__ void tail_coarsest(const MgTailLevel<T> &L, const T *__restrict__ cinv) {
+    __shared__ T rl[MG_COARSEST_CELLS];
+    const MgGeom &g = L.g;
+    const int n = g.m * g.n * g.l;
+    for (int q = threadIdx.x; q < n; q += MG_TAIL_THREADS) {
+        int I, J, K;
+        mg_cell(g, q, I, J, K);
+        rl[q] = L.r[mg_off(g, I, J, K)];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += MG_TAIL_THREADS) {
+        T s = T(0);
+        for (int j = 0; j < n; ++j) s = s + cinv[(int64_t)j * n + q] * rl[j];
+        int I, J, K;
+        mg_cell(g, q, I, J, K);
+        mg_store(g, L.e, I, J, K, s, g.fuse_y, g.fuse_z);
+    }
+    __syncthreads();
+    tail_refresh(g, L.e);
+}
+
+template <typename T>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(MgTailArgs<T> t) {
+    const int nl = t.nl;
+    for (int q = 0; q + 1 < nl; ++q) {
+        for (int s = 0; s < t.nuc; ++s) {
+            tail_smooth(t.L[q], 0);
+            tail_smooth(t.L[q], 1);
+        }
+        tail_restrict(t.L[q], t.L[q + 1]);
+    }
+    tail_coarsest(t.L[nl - 1], t.cinv);
+    for (int q = nl - 2; q >= 0; --q) {
+        tail_prolong(t.L[q], t.L[q + 1], t.alpha);
+        for (int s = 0; s < t.nuc; ++s) {
+            tail_smooth(t.L[q], 1);
+            tail_smooth(t.L[q], 0);
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -709,6 +905,10 @@ static int64_t mg_bytes_t(const PfGeom &g, int bc, int64_t tsz) {
     for (size_t q = 1; q < L.size(); ++q) b += 9 * align256(L[q].N * tsz) + align256(L[q].N);
     b += align256(16 * 8) + align256((int64_t)MG_NPART * 4 * 8);
     b += 2 * align256((int64_t)mg_nslots(g) * 8) + align256(8);
+    if (L.size() > 1) {
+        const int64_t n = (int64_t)L.back().m * L.back().n * L.back().l;
+        b += align256(n * n * tsz);
+    }
     return b;
 }
 
@@ -787,6 +987,22 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
     S.res_slots = (unsigned long long *)take((int64_t)S.nslots * 8);
     S.err_slots = (unsigned long long *)take((int64_t)S.nslots * 8);
     S.res_val = (unsigned long long *)take(8);
+    S.cinv = nullptr;
+    S.tail_q = 0;
+    if (L.size() > 1) {
+        const int64_t n = (int64_t)L.back().m * L.back().n * L.back().l;
+        S.cinv = (T *)take(n * n * (int64_t)sizeof(T));
+        // the tail starts at the first level of at most MG_TAIL_CELLS cells (PF_MG_TAIL=0
+        // in the environment keeps one launch per level and step)
+        const char *tv = std::getenv("PF_MG_TAIL");
+        const bool tail_on = !(tv && *tv && std::atoi(tv) == 0);
+        for (size_t q = 1; tail_on && q < L.size(); ++q)
+            if ((int64_t)L[q].m * L[q].n * L[q].l <= MG_TAIL_CELLS) {
+                if (L.size() - q <= (size_t)MG_TAIL_LEVELS && n <= MG_COARSEST_CELLS) S.tail_q = (int)q;
+                break;
+            }
+    }
+    S.cinv_ok = false;
     S.built = false;
     return true;
 }
@@ -824,6 +1040,83 @@ static MgOp<T> op_of(const MgLevel<T> &L) {
     return MgOp<T>{L.ap, L.cE, L.cW, L.cN, L.cS, L.cT, L.cB};
 }
 
+// The coarsest operator as a dense matrix (cells in mg_cell order; a periodic neighbour
+// wraps, a non-periodic ghost is zero), inverted by Gauss-Jordan elimination with partial
+// pivoting in FP64 and stored transposed in S.cinv.  False (the tail is not used) when a
+// pivot is below 1e-12 of the largest entry or a HIP call fails.
+template <typename T>
+static bool coarsest_inverse(MgSolver<T> &S, hipStream_t st) {
+    const MgLevel<T> &L = S.lev.back();
+    const MgGeom &g = L.g;
+    const int n = g.m * g.n * g.l;
+    if (hipStreamSynchronize(st) != hipSuccess) return false;
+    std::vector<T> h[7];
+    const T *src[7] = {L.ap, L.cE, L.cW, L.cN, L.cS, L.cT, L.cB};
+    for (int a = 0; a < 7; ++a) {
+        h[a].resize((size_t)g.N);
+        if (hipMemcpy(h[a].data(), src[a], (size_t)g.N * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    }
+    auto idx = [&](int I, int J, int K) { return I + g.m * (J + g.n * K); };
+    std::vector<double> A((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
+    double amax = 0.0;
+    for (int q = 0; q < n; ++q) {
+        int I, J, K;
+        mg_cell(g, q, I, J, K);
+        const int64_t o = mg_off(g, I, J, K);
+        double *row = &A[(size_t)q * n];
+        row[q] += (double)h[0][(size_t)o];
+        if (I + 1 < g.m) row[idx(I + 1, J, K)] += (double)h[1][(size_t)o];
+        if (I > 0) row[idx(I - 1, J, K)] += (double)h[2][(size_t)o];
+        if (J + 1 < g.n) row[idx(I, J + 1, K)] += (double)h[3][(size_t)o];
+        else if (g.per) row[idx(I, 0, K)] += (double)h[3][(size_t)o];
+        if (J > 0) row[idx(I, J - 1, K)] += (double)h[4][(size_t)o];
+        else if (g.per) row[idx(I, g.n - 1, K)] += (double)h[4][(size_t)o];
+        if (g.dim == 3) {
+            if (K + 1 < g.l) row[idx(I, J, K + 1)] += (double)h[5][(size_t)o];
+            else if (g.per) row[idx(I, J, 0)] += (double)h[5][(size_t)o];
+            if (K > 0) row[idx(I, J, K - 1)] += (double)h[6][(size_t)o];
+            else if (g.per) row[idx(I, J, g.l - 1)] += (double)h[6][(size_t)o];
+        }
+        X[(size_t)q * n + q] = 1.0;
+    }
+    for (double v : A) amax = std::max(amax, std::fabs(v));
+    if (!(amax > 0.0) || !std::isfinite(amax)) return false;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)piv * n + c])) piv = r;
+        const double pv = A[(size_t)piv * n + c];
+        if (!(std::fabs(pv) > 1e-12 * amax)) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)piv * n + k], A[(size_t)c * n + k]);
+                std::swap(X[(size_t)piv * n + k], X[(size_t)c * n + k]);
+            }
+        const double inv = 1.0 / pv;
+        for (int k = 0; k < n; ++k) {
+            A[(size_t)c * n + k] *= inv;
+            X[(size_t)c * n + k] *= inv;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                X[(size_t)r * n + k] -= f * X[(size_t)c * n + k];
+            }
+        }
+    }
+    std::vector<T> t((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const double v = X[(size_t)i * n + j];
+            if (!std::isfinite(v)) return false;
+            t[(size_t)j * n + i] = (T)v;
+        }
+    return hipMemcpy(S.cinv, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
 template <typename T>
 void mg_setup(MgSolver<T> &S, hipStream_t st) {
     const PfGeom &g = S.c.g;
@@ -844,6 +1137,8 @@ void mg_setup(MgSolver<T> &S, hipStream_t st) {
                                out);
         }
     }
+    S.cinv_ok = false;
+    if (S.tail_q > 0) S.cinv_ok = coarsest_inverse(S, st);
     S.built = true;
 }
 
@@ -893,6 +1188,19 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
     MgLevel<T> &L = S.lev[q];
     const MgGeom &g = L.g;
     const int64_t cells = (int64_t)g.m * g.n * g.l;
+    if (S.tail_q > 0 && q == (size_t)S.tail_q && S.cinv_ok) {
+        MgTailArgs<T> t{};
+        t.nl = (int)(S.lev.size() - q);
+        for (int k = 0; k < t.nl; ++k) {
+            MgLevel<T> &lv = S.lev[q + (size_t)k];
+            t.L[k] = MgTailLevel<T>{lv.g, op_of(lv), (const uint8_t *)lv.keep, lv.r, lv.e};
+        }
+        t.nuc = S.prm.nuc;
+        t.alpha = (T)S.prm.alpha;
+        t.cinv = S.cinv;
+        hipLaunchKernelGGL((k_mg_tail<T>), dim3(1), dim3(MG_TAIL_THREADS), 0, st, t);
+        return;
+    }
     if (q + 1 == S.lev.size()) {
         hipLaunchKernelGGL((k_mg_coarsest<T>), dim3(1), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e,
                            S.prm.coarsest_sweeps);

@@ -179,8 +179,8 @@ def test_gpu_sweep_kernels_agree(name, variant, monkeypatch):
 @pytest.mark.parametrize("name", ["g2d_weird_odd", "g3d_sph32", "g3d_odd_33_17_15", "g3d_air_room24", "g3d_weird",
                                   "g3d_boxes"])
 def test_gpu_sweep_porosity_storage_matches_reference(name, coded, monkeypatch):
-    """The FP64 sweep with the porosity read as stored FP64 (PF_PHI_CODE=0, the default) and
-    coded in 4 bytes (PF_PHI_CODE=1) reproduces the reference identically; the binary-porosity boxes
+    """The FP64 sweep with the porosity read as stored FP64 (PF_PHI_CODE=0, the default at
+    these sizes) and coded in 4 bytes (PF_PHI_CODE=1) reproduces the reference identically; the binary-porosity boxes
     hold the clamped 1e-6, which FP32 cannot hold exactly, and the tanh interfaces hold
     values that need the table."""
     monkeypatch.setenv("PF_PHI_CODE", coded)
