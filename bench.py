@@ -22,7 +22,9 @@ sweeps), projection and boundary update, all resident in HBM.  The line printed:
               on one thread for one step
 Further sections: mixed precision on the same workload; the 512x256x256 boxes solved to
 the residual rule max|r/ap| <= 1e-8 every step (multigrid-preconditioned, FP64 and mixed,
-time to solution per step) with the HBM roofline of the FP64 colour sweep at that size; the
+time to solution per step, and FP64 with one long inner solve per step) with the HBM
+roofline of the FP64 colour sweep at that size (20 B per cell there: the porosity is read
+coded in 4 bytes); the
 2D cylinder benchmark case (256x128, 2000 steps) on the GPU and on the reference's build;
 the batched 2D workload, the served 3D rooms and the same rooms resident together.
 """
@@ -295,13 +297,26 @@ def tolerance_large(steps: int, warmup: int, tol: float):
                                    for f in ("u", "v", "w", "p")}
     res["divergence_max_diff_rel_vs_fp64"] = float(np.max(np.abs(b["div"] - a["div"])) /
                                                    max(np.max(np.abs(a["div"])), 1e-300))
+    # FP64 with one long inner solve per step (inner tolerance 1e-8): the inner solve is
+    # FP64 already, so the outer cycles only restart it; the best FP64 setting measured
+    with Solver(p, phi, precision=PF_PREC_FP64, inner_tol=1e-8, inner_iters=100, **mg) as s:
+        wall, _, st = _timed_run(s, steps, warmup)
+    res["fp64_single_solve"] = {"ms_per_step": wall * 1e3 / steps, "inner_tol": 1e-8,
+                                "max_residual": float(np.max(st["residual"])),
+                                "vcycles_per_step": float(np.mean(st["sor_iters"])),
+                                "cycles_per_step": float(np.mean(st["cycles"]))}
+    res["speedup_vs_fp64_single_solve"] = res["fp64_single_solve"]["ms_per_step"] / res["mixed"]["ms_per_step"]
     with Solver(p, phi) as s:
         wall, sweep_us, _ = _timed_run(s, steps, warmup)
+        coded = s.sor_coded()
     res["fixed_count_fp64"] = {"ms_per_step": wall * 1e3 / steps, "cell_steps_per_s": p.cells * steps / wall,
                                "iter_max": p.iter_max, "avg_colour_sweep_us": sweep_us}
-    bytes_ = 3 * 8 * p.cells
+    # per grid cell: p of both colours, bb and p written (half a word each, 16 B), porosity
+    # as stored FP64 (8 B) or coded in 4 bytes (the default past 512 MB of sweep arrays)
+    bytes_ = (20 if coded else 24) * p.cells
     achieved = bytes_ / (sweep_us * 1e-6) / 1e9
-    res["sweep_roofline"] = {"bound": "hbm", "kernel": "k_sor_phi (red/black colour sweep, FP64)",
+    res["sweep_roofline"] = {"bound": "hbm", "kernel": "k_sor_phi (red/black colour sweep, FP64" +
+                             (", porosity coded in 4 bytes)" if coded else ")"),
                              "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS,
                              "traffic": measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"),
