@@ -19,8 +19,10 @@
 // * coarse levels are dense arrays with one ghost layer; periodic ghosts are copies (written
 //   with the update for even extents, by a refresh pass for odd ones, so every update of a
 //   pass reads values from before the pass across a wrap, as the reference's p_old does);
-// * the coarsest level (at most MG_COARSEST_CELLS cells) is solved by red/black sweeps in
-//   LDS inside one block.
+// * the levels of at most 8192 cells run in one block (the tail: one phase per launch of
+//   the per-level path), and the coarsest level (at most MG_COARSEST_CELLS cells where the
+//   extents allow) is solved exactly by its inverse, formed on the host when the operators
+//   are built; red/black sweeps in LDS take its place when that inverse is not available.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,7 +35,7 @@
 
 namespace pf {
 
-constexpr int MG_COARSEST_CELLS = 512;   // cells of the coarsest level (one block, LDS)
+constexpr int MG_COARSEST_CELLS = 512;   // coarsening stops at this many cells or fewer
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_NPART = 4096;           // blocks of the reduction kernels (partial sums)
 constexpr int MG_TAIL_LEVELS = 8;        // levels the one-block tail of the V-cycle holds at most
@@ -101,13 +103,14 @@ struct MgCase {
     int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings
 };
 
-// Tuning of the V-cycle, the same for every case (PF_MG_NU, PF_MG_NUC, PF_MG_ALPHA and
-// PF_MG_CS in the environment override it when a solver is created).
+// Tuning of the V-cycle, the same for every case (PF_MG_NU, PF_MG_NUC, PF_MG_ALPHA,
+// PF_MG_CS and PF_MG_GAMMA in the environment override it when a solver is created).
 struct MgParams {
     int nu1 = 2, nu2 = 2;            // level 0: red/black sweeps before and after the coarse correction
     int nuc = 2;                     // coarse levels: sweeps before and after
     double alpha = 1.6;              // over-correction of the piecewise-constant interpolation
     int coarsest_sweeps = 60;        // red/black sweeps of the coarsest level
+    int tail_gamma = 2;              // cycles per level inside the one-block tail (1: V, 2: W)
 };
 
 // The solver of one case in precision T (float: mixed precision; double: FP64).

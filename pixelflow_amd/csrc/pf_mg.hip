@@ -438,6 +438,7 @@ struct MgTailArgs {
     MgTailLevel<T> L[MG_TAIL_LEVELS];
     int nl;                 // levels in L; L[nl - 1] is the coarsest
     int nuc;                // sweeps before and after the coarse correction
+    int gamma;              // cycles of the next level per level (1: V, 2: W)
     T alpha;
     const T *cinv;          // coarsest inverse, transposed: cinv[j * n + i] = (A^-1)(i, j)
 };
@@ -587,24 +588,33 @@ __device__ __forceinline__ void tail_coarsest(const MgTailLevel<T> &L, const T *
     tail_refresh(g, L.e);
 }
 
+// The cycle from tail level Q down: pre-smoothing, restriction, `gamma` cycles of the next
+// level (1: V-cycle, 2: W-cycle; a later visit continues from the correction the earlier
+// one left), prolongation, post-smoothing.  Unrolled over the level depth at compile time.
+template <typename T, int Q>
+__device__ void tail_cycle(const MgTailArgs<T> &t) {
+    if constexpr (Q < MG_TAIL_LEVELS) {
+        if (Q == t.nl - 1) {
+            tail_coarsest(t.L[Q], t.cinv);
+            return;
+        }
+        for (int s = 0; s < t.nuc; ++s) {
+            tail_smooth(t.L[Q], 0);
+            tail_smooth(t.L[Q], 1);
+        }
+        tail_restrict(t.L[Q], t.L[Q + 1]);
+        for (int c = 0; c < (Q + 2 == t.nl ? 1 : t.gamma); ++c) tail_cycle<T, Q + 1>(t);
+        tail_prolong(t.L[Q], t.L[Q + 1], t.alpha);
+        for (int s = 0; s < t.nuc; ++s) {
+            tail_smooth(t.L[Q], 1);
+            tail_smooth(t.L[Q], 0);
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(MgTailArgs<T> t) {
-    const int nl = t.nl;
-    for (int q = 0; q + 1 < nl; ++q) {
-        for (int s = 0; s < t.nuc; ++s) {
-            tail_smooth(t.L[q], 0);
-            tail_smooth(t.L[q], 1);
-        }
-        tail_restrict(t.L[q], t.L[q + 1]);
-    }
-    tail_coarsest(t.L[nl - 1], t.cinv);
-    for (int q = nl - 2; q >= 0; --q) {
-        tail_prolong(t.L[q], t.L[q + 1], t.alpha);
-        for (int s = 0; s < t.nuc; ++s) {
-            tail_smooth(t.L[q], 1);
-            tail_smooth(t.L[q], 0);
-        }
-    }
+    tail_cycle<T, 0>(t);
 }
 
 // ------------------------------------------------------------------------------------
@@ -933,6 +943,7 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
         if ((v = std::getenv("PF_MG_NUC")) && *v) S.prm.nuc = std::max(1, std::atoi(v));
         if ((v = std::getenv("PF_MG_CS")) && *v) S.prm.coarsest_sweeps = std::max(1, std::atoi(v));
         if ((v = std::getenv("PF_MG_ALPHA")) && *v) S.prm.alpha = std::atof(v);
+        if ((v = std::getenv("PF_MG_GAMMA")) && *v) S.prm.tail_gamma = std::min(3, std::max(1, std::atoi(v)));
     }
     S.bytes = mg_bytes_t(g, c.a.bc, sizeof(T));
     S.own_mem = (mem == nullptr);
@@ -1196,6 +1207,7 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
             t.L[k] = MgTailLevel<T>{lv.g, op_of(lv), (const uint8_t *)lv.keep, lv.r, lv.e};
         }
         t.nuc = S.prm.nuc;
+        t.gamma = S.prm.tail_gamma;
         t.alpha = (T)S.prm.alpha;
         t.cinv = S.cinv;
         hipLaunchKernelGGL((k_mg_tail<T>), dim3(1), dim3(MG_TAIL_THREADS), 0, st, t);
