@@ -65,13 +65,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if present."""
+def measured_traffic(kernel: str, workload: str, variant: str = None):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if present
+    (`variant`: a sub-record of the summary, such as the kernel with stored FP64 porosity)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if variant:
+        d = d.get(variant, {})
     return d.get("bytes_per_launch")
 
 
@@ -306,21 +309,42 @@ def tolerance_large(steps: int, warmup: int, tol: float):
                                 "vcycles_per_step": float(np.mean(st["sor_iters"])),
                                 "cycles_per_step": float(np.mean(st["cycles"]))}
     res["speedup_vs_fp64_single_solve"] = res["fp64_single_solve"]["ms_per_step"] / res["mixed"]["ms_per_step"]
-    with Solver(p, phi) as s:
-        wall, sweep_us, _ = _timed_run(s, steps, warmup)
-        coded = s.sor_coded()
+    # the fixed-count FP64 path twice: porosity read as stored FP64 (the roofline figure:
+    # 24 B per grid cell and launch) and coded in 4 bytes (the library's default at this
+    # size: 20 B)
+    runs = {}
+    saved = os.environ.get("PF_PHI_CODE")
+    try:
+        for code in ("0", "1"):
+            os.environ["PF_PHI_CODE"] = code
+            with Solver(p, phi) as s:
+                wall, sweep_us, _ = _timed_run(s, steps, warmup)
+                runs[code] = (wall, sweep_us, s.sor_coded())
+    finally:
+        if saved is None:
+            os.environ.pop("PF_PHI_CODE", None)
+        else:
+            os.environ["PF_PHI_CODE"] = saved
+    wall, sweep_us, _ = runs["1"]
     res["fixed_count_fp64"] = {"ms_per_step": wall * 1e3 / steps, "cell_steps_per_s": p.cells * steps / wall,
-                               "iter_max": p.iter_max, "avg_colour_sweep_us": sweep_us}
-    # per grid cell: p of both colours, bb and p written (half a word each, 16 B), porosity
-    # as stored FP64 (8 B) or coded in 4 bytes (the default past 512 MB of sweep arrays)
-    bytes_ = (20 if coded else 24) * p.cells
-    achieved = bytes_ / (sweep_us * 1e-6) / 1e9
-    res["sweep_roofline"] = {"bound": "hbm", "kernel": "k_sor_phi (red/black colour sweep, FP64" +
-                             (", porosity coded in 4 bytes)" if coded else ")"),
-                             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved / HBM_PEAK_GBS,
-                             "traffic": measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"),
-                             "bytes_per_launch_algorithmic": bytes_, "avg_launch_us": sweep_us}
+                               "iter_max": p.iter_max, "avg_colour_sweep_us": sweep_us,
+                               "porosity": "coded in 4 bytes" if runs["1"][2] else "stored FP64",
+                               "ms_per_step_porosity_fp64": runs["0"][0] * 1e3 / steps,
+                               "avg_colour_sweep_us_porosity_fp64": runs["0"][1]}
+
+    def roofline(sweep_us, bytes_per_cell, kernel, traffic):
+        bytes_ = bytes_per_cell * p.cells
+        achieved = bytes_ / (sweep_us * 1e-6) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch_algorithmic": bytes_,
+                "avg_launch_us": sweep_us}
+
+    res["sweep_roofline"] = roofline(runs["0"][1], 24, "k_sor_phi (red/black colour sweep, FP64)",
+                                     measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}", "uncoded_kernel"))
+    if runs["1"][2]:
+        res["sweep_roofline"]["coded"] = roofline(
+            runs["1"][1], 20, "k_sor_phi (red/black colour sweep, FP64, porosity coded in 4 bytes)",
+            measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"))
     return res
 
 
