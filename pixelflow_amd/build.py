@@ -134,6 +134,25 @@ def build_asan_driver(jobs: int = 4) -> str:
     return ASAN_DRIVER
 
 
+SERVE_FAKE = os.path.join(OUT, "test", "libpfserve_fake.so")
+
+
+def build_serve_fake() -> str:
+    """The library's serving loop (csrc/pf_serve.cpp) linked with a stand-in batch
+    (tests/native/serve_fake_batch.cpp): a host-only library for CPU tests of the loop's
+    policy.  Built with g++; only the loop's device-selection calls need the HIP runtime."""
+    os.makedirs(os.path.dirname(SERVE_FAKE), exist_ok=True)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    gxx = shutil.which("g++") or "g++"
+    tmp = SERVE_FAKE + ".tmp"
+    _run([gxx, "-std=c++17", "-O1", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include",
+          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "pf_serve.cpp"),
+          os.path.join(ROOT, "tests", "native", "serve_fake_batch.cpp"), f"-L{rocm}/lib", "-lamdhip64",
+          f"-Wl,-rpath,{rocm}/lib", "-pthread", "-o", tmp])
+    os.replace(tmp, SERVE_FAKE)
+    return SERVE_FAKE
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib-only", action="store_true")
@@ -143,6 +162,7 @@ def main(argv=None) -> int:
     build_lib()
     if not args.lib_only:
         build_fortran()
+        build_serve_fake()
     if args.asan:
         build_asan_driver()
     return 0

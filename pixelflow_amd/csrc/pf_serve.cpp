@@ -14,7 +14,7 @@
 // run of that case alone.  Request state, timestamps and results are read with
 // pf_serve_poll / pf_serve_times / pf_serve_result.  No reference counterpart: the
 // reference runs one case per process.
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <chrono>
 #include <cmath>

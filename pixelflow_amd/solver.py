@@ -71,6 +71,30 @@ EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set
            "pf_serve_counters", "pf_serve_error")
 
 
+
+def _bind_serve(L: ctypes.CDLL) -> None:
+    """Signatures of the serving calls (pf_serve_*) and the batch calls they rely on."""
+    vp, dp, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+    L.pf_batch_last_error.restype = ctypes.c_char_p
+    L.pf_batch_last_error.argtypes = [vp]
+    L.pf_serve_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.pf_serve_destroy.argtypes = [vp]
+    L.pf_serve_batch.restype = vp
+    L.pf_serve_batch.argtypes = [vp]
+    L.pf_serve_submit.argtypes = [vp, ctypes.POINTER(PfDesc), dp, ctypes.c_double, ctypes.c_int64, ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int64)]
+    L.pf_serve_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(PfStats)]
+    L.pf_serve_worst.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(PfStats)]
+    L.pf_serve_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.pf_serve_times.argtypes = [vp, ctypes.c_int64, dp, dp, dp]
+    L.pf_serve_result.argtypes = [vp, ctypes.c_int64, ctypes.c_int, dp, i64, i64]
+    L.pf_serve_release.argtypes = [vp, ctypes.c_int64]
+    L.pf_serve_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.pf_serve_error.restype = ctypes.c_char_p
+    L.pf_serve_error.argtypes = [vp, ctypes.c_int64]
+
 def lib() -> ctypes.CDLL:
     """Load libpixelflow.so (build with ``python -m pixelflow_amd.build``)."""
     global _lib
@@ -122,23 +146,7 @@ def lib() -> ctypes.CDLL:
     L.pf_batch_case_steps.restype = ctypes.c_int64
     L.pf_batch_case_steps.argtypes = [vp, ctypes.c_int]
     L.pf_device_memory.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
-    L.pf_serve_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(vp)]
-    L.pf_serve_destroy.argtypes = [vp]
-    L.pf_serve_batch.restype = vp
-    L.pf_serve_batch.argtypes = [vp]
-    L.pf_serve_submit.argtypes = [vp, ctypes.POINTER(PfDesc), dp, ctypes.c_double, ctypes.c_int64, ctypes.c_int32,
-                                  ctypes.POINTER(ctypes.c_int64)]
-    L.pf_serve_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
-                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(PfStats)]
-    L.pf_serve_worst.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(PfStats)]
-    L.pf_serve_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
-    L.pf_serve_times.argtypes = [vp, ctypes.c_int64, dp, dp, dp]
-    L.pf_serve_result.argtypes = [vp, ctypes.c_int64, ctypes.c_int, dp, i64, i64]
-    L.pf_serve_release.argtypes = [vp, ctypes.c_int64]
-    L.pf_serve_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int),
-                                    ctypes.POINTER(ctypes.c_int)]
-    L.pf_serve_error.restype = ctypes.c_char_p
-    L.pf_serve_error.argtypes = [vp, ctypes.c_int64]
+    _bind_serve(L)
     _lib = L
     return L
 
@@ -420,31 +428,56 @@ class Batch:
 SERVE_QUEUED, SERVE_RUNNING, SERVE_DONE, SERVE_FAILED = 0, 1, 2, 3
 
 
+SERVE_STAND_IN_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "test", "libpfserve_fake.so")
+
+
+def serve_stand_in() -> ctypes.CDLL:
+    """The library's serving loop (pf_serve.cpp) linked with a stand-in batch, for host
+    tests of the loop's policy (tests/native/serve_fake_batch.cpp; built by
+    ``python -m pixelflow_amd.build``).  Test infrastructure: no GPU work is done."""
+    if not os.path.exists(SERVE_STAND_IN_PATH):
+        raise PixelFlowError(f"stand-in serving library not built: {SERVE_STAND_IN_PATH}")
+    L = ctypes.CDLL(SERVE_STAND_IN_PATH)
+    _bind_serve(L)
+    L.fake_reset.argtypes = [ctypes.c_int64]
+    L.fake_set_gate.argtypes = [ctypes.c_int]
+    L.fake_hold_adds.argtypes = [ctypes.c_int]
+    L.fake_grant_steps.argtypes = [ctypes.c_int]
+    L.fake_wait_step_calls.argtypes = [ctypes.c_int, ctypes.c_double]
+    L.fake_wait_blocked.argtypes = [ctypes.c_double]
+    L.fake_diverge.argtypes = [ctypes.c_int, ctypes.c_int64]
+    L.fake_log.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    return L
+
+
 class NativeServer:
     """The library's continuous-batching server (pf_serve_*): a worker thread inside the
     library admits submitted cases as slots and arena memory free up and advances them
     in shared step chunks; results are kept per request until released."""
 
     def __init__(self, kind: CaseParams, max_cases: int, arena_bytes: int = 0, chunk: int = 5,
-                 precision: int = PF_PREC_FP64, batch_config: Optional[dict] = None, **kw):
+                 precision: int = PF_PREC_FP64, batch_config: Optional[dict] = None, library: ctypes.CDLL = None, **kw):
+        """`library`: the library holding pf_serve_* (default libpixelflow.so; the tests pass
+        the serving loop built over a stand-in batch, see serve_stand_in)."""
+        self.L = library if library is not None else lib()
         self.kw = dict(precision=precision, **kw)
         d = make_desc(kind, **self.kw)
         h = ctypes.c_void_p()
-        rc = lib().pf_serve_create(ctypes.byref(d), int(max_cases), int(arena_bytes), int(chunk), ctypes.byref(h))
+        rc = self.L.pf_serve_create(ctypes.byref(d), int(max_cases), int(arena_bytes), int(chunk), ctypes.byref(h))
         if rc != 0:
-            raise PixelFlowError(f"pf_serve_create: {ERRORS.get(rc, rc)}: {lib().pf_batch_last_error(None).decode()}")
+            raise PixelFlowError(f"pf_serve_create: {ERRORS.get(rc, rc)}: {self.L.pf_batch_last_error(None).decode()}")
         self._h = h
         self.params: Dict[int, CaseParams] = {}
         if batch_config:
             c = PfBatchConfig(int(batch_config.get("persistent_blocks", 0)), 1 if batch_config.get("use_graph", True) else 0,
                               int(batch_config.get("inner_iters", 0)), int(batch_config.get("tile_x", 0)))
-            rc = lib().pf_batch_set_config(lib().pf_serve_batch(self._h), ctypes.byref(c))
+            rc = self.L.pf_batch_set_config(self.L.pf_serve_batch(self._h), ctypes.byref(c))
             if rc != 0:
                 raise PixelFlowError(f"pf_batch_set_config: {ERRORS.get(rc, rc)}")
 
     def _check(self, rc: int, what: str, rid: int = -1) -> None:
         if rc != 0:
-            raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {lib().pf_serve_error(self._h, rid).decode()}")
+            raise PixelFlowError(f"{what}: {ERRORS.get(rc, rc)}: {self.L.pf_serve_error(self._h, rid).decode()}")
 
     def case_bytes(self, p: CaseParams) -> int:
         return batch_case_bytes(p, **self.kw)
@@ -459,7 +492,7 @@ class NativeServer:
             if not (f == "w" and p.dim == 2):
                 mask |= 1 << FIELDS[f]
         rid = ctypes.c_int64()
-        self._check(lib().pf_serve_submit(self._h, ctypes.byref(d), _dp(raw), float(p.threshold), int(nsteps), mask,
+        self._check(self.L.pf_serve_submit(self._h, ctypes.byref(d), _dp(raw), float(p.threshold), int(nsteps), mask,
                                           ctypes.byref(rid)), "pf_serve_submit")
         self.params[rid.value] = p
         return rid.value
@@ -467,7 +500,7 @@ class NativeServer:
     def poll(self, rid: int):
         """(state, steps done, SOR iterations over them, statistics of the latest step)."""
         st, done, its, last = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), PfStats()
-        self._check(lib().pf_serve_poll(self._h, int(rid), ctypes.byref(st), ctypes.byref(done), ctypes.byref(its),
+        self._check(self.L.pf_serve_poll(self._h, int(rid), ctypes.byref(st), ctypes.byref(done), ctypes.byref(its),
                                         ctypes.byref(last)), "pf_serve_poll", rid)
         return st.value, done.value, its.value, {"sor_error": last.sor_error, "residual": last.residual,
                                                  "sor_iters": last.sor_iters, "cycles": last.cycles}
@@ -475,24 +508,24 @@ class NativeServer:
     def worst(self, rid: int) -> Dict[str, float]:
         """Per field, the maximum of the request's per-step statistics so far."""
         w = PfStats()
-        self._check(lib().pf_serve_worst(self._h, int(rid), ctypes.byref(w)), "pf_serve_worst", rid)
+        self._check(self.L.pf_serve_worst(self._h, int(rid), ctypes.byref(w)), "pf_serve_worst", rid)
         return {"sor_error": w.sor_error, "residual": w.residual, "sor_iters": w.sor_iters, "cycles": w.cycles}
 
     def wait(self, rid: int, timeout: float = -1.0) -> int:
         st = ctypes.c_int()
-        self._check(lib().pf_serve_wait(self._h, int(rid), float(timeout), ctypes.byref(st)), "pf_serve_wait", rid)
+        self._check(self.L.pf_serve_wait(self._h, int(rid), float(timeout), ctypes.byref(st)), "pf_serve_wait", rid)
         return st.value
 
     def times(self, rid: int):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        self._check(lib().pf_serve_times(self._h, int(rid), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+        self._check(self.L.pf_serve_times(self._h, int(rid), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
                     "pf_serve_times", rid)
         return a.value, b.value, c.value
 
     def result(self, rid: int, name: str) -> np.ndarray:
         p = self.params[rid]
         out = np.zeros(p.full_shape(), dtype=np.float64)
-        self._check(lib().pf_serve_result(self._h, int(rid), FIELDS[name], _dp(out), p.m + 2, p.n + 2),
+        self._check(self.L.pf_serve_result(self._h, int(rid), FIELDS[name], _dp(out), p.m + 2, p.n + 2),
                     "pf_serve_result", rid)
         return out
 
@@ -501,21 +534,21 @@ class NativeServer:
         return a[1:-1, 1:-1, 1:-1].copy() if self.params[rid].dim == 3 else a[1:-1, 1:-1].copy()
 
     def error(self, rid: int) -> str:
-        return lib().pf_serve_error(self._h, int(rid)).decode()
+        return self.L.pf_serve_error(self._h, int(rid)).decode()
 
     def release(self, rid: int) -> None:
-        self._check(lib().pf_serve_release(self._h, int(rid)), "pf_serve_release", rid)
+        self._check(self.L.pf_serve_release(self._h, int(rid)), "pf_serve_release", rid)
         self.params.pop(rid, None)
 
     def counters(self) -> Dict[str, float]:
         steps, busy, q, r = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
-        self._check(lib().pf_serve_counters(self._h, ctypes.byref(steps), ctypes.byref(busy), ctypes.byref(q),
+        self._check(self.L.pf_serve_counters(self._h, ctypes.byref(steps), ctypes.byref(busy), ctypes.byref(q),
                                             ctypes.byref(r)), "pf_serve_counters")
         return {"steps_run": steps.value, "busy_s": busy.value, "queued": q.value, "running": r.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            lib().pf_serve_destroy(self._h)
+            self.L.pf_serve_destroy(self._h)
             self._h = None
 
     def __enter__(self):
