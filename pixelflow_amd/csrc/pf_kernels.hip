@@ -759,14 +759,19 @@ void launch_clear_u64(hipStream_t st, unsigned long long *p, int count) {
 }
 
 // Plain device copy (bandwidth calibration: a kernel whose HBM bytes are known exactly).
+// One 16-byte element per thread over a grid that covers the array: on MI355X this form
+// copies 1-4 GB at 6.2-6.5 TB/s, where a grid-stride loop over 8192 blocks reached
+// 4.5-4.9 TB/s.
 __global__ void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n2) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (int64_t)gridDim.x * blockDim.x)
-        dst[t] = src[t];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) dst[t] = src[t];
 }
 
 void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n) {
     const int64_t n2 = n / 2;
-    hipLaunchKernelGGL(k_copy_probe, dim3(256 * 8 * 4), dim3(256), 0, st, (const double2 *)src, (double2 *)dst, n2);
+    const int64_t blocks = (n2 + 255) / 256;
+    hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)blocks), dim3(256), 0, st, (const double2 *)src, (double2 *)dst,
+                       n2);
 }
 
 }  // namespace pf
