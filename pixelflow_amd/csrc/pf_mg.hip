@@ -425,12 +425,26 @@ constexpr int MG_TAIL_THREADS = 1024;
 constexpr int MG_TAIL_U = 8;
 constexpr int64_t MG_TAIL_CELLS = (int64_t)MG_TAIL_THREADS * MG_TAIL_U;
 
+// Division of a cell index by a level's extent inside the tail: x / d = umulhi(x, M) with
+// M = floor(2^32 / d) + 1, exact whenever x * d < 2^32 (tail levels hold at most
+// MG_TAIL_CELLS cells, so x and d are below 2^13); M = 0 stands for d = 1.  The host
+// checks every quotient a tail level uses against true division (tail_magic_ok).
+PF_HD uint32_t tail_magic(uint32_t d) { return d <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / d + 1); }
+PF_HD uint32_t tail_div(uint32_t x, uint32_t magic) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return magic ? __umulhi(x, magic) : x;
+#else
+    return magic ? (uint32_t)(((uint64_t)x * magic) >> 32) : x;
+#endif
+}
+
 template <typename T>
 struct MgTailLevel {
     MgGeom g;
     MgOp<T> A;
     const uint8_t *keep;
     T *r, *e;
+    uint32_t mm, mh, mn;    // tail_magic of m, (m + 1) / 2 and n
 };
 
 template <typename T>
@@ -443,45 +457,83 @@ struct MgTailArgs {
     const T *cinv;          // coarsest inverse, transposed: cinv[j * n + i] = (A^-1)(i, j)
 };
 
+// mg_off in 32 bits (a tail level's arrays hold far fewer than 2^32 elements).
+__device__ __forceinline__ uint32_t tail_off(const MgGeom &g, int I, int J, int K) {
+    return (uint32_t)(I + 1) + (uint32_t)g.PX * ((uint32_t)(J + 1) + (uint32_t)g.Y * (uint32_t)(K + g.KG));
+}
+
+// Cell q of a level in natural order (I fastest).
 template <typename T>
-__device__ __forceinline__ void tail_refresh(const MgGeom &g, T *__restrict__ e) {
+__device__ __forceinline__ void tail_cell(const MgTailLevel<T> &L, uint32_t q, int &I, int &J, int &K) {
+    const uint32_t row = tail_div(q, L.mm);
+    I = (int)(q - row * (uint32_t)L.g.m);
+    K = (int)tail_div(row, L.mn);
+    J = (int)(row - (uint32_t)K * (uint32_t)L.g.n);
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_store(const MgGeom &g, T *__restrict__ e, int I, int J, int K, T v, bool my,
+                                           bool mz) {
+    e[tail_off(g, I, J, K)] = v;
+    if (my) {
+        if (J == 0) e[tail_off(g, I, g.n, K)] = v;
+        if (J == g.n - 1) e[tail_off(g, I, -1, K)] = v;
+    }
+    if (mz && g.dim == 3) {
+        if (K == 0) e[tail_off(g, I, J, g.l)] = v;
+        if (K == g.l - 1) e[tail_off(g, I, J, -1)] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_refresh(const MgTailLevel<T> &L, T *__restrict__ e) {
+    const MgGeom &g = L.g;
     if (!(g.ref_y || g.ref_z)) return;
-    const int64_t ny = (int64_t)g.m * g.l, nz = (int64_t)g.m * g.n;
-    const int64_t nmax = ny > nz ? ny : nz;
-    for (int64_t q = threadIdx.x; q < nmax; q += MG_TAIL_THREADS) {
+    const uint32_t ny = (uint32_t)g.m * g.l, nz = (uint32_t)g.m * g.n;
+    const uint32_t nmax = ny > nz ? ny : nz;
+    for (uint32_t q = threadIdx.x; q < nmax; q += MG_TAIL_THREADS) {
+        const uint32_t t = tail_div(q, L.mm);
+        const int I = (int)(q - t * (uint32_t)g.m);
         if (g.ref_y && q < ny) {
-            const int I = (int)(q % g.m), K = (int)(q / g.m);
-            e[mg_off(g, I, -1, K)] = e[mg_off(g, I, g.n - 1, K)];
-            e[mg_off(g, I, g.n, K)] = e[mg_off(g, I, 0, K)];
+            const int K = (int)t;
+            e[tail_off(g, I, -1, K)] = e[tail_off(g, I, g.n - 1, K)];
+            e[tail_off(g, I, g.n, K)] = e[tail_off(g, I, 0, K)];
         }
         if (g.ref_z && g.dim == 3 && q < nz) {
-            const int I = (int)(q % g.m), J = (int)(q / g.m);
-            e[mg_off(g, I, J, -1)] = e[mg_off(g, I, J, g.l - 1)];
-            e[mg_off(g, I, J, g.l)] = e[mg_off(g, I, J, 0)];
+            const int J = (int)t;
+            e[tail_off(g, I, J, -1)] = e[tail_off(g, I, J, g.l - 1)];
+            e[tail_off(g, I, J, g.l)] = e[tail_off(g, I, J, 0)];
         }
     }
     __syncthreads();
 }
 
+// One colour of the level's Gauss-Seidel sweep (omega = 1): the threads walk that colour's
+// cells only, (m + 1) / 2 slots per row, I = 2 h + ((colour + J + K) & 1).
 template <typename T>
 __device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour) {
     const MgGeom &g = L.g;
-    const int64_t cells = (int64_t)g.m * g.n * g.l;
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
-    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+    const uint32_t hm = (uint32_t)(g.m + 1) >> 1;
+    const uint32_t count = hm * (uint32_t)g.n * (uint32_t)g.l;
+    const uint32_t sy = (uint32_t)g.PX, sz = (uint32_t)g.PX * (uint32_t)g.Y;
+    for (uint32_t base = threadIdx.x; base < count; base += (uint32_t)MG_TAIL_THREADS * MG_TAIL_U) {
         T v[MG_TAIL_U];
         int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
         bool act[MG_TAIL_U];
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u) {
-            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            const uint32_t x = base + (uint32_t)u * MG_TAIL_THREADS;
             act[u] = false;
             v[u] = T(0);
-            if (q >= cells) continue;
-            mg_cell(g, q, I[u], J[u], K[u]);
-            if (((I[u] + J[u] + K[u]) & 1) != colour) continue;
+            if (x >= count) continue;
+            const uint32_t row = tail_div(x, L.mh);
+            const int h = (int)(x - row * hm);
+            K[u] = (int)tail_div(row, L.mn);
+            J[u] = (int)(row - (uint32_t)K[u] * (uint32_t)g.n);
+            I[u] = 2 * h + ((colour + J[u] + K[u]) & 1);
+            if (I[u] >= g.m) continue;
             act[u] = true;
-            const int64_t o = mg_off(g, I[u], J[u], K[u]);
+            const uint32_t o = tail_off(g, I[u], J[u], K[u]);
             T s = L.r[o];
             s = s - L.A.cE[o] * L.e[o + 1];
             s = s - L.A.cW[o] * L.e[o - 1];
@@ -495,43 +547,58 @@ __device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour)
         }
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u)
-            if (act[u]) mg_store(g, L.e, I[u], J[u], K[u], v[u], g.fuse_y, g.fuse_z);
+            if (act[u]) tail_store(g, L.e, I[u], J[u], K[u], v[u], g.fuse_y, g.fuse_z);
     }
     __syncthreads();
-    tail_refresh(g, L.e);
+    tail_refresh(L, L.e);
+}
+
+template <typename T>
+__device__ __forceinline__ T tail_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, uint32_t o) {
+    const uint32_t sy = (uint32_t)g.PX, sz = (uint32_t)g.PX * (uint32_t)g.Y;
+    T s = A.ap[o] * e[o];
+    s = s + A.cE[o] * e[o + 1];
+    s = s + A.cW[o] * e[o - 1];
+    s = s + A.cN[o] * e[o + sy];
+    s = s + A.cS[o] * e[o - sy];
+    if (g.dim == 3) {
+        s = s + A.cT[o] * e[o + sz];
+        s = s + A.cB[o] * e[o - sz];
+    }
+    return s;
 }
 
 template <typename T>
 __device__ __forceinline__ void tail_restrict(const MgTailLevel<T> &F, const MgTailLevel<T> &C) {
     const MgGeom &f = F.g, &c = C.g;
-    const int64_t cells = (int64_t)c.m * c.n * c.l;
-    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+    const uint32_t cells = (uint32_t)c.m * c.n * c.l;
+    for (uint32_t base = threadIdx.x; base < cells; base += (uint32_t)MG_TAIL_THREADS * MG_TAIL_U) {
         T v[MG_TAIL_U];
         int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u) {
-            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            const uint32_t q = base + (uint32_t)u * MG_TAIL_THREADS;
             v[u] = T(0);
             if (q >= cells) continue;
-            mg_cell(c, q, I[u], J[u], K[u]);
+            tail_cell(C, q, I[u], J[u], K[u]);
             T sum = T(0);
             for (int dz = 0; dz < (f.dim == 3 ? 2 : 1); ++dz)
                 for (int dy = 0; dy < 2; ++dy)
                     for (int dx = 0; dx < 2; ++dx) {
                         const int i = 2 * I[u] + dx, j = 2 * J[u] + dy, k = (f.dim == 3) ? 2 * K[u] + dz : 0;
                         if (i >= f.m || j >= f.n || k >= f.l) continue;
-                        const int64_t o = mg_off(f, i, j, k);
+                        const uint32_t o = tail_off(f, i, j, k);
                         if (!F.keep[o]) continue;
-                        sum = sum + (F.r[o] - mg_ax(f, F.A, F.e, o));
+                        sum = sum + (F.r[o] - tail_ax(f, F.A, F.e, o));
                     }
             v[u] = sum;
         }
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u) {
-            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            const uint32_t q = base + (uint32_t)u * MG_TAIL_THREADS;
             if (q >= cells) continue;
-            C.r[mg_off(c, I[u], J[u], K[u])] = v[u];
-            mg_store(c, C.e, I[u], J[u], K[u], T(0), c.per, c.per);
+            C.r[tail_off(c, I[u], J[u], K[u])] = v[u];
+            tail_store(c, C.e, I[u], J[u], K[u], T(0), c.per, c.per);
         }
     }
     __syncthreads();
@@ -540,29 +607,29 @@ __device__ __forceinline__ void tail_restrict(const MgTailLevel<T> &F, const MgT
 template <typename T>
 __device__ __forceinline__ void tail_prolong(const MgTailLevel<T> &F, const MgTailLevel<T> &C, T alpha) {
     const MgGeom &f = F.g, &c = C.g;
-    const int64_t cells = (int64_t)f.m * f.n * f.l;
-    for (int64_t base = threadIdx.x; base < cells; base += (int64_t)MG_TAIL_THREADS * MG_TAIL_U) {
+    const uint32_t cells = (uint32_t)f.m * f.n * f.l;
+    for (uint32_t base = threadIdx.x; base < cells; base += (uint32_t)MG_TAIL_THREADS * MG_TAIL_U) {
         T v[MG_TAIL_U];
         int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
         bool act[MG_TAIL_U];
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u) {
-            const int64_t q = base + (int64_t)u * MG_TAIL_THREADS;
+            const uint32_t q = base + (uint32_t)u * MG_TAIL_THREADS;
             act[u] = false;
             v[u] = T(0);
             if (q >= cells) continue;
-            mg_cell(f, q, I[u], J[u], K[u]);
-            const int64_t o = mg_off(f, I[u], J[u], K[u]);
+            tail_cell(F, q, I[u], J[u], K[u]);
+            const uint32_t o = tail_off(f, I[u], J[u], K[u]);
             if (!F.keep[o]) continue;
             act[u] = true;
-            v[u] = F.e[o] + alpha * C.e[mg_off(c, I[u] >> 1, J[u] >> 1, (f.dim == 3) ? K[u] >> 1 : 0)];
+            v[u] = F.e[o] + alpha * C.e[tail_off(c, I[u] >> 1, J[u] >> 1, (f.dim == 3) ? K[u] >> 1 : 0)];
         }
 #pragma unroll
         for (int u = 0; u < MG_TAIL_U; ++u)
-            if (act[u]) mg_store(f, F.e, I[u], J[u], K[u], v[u], f.fuse_y, f.fuse_z);
+            if (act[u]) tail_store(f, F.e, I[u], J[u], K[u], v[u], f.fuse_y, f.fuse_z);
     }
     __syncthreads();
-    tail_refresh(f, F.e);
+    tail_refresh(F, F.e);
 }
 
 // e = A^-1 r on the coarsest level (n <= MG_COARSEST_CELLS cells).
@@ -573,41 +640,43 @@ __device__ __forceinline__ void tail_coarsest(const MgTailLevel<T> &L, const T *
     const int n = g.m * g.n * g.l;
     for (int q = threadIdx.x; q < n; q += MG_TAIL_THREADS) {
         int I, J, K;
-        mg_cell(g, q, I, J, K);
-        rl[q] = L.r[mg_off(g, I, J, K)];
+        tail_cell(L, (uint32_t)q, I, J, K);
+        rl[q] = L.r[tail_off(g, I, J, K)];
     }
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += MG_TAIL_THREADS) {
         T s = T(0);
-        for (int j = 0; j < n; ++j) s = s + cinv[(int64_t)j * n + q] * rl[j];
+        for (int j = 0; j < n; ++j) s = s + cinv[(uint32_t)j * (uint32_t)n + (uint32_t)q] * rl[j];
         int I, J, K;
-        mg_cell(g, q, I, J, K);
-        mg_store(g, L.e, I, J, K, s, g.fuse_y, g.fuse_z);
+        tail_cell(L, (uint32_t)q, I, J, K);
+        tail_store(g, L.e, I, J, K, s, g.fuse_y, g.fuse_z);
     }
     __syncthreads();
-    tail_refresh(g, L.e);
+    tail_refresh(L, L.e);
 }
 
 // The cycle from tail level Q down: pre-smoothing, restriction, `gamma` cycles of the next
 // level (1: V-cycle, 2: W-cycle; a later visit continues from the correction the earlier
 // one left), prolongation, post-smoothing.  Unrolled over the level depth at compile time.
 template <typename T, int Q>
-__device__ void tail_cycle(const MgTailArgs<T> &t) {
+__device__ __forceinline__ void tail_cycle(const MgTailArgs<T> &t) {
     if constexpr (Q < MG_TAIL_LEVELS) {
         if (Q == t.nl - 1) {
             tail_coarsest(t.L[Q], t.cinv);
             return;
         }
-        for (int s = 0; s < t.nuc; ++s) {
-            tail_smooth(t.L[Q], 0);
-            tail_smooth(t.L[Q], 1);
-        }
-        tail_restrict(t.L[Q], t.L[Q + 1]);
-        for (int c = 0; c < (Q + 2 == t.nl ? 1 : t.gamma); ++c) tail_cycle<T, Q + 1>(t);
-        tail_prolong(t.L[Q], t.L[Q + 1], t.alpha);
-        for (int s = 0; s < t.nuc; ++s) {
-            tail_smooth(t.L[Q], 1);
-            tail_smooth(t.L[Q], 0);
+        if constexpr (Q + 1 < MG_TAIL_LEVELS) {
+            for (int s = 0; s < t.nuc; ++s) {
+                tail_smooth(t.L[Q], 0);
+                tail_smooth(t.L[Q], 1);
+            }
+            tail_restrict(t.L[Q], t.L[Q + 1]);
+            for (int c = 0; c < (Q + 2 == t.nl ? 1 : t.gamma); ++c) tail_cycle<T, Q + 1>(t);
+            tail_prolong(t.L[Q], t.L[Q + 1], t.alpha);
+            for (int s = 0; s < t.nuc; ++s) {
+                tail_smooth(t.L[Q], 1);
+                tail_smooth(t.L[Q], 0);
+            }
         }
     }
 }
@@ -615,6 +684,23 @@ __device__ void tail_cycle(const MgTailArgs<T> &t) {
 template <typename T>
 __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(MgTailArgs<T> t) {
     tail_cycle<T, 0>(t);
+}
+
+// Every quotient the tail takes on a level (cell index by m, by (m + 1) / 2 and the row by
+// n, the refresh's index by m) equals true division.
+static bool tail_magic_ok(const MgGeom &g) {
+    const uint32_t m = (uint32_t)g.m, n = (uint32_t)g.n, hm = (m + 1) / 2;
+    const uint32_t cells = m * n * (uint32_t)g.l;
+    const uint32_t mm = tail_magic(m), mh = tail_magic(hm), mn = tail_magic(n);
+    const uint32_t rows = n * (uint32_t)g.l;
+    const uint32_t span = cells > hm * rows ? cells : hm * rows;
+    for (uint32_t x = 0; x < span; ++x) {
+        if (tail_div(x, mm) != x / m) return false;
+        if (tail_div(x, mh) != x / hm) return false;
+    }
+    for (uint32_t r = 0; r < rows; ++r)
+        if (tail_div(r, mn) != r / n) return false;
+    return true;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1009,7 +1095,9 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
         const bool tail_on = !(tv && *tv && std::atoi(tv) == 0);
         for (size_t q = 1; tail_on && q < L.size(); ++q)
             if ((int64_t)L[q].m * L[q].n * L[q].l <= MG_TAIL_CELLS) {
-                if (L.size() - q <= (size_t)MG_TAIL_LEVELS && n <= MG_COARSEST_CELLS) S.tail_q = (int)q;
+                bool ok = L.size() - q <= (size_t)MG_TAIL_LEVELS && n <= MG_COARSEST_CELLS;
+                for (size_t k = q; ok && k < L.size(); ++k) ok = tail_magic_ok(L[k]);
+                if (ok) S.tail_q = (int)q;
                 break;
             }
     }
@@ -1204,7 +1292,9 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
         t.nl = (int)(S.lev.size() - q);
         for (int k = 0; k < t.nl; ++k) {
             MgLevel<T> &lv = S.lev[q + (size_t)k];
-            t.L[k] = MgTailLevel<T>{lv.g, op_of(lv), (const uint8_t *)lv.keep, lv.r, lv.e};
+            const uint32_t m = (uint32_t)lv.g.m;
+            t.L[k] = MgTailLevel<T>{lv.g, op_of(lv), (const uint8_t *)lv.keep, lv.r, lv.e, tail_magic(m),
+                                    tail_magic((m + 1) / 2), tail_magic((uint32_t)lv.g.n)};
         }
         t.nuc = S.prm.nuc;
         t.gamma = S.prm.tail_gamma;
