@@ -125,8 +125,9 @@ def cpu_baseline_reference(p, phi, steps: int, threads: int, main: str = "ref_ib
     output goes to a pseudo-terminal, so the Fortran runtime writes each log line as it
     happens; the time between the first and the last per-step 'SOR iteration' line is
     `steps` whole steps, without the CSV read before them or the output files after (the
-    process is stopped once the last step is logged).  Returns (cell-steps/s, seconds) or
-    None when the build is absent."""
+    process is stopped once the last step is logged).  Returns (cell-steps/s, seconds), or
+    None when the build is absent or the lines came too close together to time (lines read
+    together carry one time stamp)."""
     import dataclasses
     import pty
     import select
@@ -173,6 +174,8 @@ def cpu_baseline_reference(p, phi, steps: int, threads: int, main: str = "ref_ib
     if len(stamps) < steps + 1:
         return None
     dt = stamps[-1] - stamps[0]
+    if dt <= 0.:
+        return None
     return p.cells * steps / dt, dt
 
 

@@ -28,7 +28,8 @@ struct pf_ctx {
     double *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     double *phi = nullptr, *div = nullptr, *dout = nullptr;
     double *force_buf = nullptr;   // body force: per-block partial sums, then the 6 totals
-    double *P2 = nullptr;          // fused sweep: second pressure buffer (ping-pong)
+    double *P2 = nullptr;          // fused and 2D blocked sweeps: second pressure buffer (ping-pong)
+    int tb2_k = 0, tb2_r = 0;      // 2D blocked sweep: iterations per launch, rows per block (0: off)
     float *E2 = nullptr;           // fused sweep, mixed precision: second correction buffer
     bool fused = false;            // one launch per SOR iteration (PF_FUSED=1; see pf_fused.h)
     // colour-split layout (2 * HS doubles each)
@@ -97,6 +98,24 @@ static bool desc_ok(const pf_desc *d, std::string &why) { return pf::desc_valid(
 
 extern "C" int pf_abi_version(void) { return PF_ABI_VERSION; }
 
+// The temporally blocked 2D sweep's iterations per launch and rows per block, or 0 and 0
+// where it is not used: FP64, 2D, and the shape within what the kernel holds.  By default 2
+// iterations and 1 row (on the 256x128 cylinder, ms per step: two-launch sweeps 0.86, K,R =
+// 1,1 0.60, 2,1 0.47, 2,2 0.54, 3,1 0.48, 4,1 0.52 with porosity in LDS, 4,8 0.80); rows
+// wider than about 310 cells do not fit LDS and keep the two-launch sweeps.
+// PF_SOR_TB2="K,R" sets them ("0,0": off).
+static void tb2_params(const PfGeom &g, const pf_desc *d, int &K, int &R) {
+    K = R = 0;
+    if (d->precision != PF_PREC_FP64 || g.dim != 2) return;
+    int k = 2, r = 1;
+    const char *e = std::getenv("PF_SOR_TB2");
+    if (e && *e && std::sscanf(e, "%d,%d", &k, &r) != 2) k = r = 0;
+    if (k > 0 && r > 0 && pf::sor_tb2_ok(g, d->bc, k, r)) {
+        K = k;
+        R = r;
+    }
+}
+
 extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     std::string why;
     if (!desc_ok(d, why)) return -1;
@@ -112,7 +131,10 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     const bool code_on = (pc && *pc) ? std::atoi(pc) != 0 : interior * 24 > ((int64_t)512 << 20);
     const int64_t coded =
         (d->precision == PF_PREC_FP64 && code_on) ? (int64_t)4 * 2 * g.HS + (int64_t)8 * PHI_TAB_MAX : 0;
-    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + 4096;
+    int tk = 0, tr = 0;
+    tb2_params(g, d, tk, tr);
+    const int64_t p2 = (tk > 0) ? (int64_t)8 * 2 * g.HS : 0;
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + p2 + 4096;
 }
 
 static int alloc(pf_ctx *c, double **p, int64_t n) {
@@ -199,7 +221,9 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         if (fv && *fv) c->fused = std::atoi(fv) != 0;
         c->fused = c->fused && pf::sor_fused_ok(g, desc->bc);
     }
-    if (c->fused && desc->precision == PF_PREC_FP64 && (rc = alloc(c, &c->P2, 2 * g.HS)) != PF_OK) goto bad;
+    tb2_params(g, desc, c->tb2_k, c->tb2_r);
+    if ((c->fused || c->tb2_k > 0) && desc->precision == PF_PREC_FP64 && (rc = alloc(c, &c->P2, 2 * g.HS)) != PF_OK)
+        goto bad;
     if (desc->precision == PF_PREC_MIXED) {
         float **f32[] = {&c->E32, &c->R32, &c->PHI32, &c->E2};
         for (float **p : f32) {
@@ -466,7 +490,34 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
     // 3D sweeps (i+j+k) even first (:493), 2D sweeps (i+j) odd first (:339)
     const int first = (g.dim == 3) ? 0 : 1;
     int done = 0;
-    if (c->fused && c->sor_variant != 0 && iters >= 2) {
+    if (c->tb2_k > 0 && c->sor_variant != 0 && iters >= 2) {
+        // K iterations per launch, P -> P2 -> P ...: an even number of launches leaves the
+        // result in P, so a launch of two or more iterations is split when the count is odd
+        std::vector<int> ks;
+        for (int left = iters; left > 0;) {
+            const int k = std::min(left, c->tb2_k);
+            ks.push_back(k);
+            left -= k;
+        }
+        if (ks.size() % 2 == 1)
+            for (size_t q = ks.size(); q-- > 0;)
+                if (ks[q] >= 2) {
+                    const int half = ks[q] / 2;
+                    ks[q] -= half;
+                    ks.insert(ks.begin() + (std::ptrdiff_t)q + 1, half);
+                    break;
+                }
+        if (ks.size() % 2 == 0) {
+            for (size_t q = 0; q < ks.size(); ++q) {
+                const double *in = (q % 2 == 0) ? c->P : c->P2;
+                double *out = (q % 2 == 0) ? c->P2 : c->P;
+                pf::launch_sor_tb2(c->st, g, c->a, first, ks[q], c->tb2_r, in, out, c->PHIs, c->BB, c->phi, slot,
+                                   c->sor_variant == 2);
+            }
+            done = iters;
+        }
+    }
+    if (c->fused && c->sor_variant != 0 && iters - done >= 2) {
         // pairs of fused iterations P -> P2 -> P; P2 takes P's ghost values first (the
         // x ghosts and the air room's faces are not written by the sweep)
         (void)hipMemcpyAsync(c->P2, c->P, sizeof(double) * (size_t)(2 * g.HS), hipMemcpyDeviceToDevice, c->st);

@@ -326,13 +326,12 @@ __device__ __forceinline__ void rhs_cell(const PfGeom &g, const StepArgs &a, con
 // The SOR update of one cell from the values around it: pressure of the cell and its six
 // neighbours, porosity of the same seven cells, the RHS (3D :492-516 / :549-572, 2D
 // :339-352 / :373-386).  Shared by the two-launch sweep below and the fused sweep.
+// The cell's six coefficients after the boundary rows' rules, and ap (the sum before them).
 template <int DIM, bool FAST>
-__device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, int i, int j, int k, double pc,
-                                            double pW, double pE, double pS, double pN, double pB, double pT,
-                                            double fc, double fW, double fE, double fS, double fN, double fB,
-                                            double fT, double bb, const double *__restrict__ phi) {
+__device__ __forceinline__ double sor_coefs(const PfGeom &g, const StepArgs &a, int i, int j, int k, double fc,
+                                            double fW, double fE, double fS, double fN, double fB, double fT,
+                                            const double *__restrict__ phi, Coef &c) {
     const double dt = a.dt;
-    Coef c;
     c.ae = face_any<FAST>(fE, fc, dt, a.dx, a.rdx);
     c.aw = face_any<FAST>(fc, fW, dt, a.dx, a.rdx);
     c.an = face_any<FAST>(fN, fc, dt, a.dy, a.rdy);
@@ -348,12 +347,29 @@ __device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, 
     if (DIM == 3) ap = (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab;
     else ap = (((-c.ae) - c.aw) - c.an) - c.as;
     if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    return ap;
+}
+
+// The SOR update from the coefficients, ap and the neighbours.
+template <int DIM>
+__device__ __forceinline__ double sor_apply(const StepArgs &a, const Coef &c, double ap, double bb, double pc,
+                                            double pW, double pE, double pS, double pN, double pB, double pT) {
     double num;
     if (DIM == 3)
         num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
     else
         num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
     return ((num / ap) * a.omega) + (pc * a.omega_c);
+}
+
+template <int DIM, bool FAST>
+__device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, int i, int j, int k, double pc,
+                                            double pW, double pE, double pS, double pN, double pB, double pT,
+                                            double fc, double fW, double fE, double fS, double fN, double fB,
+                                            double fT, double bb, const double *__restrict__ phi) {
+    Coef c;
+    const double ap = sor_coefs<DIM, FAST>(g, a, i, j, k, fc, fW, fE, fS, fN, fB, fT, phi, c);
+    return sor_apply<DIM>(a, c, ap, bb, pc, pW, pE, pS, pN, pB, pT);
 }
 
 // The operands of one colour-sweep cell: the unknown at the cell and at its six neighbours

@@ -71,6 +71,18 @@ void launch_divergence_out(hipStream_t st, const PfGeom &g, const StepArgs &a, c
                            const double *v, const double *w, double *out);
 void launch_clear_u64(hipStream_t st, unsigned long long *p, int count);
 int sor_blocks(const PfGeom &g);
+// Temporally blocked 2D FP64 sweeps: K red/black iterations per launch, R owned rows per
+// block, from Pin into Pout (two pressure buffers, alternated by the caller).  sor_tb2_ok:
+// 2D, y periodic with n even, the held rows distinct and within TB2_LDS_MAX bytes of LDS.
+constexpr int64_t TB2_LDS_MAX = 160 * 1024 - 1024;
+constexpr int TB2_MAX_ROWS = 256;
+constexpr int TB2_THREADS = 1024;
+constexpr int TB2_ARRAYS = 14;   // LDS arrays per held cell slot: p, bb and 5 coefficients, per colour
+int64_t sor_tb2_lds_bytes(const PfGeom &g, int K, int R);
+bool sor_tb2_ok(const PfGeom &g, int bc, int K, int R);
+void launch_sor_tb2(hipStream_t st, const PfGeom &g, const StepArgs &a, int first, int K, int R, const double *Pin,
+                    double *Pout, const double *PHIs, const double *BB, const double *phi, unsigned long long *slot,
+                    bool fast);
 int sweep_wave_width(const PfGeom &g);   // 64 or 32 colour slots per wave row
 // One red/black iteration in one launch, ping-pong A -> B (pf_fused.h); usable when
 // sor_fused_ok (even periodic extents).  B must hold A's non-periodic ghost values.

@@ -274,6 +274,160 @@ __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs
     if (s.track_err) wave_max_to_slot(err, err_slot);
 }
 
+// ------------------------------------------------------------------------------------
+// Temporally blocked 2D colour sweeps (FP64, faces from porosity) for grids whose sweeps
+// are launch-bound.  A block owns R rows and holds them with 2K halo rows on each side
+// (rows wrap periodically in y; n even) in LDS: p and bb of both colours and every cell's
+// coefficients, built once per launch from porosity by sor_coefs.  It
+// runs K red/black iterations there: every half-sweep updates the rows whose neighbours
+// are still exact, one row fewer on each side per half-sweep, so after 2K half-sweeps the
+// owned rows hold what K iterations of the two-launch sweep give, bit for bit (the same
+// sor_value on the same operands; a half-sweep's updates read only the other colour).  The
+// owned rows and their periodic mirrors go to the other pressure buffer: blocks do not wait
+// for each other, so a block must not overwrite rows another block has yet to read.  |dp|
+// is folded over the owned rows only, for each of the 2K half-sweeps, as 2D tracks both
+// colours.  Rows are (r, h) = (x / H, x % H) with x / H = umulhi(x, hmul) (tb2_magic).
+// ------------------------------------------------------------------------------------
+PF_HD uint32_t tb2_magic(uint32_t d) { return d <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / d + 1); }
+
+template <bool FAST>
+__global__ __launch_bounds__(TB2_THREADS) void k_sor_tb2(PfGeom g, StepArgs a, int first, int K, int R,
+                                                         uint32_t hmul, const double *__restrict__ Pin,
+                                                         double *__restrict__ Pout, const double *__restrict__ PHIs,
+                                                         const double *__restrict__ BB,
+                                                         const double *__restrict__ phi,
+                                                         unsigned long long *err_slot) {
+    extern __shared__ double tb[];
+    __shared__ int jrow[TB2_MAX_ROWS];          // the grid row held in each local row
+    const int H = g.H, NR = R + 4 * K;
+    const uint32_t NH = (uint32_t)NR * (uint32_t)H;
+    // per colour: p, bb and the cell's coefficients (ae, aw, an, as after the boundary
+    // rows' rules, and ap), computed once per launch from porosity
+    double *lp[2], *lb[2], *le[2], *lw[2], *ln[2], *ls[2], *la[2];
+    for (int c = 0; c < 2; ++c) {
+        double *base = tb + (size_t)c * TB2_ARRAYS / 2 * NH;
+        lp[c] = base;
+        lb[c] = base + NH;
+        le[c] = base + 2 * NH;
+        lw[c] = base + 3 * NH;
+        ln[c] = base + 4 * NH;
+        ls[c] = base + 5 * NH;
+        la[c] = base + 6 * NH;
+    }
+    const int jb = (int)blockIdx.x * R + 1;   // the first owned row
+    const int v0 = jb - 2 * K;                // the row held in local row 0
+    const int64_t HS = g.HS, HP = g.HP;
+    const uint32_t tid = threadIdx.x;
+    for (int r = (int)tid; r < NR; r += TB2_THREADS) jrow[r] = ((v0 + r - 1) % g.n + g.n) % g.n + 1;
+    __syncthreads();
+    for (uint32_t x = tid; x < NH; x += TB2_THREADS) {
+        const uint32_t r = hmul ? __umulhi(x, hmul) : x;
+        const int h = (int)(x - r * (uint32_t)H);
+        const int j = jrow[r];
+        const int64_t o = (int64_t)h + HP * j;
+        lp[0][x] = Pin[o];
+        lp[1][x] = Pin[HS + o];
+        lb[0][x] = BB[o];
+        lb[1][x] = BB[HS + o];
+        if (r < 1 || (int)r > NR - 2) continue;   // the first and last local rows are never updated
+        const int64_t os = (int64_t)h + HP * jrow[r - 1], on = (int64_t)h + HP * jrow[r + 1];
+        for (int c = 0; c < 2; ++c) {
+            const int sh = (c + j) & 1;
+            const int i = 2 * h + sh;
+            if (i < 1 || i > g.m) continue;
+            const int64_t own = (int64_t)c * HS, oth = (int64_t)(1 - c) * HS;
+            Coef cf;
+            const double ap = sor_coefs<2, FAST>(g, a, i, j, 0, PHIs[own + o], PHIs[oth + o - 1 + sh],
+                                                 PHIs[oth + o + sh], PHIs[oth + os], PHIs[oth + on], 0., 0., phi, cf);
+            le[c][x] = cf.ae;
+            lw[c][x] = cf.aw;
+            ln[c][x] = cf.an;
+            ls[c][x] = cf.as;
+            la[c][x] = ap;
+        }
+    }
+    __syncthreads();
+    double err = 0.;
+    for (int s = 1; s <= 2 * K; ++s) {
+        const int c = ((s & 1) == 1) ? first : 1 - first;
+        const double *Xo = lp[1 - c];
+        double *Xc = lp[c];
+        const uint32_t x0 = (uint32_t)s * (uint32_t)H, x1 = (uint32_t)(NR - s) * (uint32_t)H;
+        for (uint32_t x = x0 + tid; x < x1; x += TB2_THREADS) {
+            const uint32_t r = hmul ? __umulhi(x, hmul) : x;
+            const int h = (int)(x - r * (uint32_t)H);
+            const int sh = (c + jrow[r]) & 1;
+            const int i = 2 * h + sh;
+            if (i < 1 || i > g.m) continue;
+            const double pc = Xc[x];
+            Coef cf;
+            cf.ae = le[c][x];
+            cf.aw = lw[c][x];
+            cf.an = ln[c][x];
+            cf.as = ls[c][x];
+            cf.at = cf.ab = 0.;
+            const double pn = sor_apply<2>(a, cf, la[c][x], lb[c][x], pc, Xo[x - 1 + sh], Xo[x + sh], Xo[x - H],
+                                           Xo[x + H], 0., 0.);
+            Xc[x] = pn;
+            if ((int)r >= 2 * K && (int)r < 2 * K + R) err = max_keep_nan(err, fabs(pn - pc));
+        }
+        __syncthreads();
+    }
+    // the owned rows (both colours, every slot) and their periodic mirrors
+    const uint32_t xo0 = (uint32_t)(2 * K) * (uint32_t)H, xo1 = (uint32_t)(2 * K + R) * (uint32_t)H;
+    for (uint32_t x = xo0 + tid; x < xo1; x += TB2_THREADS) {
+        const uint32_t r = hmul ? __umulhi(x, hmul) : x;
+        const uint32_t h = x - r * (uint32_t)H;
+        const int j = v0 + (int)r;
+        if (j > g.n) continue;
+        const int64_t o = (int64_t)h + HP * j;
+        Pout[o] = lp[0][x];
+        Pout[HS + o] = lp[1][x];
+        if (j == 1) {
+            Pout[o + HP * g.n] = lp[0][x];
+            Pout[HS + o + HP * g.n] = lp[1][x];
+        }
+        if (j == g.n) {
+            Pout[o - HP * g.n] = lp[0][x];
+            Pout[HS + o - HP * g.n] = lp[1][x];
+        }
+    }
+    wave_max_to_slot(err, err_slot);
+}
+
+int64_t sor_tb2_lds_bytes(const PfGeom &g, int K, int R) { return (int64_t)TB2_ARRAYS * (R + 4 * K) * g.H * 8; }
+
+bool sor_tb2_ok(const PfGeom &g, int bc, int K, int R) {
+    if (g.dim != 2 || bc != PF_BC_UNIFORM || (g.n % 2) != 0 || K < 1 || R < 1) return false;
+    if (R + 4 * K > g.n || R + 4 * K > TB2_MAX_ROWS) return false;   // distinct rows of the grid
+    if (sor_tb2_lds_bytes(g, K, R) > TB2_LDS_MAX) return false;
+    // the row split x / H by multiplier is exact for every x a block uses (x * H < 2^32)
+    const uint64_t nh = (uint64_t)(R + 4 * K) * (uint64_t)g.H;
+    return nh * (uint64_t)g.H < ((uint64_t)1 << 32);
+}
+
+void launch_sor_tb2(hipStream_t st, const PfGeom &g, const StepArgs &a, int first, int K, int R, const double *Pin,
+                    double *Pout, const double *PHIs, const double *BB, const double *phi, unsigned long long *slot,
+                    bool fast) {
+    const size_t lds = (size_t)sor_tb2_lds_bytes(g, K, R);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)k_sor_tb2<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)TB2_LDS_MAX);
+        (void)hipFuncSetAttribute((const void *)k_sor_tb2<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)TB2_LDS_MAX);
+        attr_set = true;
+    }
+    const unsigned blocks = (unsigned)((g.n + R - 1) / R);
+    const uint32_t hmul = tb2_magic((uint32_t)g.H);
+    if (fast)
+        hipLaunchKernelGGL(k_sor_tb2<true>, dim3(blocks), dim3(TB2_THREADS), lds, st, g, a, first, K, R, hmul, Pin, Pout, PHIs,
+                           BB, phi, slot);
+    else
+        hipLaunchKernelGGL(k_sor_tb2<false>, dim3(blocks), dim3(TB2_THREADS), lds, st, g, a, first, K, R, hmul, Pin, Pout,
+                           PHIs, BB, phi, slot);
+}
+
 // Porosity (natural layout, ghosts included) -> colour-split copy.
 __global__ void k_split(PfGeom g, const double *__restrict__ src, double *__restrict__ dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -717,6 +871,8 @@ int sor_blocks(const PfGeom &g) {
             const int v = sweep_shape(g, kb, xw).blocks();
             n = v > n ? v : n;
         }
+    // the temporally blocked 2D sweep: one row per block, TB2_THREADS / 64 waves per block
+    if (g.dim == 2) n = g.n * (TB2_THREADS / 256) > n ? g.n * (TB2_THREADS / 256) : n;
     n *= 4;
     const FusedGeom f = fused_geom(g, 1);
     const int nf = f.gx * f.gy * f.gz;

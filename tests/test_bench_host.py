@@ -24,10 +24,11 @@ def test_bench_refuses_more_than_one_gpu():
                     reason="the reference's OpenMP build is not built here (make -C oracle ref)")
 def test_cpu_baseline_times_reference_build_per_step():
     """The reference build runs steps + 1 steps on a small sphere; the baseline is the time
-    between its first and last per-step log line, so it covers exactly `steps` steps."""
+    between its first and last per-step log line, so it covers exactly `steps` steps.  The
+    steps take tens of milliseconds, so each log line is read on its own."""
     import bench
     from pixelflow_amd import cases
-    p, phi = cases.sphere_case(m=32, n=16, l=16, steps=4, iter_max=20)
+    p, phi = cases.sphere_case(m=64, n=32, l=32, steps=4, iter_max=50)
     res = bench.cpu_baseline_reference(p, phi, 3, 2)
     assert res is not None
     rate, seconds = res
@@ -51,7 +52,7 @@ def test_cpu_baseline_times_reference_2d_build():
     """The 2D benchmark case's CPU leg uses the 2D executable of the reference build."""
     import bench
     from pixelflow_amd import cases
-    p, phi = cases.cylinder_case(m=64, n=32, steps=6, iter_max=20)
+    p, phi = cases.cylinder_case(m=256, n=128, steps=6, iter_max=100)
     res = bench.cpu_baseline_reference(p, phi, 4, 2, main="ref_ibm_2d_uniform")
     assert res is not None and res[1] > 0 and res[0] == pytest.approx(p.cells * 4 / res[1])
 

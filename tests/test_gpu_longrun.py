@@ -25,8 +25,12 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_c1_cylinder_2000_steps_matches_reference():
+@pytest.mark.parametrize("tb2", ["0,0", "2,1", "1,3"])
+def test_c1_cylinder_2000_steps_matches_reference(tb2, monkeypatch):
+    """With the two-launch colour sweeps and with the temporally blocked 2D sweep (2
+    iterations per launch and 1 row per block; 1 iteration and 3 rows)."""
     from pixelflow_amd.solver import Solver
+    monkeypatch.setenv("PF_SOR_TB2", tb2)
     p, g = load_golden("g2d_c1_2000")
     assert (p.m, p.n, p.istep_max, p.iter_max) == (256, 128, 2000, 100)
     with Solver(p, _c1_porosity(p)) as s:

@@ -339,6 +339,25 @@ def test_gpu_sweep_cells_per_thread_matches_reference(name, kb, monkeypatch):
         np.testing.assert_array_equal(st["sor_error"], g["errs"])
 
 
+@pytest.mark.parametrize("shape", ["1,1", "1,5", "2,1", "2,3", "3,2", "4,8"])
+@pytest.mark.parametrize("name", ["g2d_cyl48", "g2d_c1_10", "g2d_drag", "g2d_backstep", "g2d_stage_i1_s3",
+                                  "g2d_weird_odd"])
+def test_gpu_blocked_2d_sweep_matches_reference(name, shape, monkeypatch):
+    """The temporally blocked 2D sweep (K iterations per launch in LDS, R rows per block,
+    rows wrapped periodically in y; launches split so their count is even) reproduces the
+    reference.  Shapes whose rows do not fit LDS (K = 3, 4 on the 256-wide grid) and the odd
+    extent of g2d_weird_odd keep the two-launch sweeps."""
+    monkeypatch.setenv("PF_SOR_TB2", shape)
+    p, g = load_golden(name)
+    print(f"[gpu] blocked 2D sweep {shape} fixture {name}", flush=True)
+    with _solver(p, g["phi_raw"]) as s:
+        st = s.step(p.istep_max)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), g[f], err_msg=f"{name}:{f}:tb2 {shape}")
+        np.testing.assert_array_equal(st["sor_error"], g["errs"])
+        np.testing.assert_array_equal(s.divergence(), g["div"], err_msg=f"{name}:div")
+
+
 @pytest.mark.parametrize("name", ["3d_sphere_64x32x32", "3d_odd_45x27x19", "3d_air_room_40x36x32"])
 def test_gpu_mixed_cells_per_thread_identical(name, monkeypatch):
     """FP32 sweeps give the same bits with 1, 2 or 4 cells per thread."""
