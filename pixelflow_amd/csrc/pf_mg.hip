@@ -170,9 +170,14 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_mg_prolong0(PfGeom g, StepArgs a, const double *phi, MgGeom c1,
                                                      const T *__restrict__ e1, T *__restrict__ e, T alpha, int fuse_y,
                                                      int fuse_z) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    // blockIdx.x: colour (outer) and chunk of 64 slots; lanes on consecutive slots of that
+    // colour (i = 2 h + sh), so the wave's loads and stores of e are contiguous
+    const int chunks = ((g.m + 1) / 2 + 63) / 64;
+    const int cb = (int)blockIdx.x / chunks;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = (DIM == 3) ? (int)blockIdx.z + 1 : 0;
+    const int sh = (cb + j + k) & 1;
+    const int i = 2 * ((1 - sh) + ((int)blockIdx.x - cb * chunks) * 64 + (int)threadIdx.x) + sh;
     if (i > g.m || j > g.n) return;
     if (dec0(g, a, i, j, k, phi)) return;
     const T ec = e1[mg_off(c1, (i - 1) >> 1, (j - 1) >> 1, (DIM == 3) ? (k - 1) >> 1 : 0)];
@@ -199,16 +204,23 @@ __global__ __launch_bounds__(256) void k_mg_apply0(PfGeom g, StepArgs a, MgK<T> 
     for (int row = (int)blockIdx.x * 4 + (int)threadIdx.y; row < rows; row += (int)gridDim.x * 4) {
         const int j = row % g.n + 1;
         const int k = (DIM == 3) ? row / g.n + 1 : 0;
-        for (int i = (int)threadIdx.x + 1; i <= g.m; i += 64) {
-            const T Ay = apply0_at<T, DIM, FAST>(g, a, K, F, phi, y, i, j, k);
-            const int64_t so = pf_soff(g, i, j, k);
-            out[so] = Ay;
-            const double wv = (double)w[so];
-            if (MODE == 0) {
-                d0 += wv * (double)Ay;
-            } else {
-                d0 += (double)Ay * wv;
-                d1 += (double)Ay * (double)Ay;
+        // the row's cells colour by colour, the lanes of a wave on consecutive slots of one
+        // colour array (i = 2 h + sh), so every load of the wave is one contiguous run
+        for (int c = 0; c < 2; ++c) {
+            const int sh = (c + j + k) & 1;
+            for (int h0 = 0; h0 < (g.m + 1) / 2; h0 += 64) {
+                const int i = 2 * ((1 - sh) + h0 + (int)threadIdx.x) + sh;
+                if (i > g.m) continue;
+                const T Ay = apply0_at<T, DIM, FAST>(g, a, K, F, phi, y, i, j, k);
+                const int64_t so = pf_soff(g, i, j, k);
+                out[so] = Ay;
+                const double wv = (double)w[so];
+                if (MODE == 0) {
+                    d0 += wv * (double)Ay;
+                } else {
+                    d0 += (double)Ay * wv;
+                    d1 += (double)Ay * (double)Ay;
+                }
             }
         }
     }
@@ -1359,7 +1371,7 @@ static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
         coarse_cycle(S, st, 1);
         const MgGeom &l0 = S.lev[0].g;
         dim3 b(64, 4, 1);
-        dim3 gr((unsigned)((g.m + 63) / 64), (unsigned)((g.n + 3) / 4), (unsigned)(g.dim == 3 ? g.l : 1));
+        dim3 gr((unsigned)(2 * (((g.m + 1) / 2 + 63) / 64)), (unsigned)((g.n + 3) / 4), (unsigned)(g.dim == 3 ? g.l : 1));
         if (g.dim == 3)
             hipLaunchKernelGGL((k_mg_prolong0<T, 3>), gr, b, 0, st, g, S.c.a, S.c.phi, C.g, (const T *)C.e, e,
                                (T)S.prm.alpha, l0.fuse_y, l0.fuse_z);
