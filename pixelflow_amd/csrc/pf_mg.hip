@@ -111,6 +111,8 @@ __device__ __forceinline__ T apply0_at(const PfGeom &g, const StepArgs &a, const
     return s;
 }
 
+constexpr int PROLONG0_ROWS = 4;   // rows per thread of the level-0 prolongation
+
 // Store v at coarse cell (I,J,K) and at the periodic ghost slots that mirror it.
 template <typename T>
 __device__ __forceinline__ void mg_store(const MgGeom &g, T *__restrict__ e, int I, int J, int K, T v, bool my,
@@ -171,26 +173,44 @@ __global__ __launch_bounds__(256) void k_mg_prolong0(PfGeom g, StepArgs a, const
                                                      const T *__restrict__ e1, T *__restrict__ e, T alpha, int fuse_y,
                                                      int fuse_z) {
     // blockIdx.x: colour (outer) and chunk of 64 slots; lanes on consecutive slots of that
-    // colour (i = 2 h + sh), so the wave's loads and stores of e are contiguous
+    // colour (i = 2 h + sh), so the wave's loads and stores of e are contiguous.  A thread
+    // takes PROLONG0_ROWS rows, 4 apart, and loads all of them before it stores any.
     const int chunks = ((g.m + 1) / 2 + 63) / 64;
     const int cb = (int)blockIdx.x / chunks;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int hx = ((int)blockIdx.x - cb * chunks) * 64 + (int)threadIdx.x;
     const int k = (DIM == 3) ? (int)blockIdx.z + 1 : 0;
-    const int sh = (cb + j + k) & 1;
-    const int i = 2 * ((1 - sh) + ((int)blockIdx.x - cb * chunks) * 64 + (int)threadIdx.x) + sh;
-    if (i > g.m || j > g.n) return;
-    if (dec0(g, a, i, j, k, phi)) return;
-    const T ec = e1[mg_off(c1, (i - 1) >> 1, (j - 1) >> 1, (DIM == 3) ? (k - 1) >> 1 : 0)];
-    const int64_t so = pf_soff(g, i, j, k);
-    const T v = e[so] + alpha * ec;
-    e[so] = v;
-    if (fuse_y) {
-        if (j == g.n) e[pf_soff(g, i, 0, k)] = v;
-        if (j == 1) e[pf_soff(g, i, g.n + 1, k)] = v;
+    T v[PROLONG0_ROWS];
+    int64_t so[PROLONG0_ROWS];
+    int jj[PROLONG0_ROWS], ii[PROLONG0_ROWS];
+    bool act[PROLONG0_ROWS];
+#pragma unroll
+    for (int q = 0; q < PROLONG0_ROWS; ++q) {
+        const int j = (int)blockIdx.y * (4 * PROLONG0_ROWS) + q * 4 + (int)threadIdx.y + 1;
+        const int sh = (cb + j + k) & 1;
+        const int i = 2 * ((1 - sh) + hx) + sh;
+        jj[q] = j;
+        ii[q] = i;
+        act[q] = i <= g.m && j <= g.n && !dec0(g, a, i, j, k, phi);
+        so[q] = 0;
+        v[q] = T(0);
+        if (!act[q]) continue;
+        const T ec = e1[mg_off(c1, (i - 1) >> 1, (j - 1) >> 1, (DIM == 3) ? (k - 1) >> 1 : 0)];
+        so[q] = pf_soff(g, i, j, k);
+        v[q] = e[so[q]] + alpha * ec;
     }
-    if (DIM == 3 && fuse_z) {
-        if (k == g.l) e[pf_soff(g, i, j, 0)] = v;
-        if (k == 1) e[pf_soff(g, i, j, g.l + 1)] = v;
+#pragma unroll
+    for (int q = 0; q < PROLONG0_ROWS; ++q) {
+        if (!act[q]) continue;
+        const int i = ii[q], j = jj[q];
+        e[so[q]] = v[q];
+        if (fuse_y) {
+            if (j == g.n) e[pf_soff(g, i, 0, k)] = v[q];
+            if (j == 1) e[pf_soff(g, i, g.n + 1, k)] = v[q];
+        }
+        if (DIM == 3 && fuse_z) {
+            if (k == g.l) e[pf_soff(g, i, j, 0)] = v[q];
+            if (k == 1) e[pf_soff(g, i, j, g.l + 1)] = v[q];
+        }
     }
 }
 
@@ -1371,7 +1391,8 @@ static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
         coarse_cycle(S, st, 1);
         const MgGeom &l0 = S.lev[0].g;
         dim3 b(64, 4, 1);
-        dim3 gr((unsigned)(2 * (((g.m + 1) / 2 + 63) / 64)), (unsigned)((g.n + 3) / 4), (unsigned)(g.dim == 3 ? g.l : 1));
+        dim3 gr((unsigned)(2 * (((g.m + 1) / 2 + 63) / 64)), (unsigned)((g.n + 4 * PROLONG0_ROWS - 1) / (4 * PROLONG0_ROWS)),
+                (unsigned)(g.dim == 3 ? g.l : 1));
         if (g.dim == 3)
             hipLaunchKernelGGL((k_mg_prolong0<T, 3>), gr, b, 0, st, g, S.c.a, S.c.phi, C.g, (const T *)C.e, e,
                                (T)S.prm.alpha, l0.fuse_y, l0.fuse_z);
