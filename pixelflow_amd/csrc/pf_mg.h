@@ -19,10 +19,12 @@
 // * coarse levels are dense arrays with one ghost layer; periodic ghosts are copies (written
 //   with the update for even extents, by a refresh pass for odd ones, so every update of a
 //   pass reads values from before the pass across a wrap, as the reference's p_old does);
-// * the levels of at most 8192 cells run in one block (the tail: one phase per launch of
-//   the per-level path), and the coarsest level (at most MG_COARSEST_CELLS cells where the
-//   extents allow) is solved exactly by its inverse, formed on the host when the operators
-//   are built; red/black sweeps in LDS take its place when that inverse is not available.
+// * the coarsest level (at most MG_COARSEST_CELLS cells where the extents allow) is solved
+//   exactly by its inverse, formed on the host when the operators are built, in a one-block
+//   kernel (the tail, which can also take the levels above it as phases of that block);
+//   red/black sweeps in LDS take its place when that inverse is not available;
+// * levels of at most 1024 cells but the coarsest are cycled twice per visit of the level
+//   above (a W-cycle over the coarse end).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -455,7 +455,18 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
 // ------------------------------------------------------------------------------------
 constexpr int MG_TAIL_THREADS = 1024;
 constexpr int MG_TAIL_U = 8;
-constexpr int64_t MG_TAIL_CELLS = (int64_t)MG_TAIL_THREADS * MG_TAIL_U;
+constexpr int64_t MG_TAIL_CELLS = (int64_t)MG_TAIL_THREADS * MG_TAIL_U;   // the most a tail level may hold
+// The tail starts at the first level of at most this many cells (PF_MG_TAIL=n: at most n,
+// up to MG_TAIL_CELLS), by default the coarsest level alone: every level above it costs
+// less as separate launches over many CUs than as phases of the one block.  With the same
+// cycle (and the same bits), mixed / FP64 ms per step with the tail from 8192 / 1024 / 128
+// cells: sphere 256x128x128 9.03 / 12.23, 8.24 / 11.21, 6.97 / 9.73; boxes 512x256x256
+// 50.3 / 87.6, 49.2 / 86.0, 47.1 / 84.2.
+constexpr int64_t MG_TAIL_START_CELLS = MG_COARSEST_CELLS;
+// Levels of at most this many cells (but the coarsest) are cycled gamma times per visit of
+// the level above (a W-cycle over the coarse end), whether they run as launches or in the
+// tail: on the boxes 512x256x256 that takes the V-cycles per step from 25 to 21.5.
+constexpr int64_t MG_W_CELLS = 1024;
 
 // Division of a cell index by a level's extent inside the tail: x / d = umulhi(x, M) with
 // M = floor(2^32 / d) + 1, exact whenever x * d < 2^32 (tail levels hold at most
@@ -703,7 +714,9 @@ __device__ __forceinline__ void tail_cycle(const MgTailArgs<T> &t) {
                 tail_smooth(t.L[Q], 1);
             }
             tail_restrict(t.L[Q], t.L[Q + 1]);
-            for (int c = 0; c < (Q + 2 == t.nl ? 1 : t.gamma); ++c) tail_cycle<T, Q + 1>(t);
+            const MgGeom &cn = t.L[Q + 1].g;
+            const bool w = Q + 2 < t.nl && (int64_t)cn.m * cn.n * cn.l <= MG_W_CELLS;
+            for (int c = 0; c < (w ? t.gamma : 1); ++c) tail_cycle<T, Q + 1>(t);
             tail_prolong(t.L[Q], t.L[Q + 1], t.alpha);
             for (int s = 0; s < t.nuc; ++s) {
                 tail_smooth(t.L[Q], 1);
@@ -1121,12 +1134,14 @@ bool mg_create(MgSolver<T> &S, const MgCase &c, std::string *why, void *mem) {
     if (L.size() > 1) {
         const int64_t n = (int64_t)L.back().m * L.back().n * L.back().l;
         S.cinv = (T *)take(n * n * (int64_t)sizeof(T));
-        // the tail starts at the first level of at most MG_TAIL_CELLS cells (PF_MG_TAIL=0
+        // the tail starts at the first level of at most MG_TAIL_START_CELLS cells (PF_MG_TAIL=0
         // in the environment keeps one launch per level and step)
         const char *tv = std::getenv("PF_MG_TAIL");
         const bool tail_on = !(tv && *tv && std::atoi(tv) == 0);
+        int64_t tail_cells = MG_TAIL_START_CELLS;
+        if (tv && *tv && std::atoi(tv) > 1) tail_cells = std::min<int64_t>(MG_TAIL_CELLS, std::atoi(tv));
         for (size_t q = 1; tail_on && q < L.size(); ++q)
-            if ((int64_t)L[q].m * L[q].n * L[q].l <= MG_TAIL_CELLS) {
+            if ((int64_t)L[q].m * L[q].n * L[q].l <= tail_cells) {
                 bool ok = L.size() - q <= (size_t)MG_TAIL_LEVELS && n <= MG_COARSEST_CELLS;
                 for (size_t k = q; ok && k < L.size(); ++k) ok = tail_magic_ok(L[k]);
                 if (ok) S.tail_q = (int)q;
@@ -1353,7 +1368,10 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
     MgLevel<T> &C = S.lev[q + 1];
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks_for((int64_t)C.g.m * C.g.n * C.g.l)), dim3(256), 0, st, g,
                        op_of(L), (const uint8_t *)L.keep, (const T *)L.r, (const T *)L.e, C.g, C.r, C.e);
-    coarse_cycle(S, st, q + 1);
+    // a later visit continues from the correction the earlier one left
+    const int64_t ccells = (int64_t)C.g.m * C.g.n * C.g.l;
+    const int visits = (q + 2 < S.lev.size() && ccells <= MG_W_CELLS) ? S.prm.tail_gamma : 1;
+    for (int v = 0; v < visits; ++v) coarse_cycle(S, st, q + 1);
     hipLaunchKernelGGL((k_mg_prolong<T>), dim3(nb), dim3(256), 0, st, g, (const uint8_t *)L.keep, C.g,
                        (const T *)C.e, L.e, (T)S.prm.alpha);
     if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
