@@ -43,19 +43,24 @@ constexpr int MG_NPART = 4096;           // blocks of the reduction kernels (par
 constexpr int MG_TAIL_LEVELS = 8;        // levels the one-block tail of the V-cycle holds at most
 
 // A coarse level: interior extents m x n x l (l = 1 in 2D) with one ghost layer around
-// (in z only in 3D); cell (I,J,K), 0-based, at (I+1) + PX*((J+1) + Y*(K+KG)).
+// (in z only in 3D), stored colour-split like level 0: ghosted cell (x, y, z) = (I+1, J+1,
+// K+KG) lives in colour array (x + y + z) & 1 at slot x / 2 of row (y, z), HX slots per row,
+// so a red/black sweep reads the couplings of its own colour only.
 struct MgGeom {
     int dim;
     int m, n, l;
-    int PX, Y, Z, KG;
-    int64_t N;           // PX * Y * Z
+    int HX, Y, Z, KG;
+    int64_t CS;          // one colour's array: HX * Y * Z
+    int64_t N;           // both colours
     int per;             // y (and z in 3D) periodic
     int fuse_y, fuse_z;  // periodic with an even extent: mirrors written by the updates
     int ref_y, ref_z;    // periodic with an odd extent: a refresh pass after each update
 };
 
 PF_HD int64_t mg_off(const MgGeom &g, int I, int J, int K) {
-    return (int64_t)(I + 1) + (int64_t)g.PX * ((int64_t)(J + 1) + (int64_t)g.Y * (int64_t)(K + g.KG));
+    const int x = I + 1, y = J + 1, z = K + g.KG;
+    return (int64_t)((x + y + z) & 1) * g.CS + (int64_t)(x >> 1) +
+           (int64_t)g.HX * ((int64_t)y + (int64_t)g.Y * (int64_t)z);
 }
 
 inline MgGeom mg_make_geom(int dim, int m, int n, int l, int per) {
@@ -64,11 +69,12 @@ inline MgGeom mg_make_geom(int dim, int m, int n, int l, int per) {
     g.m = m;
     g.n = n;
     g.l = (dim == 3) ? l : 1;
-    g.PX = m + 2;
+    g.HX = (m + 3) / 2;
     g.Y = n + 2;
     g.KG = (dim == 3) ? 1 : 0;
     g.Z = (dim == 3) ? g.l + 2 : 1;
-    g.N = (int64_t)g.PX * g.Y * g.Z;
+    g.CS = (int64_t)g.HX * g.Y * g.Z;
+    g.N = 2 * g.CS;
     g.per = per;
     g.fuse_y = per && (n % 2 == 0);
     g.ref_y = per && (n % 2 == 1);

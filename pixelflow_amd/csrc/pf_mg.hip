@@ -271,39 +271,42 @@ struct MgOp {
 };
 
 template <typename T>
-__device__ __forceinline__ T mg_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, int64_t o) {
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
+__device__ __forceinline__ T mg_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, int I, int J, int K,
+                                   int64_t o) {
     T s = A.ap[o] * e[o];
-    s = s + A.cE[o] * e[o + 1];
-    s = s + A.cW[o] * e[o - 1];
-    s = s + A.cN[o] * e[o + sy];
-    s = s + A.cS[o] * e[o - sy];
+    s = s + A.cE[o] * e[mg_off(g, I + 1, J, K)];
+    s = s + A.cW[o] * e[mg_off(g, I - 1, J, K)];
+    s = s + A.cN[o] * e[mg_off(g, I, J + 1, K)];
+    s = s + A.cS[o] * e[mg_off(g, I, J - 1, K)];
     if (g.dim == 3) {
-        s = s + A.cT[o] * e[o + sz];
-        s = s + A.cB[o] * e[o - sz];
+        s = s + A.cT[o] * e[mg_off(g, I, J, K + 1)];
+        s = s + A.cB[o] * e[mg_off(g, I, J, K - 1)];
     }
     return s;
 }
 
-// One colour of a red/black Gauss-Seidel sweep (omega = 1).
+// One colour of a red/black Gauss-Seidel sweep (omega = 1).  Threads walk the colour's
+// cells only, (m + 1) / 2 per row: I = 2 h + ((colour + J + K) & 1).
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_smooth(MgGeom g, MgOp<T> A, const T *__restrict__ r, T *__restrict__ e,
                                                    int colour) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)g.m * g.n * g.l) return;
-    int I, J, K;
-    mg_cell(g, q, I, J, K);
-    if (((I + J + K) & 1) != colour) return;
+    const uint32_t hm = (uint32_t)(g.m + 1) >> 1;
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (uint64_t)hm * (uint64_t)g.n * (uint64_t)g.l) return;
+    const uint64_t row = q / hm;
+    const int h = (int)(q - row * hm);
+    const int J = (int)(row % (uint64_t)g.n), K = (int)(row / (uint64_t)g.n);
+    const int I = 2 * h + ((colour + J + K) & 1);
+    if (I >= g.m) return;
     const int64_t o = mg_off(g, I, J, K);
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
     T s = r[o];
-    s = s - A.cE[o] * e[o + 1];
-    s = s - A.cW[o] * e[o - 1];
-    s = s - A.cN[o] * e[o + sy];
-    s = s - A.cS[o] * e[o - sy];
+    s = s - A.cE[o] * e[mg_off(g, I + 1, J, K)];
+    s = s - A.cW[o] * e[mg_off(g, I - 1, J, K)];
+    s = s - A.cN[o] * e[mg_off(g, I, J + 1, K)];
+    s = s - A.cS[o] * e[mg_off(g, I, J - 1, K)];
     if (g.dim == 3) {
-        s = s - A.cT[o] * e[o + sz];
-        s = s - A.cB[o] * e[o - sz];
+        s = s - A.cT[o] * e[mg_off(g, I, J, K + 1)];
+        s = s - A.cB[o] * e[mg_off(g, I, J, K - 1)];
     }
     mg_store(g, e, I, J, K, s / A.ap[o], g.fuse_y, g.fuse_z);
 }
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGeom f, MgOp<T> A, const 
                 if (i >= f.m || j >= f.n || k >= f.l) continue;
                 const int64_t o = mg_off(f, i, j, k);
                 if (!keep[o]) continue;
-                sum = sum + (r[o] - mg_ax(f, A, e, o));
+                sum = sum + (r[o] - mg_ax(f, A, e, i, j, k, o));
             }
     rc[mg_off(c, I, J, K)] = sum;
     mg_store(c, ec, I, J, K, T(0), c.per, c.per);
@@ -375,6 +378,7 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
     const int cells = g.m * g.n * g.l;
     for (int q = tid; q < (int)g.N; q += 256) se[q] = T(0);
     int64_t o[MG_CPT];
+    int nb[MG_CPT][6];   // the six neighbours' offsets (E, W, N, S, T, B)
     int col[MG_CPT], I[MG_CPT], J[MG_CPT], K[MG_CPT];
     T rr[MG_CPT], ap[MG_CPT], cE[MG_CPT], cW[MG_CPT], cN[MG_CPT], cS[MG_CPT], cT[MG_CPT], cB[MG_CPT];
 #pragma unroll
@@ -384,6 +388,12 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
         if (q < cells) {
             mg_cell(g, q, I[u], J[u], K[u]);
             o[u] = mg_off(g, I[u], J[u], K[u]);
+            nb[u][0] = (int)mg_off(g, I[u] + 1, J[u], K[u]);
+            nb[u][1] = (int)mg_off(g, I[u] - 1, J[u], K[u]);
+            nb[u][2] = (int)mg_off(g, I[u], J[u] + 1, K[u]);
+            nb[u][3] = (int)mg_off(g, I[u], J[u] - 1, K[u]);
+            nb[u][4] = (g.dim == 3) ? (int)mg_off(g, I[u], J[u], K[u] + 1) : 0;
+            nb[u][5] = (g.dim == 3) ? (int)mg_off(g, I[u], J[u], K[u] - 1) : 0;
             col[u] = (I[u] + J[u] + K[u]) & 1;
             rr[u] = r[o[u]];
             ap[u] = A.ap[o[u]];
@@ -396,7 +406,6 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
         }
     }
     __syncthreads();
-    const int64_t sy = g.PX, sz = (int64_t)g.PX * g.Y;
     for (int it = 0; it < sweeps; ++it) {
         for (int c = 0; c < 2; ++c) {
 #pragma unroll
@@ -404,13 +413,13 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(MgGeom g, MgOp<T> A, const 
                 if (col[u] != c) continue;
                 const int64_t x = o[u];
                 T s = rr[u];
-                s = s - cE[u] * se[x + 1];
-                s = s - cW[u] * se[x - 1];
-                s = s - cN[u] * se[x + sy];
-                s = s - cS[u] * se[x - sy];
+                s = s - cE[u] * se[nb[u][0]];
+                s = s - cW[u] * se[nb[u][1]];
+                s = s - cN[u] * se[nb[u][2]];
+                s = s - cS[u] * se[nb[u][3]];
                 if (g.dim == 3) {
-                    s = s - cT[u] * se[x + sz];
-                    s = s - cB[u] * se[x - sz];
+                    s = s - cT[u] * se[nb[u][4]];
+                    s = s - cB[u] * se[nb[u][5]];
                 }
                 const T v = s / ap[u];
                 se[x] = v;
@@ -502,7 +511,8 @@ struct MgTailArgs {
 
 // mg_off in 32 bits (a tail level's arrays hold far fewer than 2^32 elements).
 __device__ __forceinline__ uint32_t tail_off(const MgGeom &g, int I, int J, int K) {
-    return (uint32_t)(I + 1) + (uint32_t)g.PX * ((uint32_t)(J + 1) + (uint32_t)g.Y * (uint32_t)(K + g.KG));
+    const uint32_t x = (uint32_t)(I + 1), y = (uint32_t)(J + 1), z = (uint32_t)(K + g.KG);
+    return ((x + y + z) & 1u) * (uint32_t)g.CS + (x >> 1) + (uint32_t)g.HX * (y + (uint32_t)g.Y * z);
 }
 
 // Cell q of a level in natural order (I fastest).
@@ -558,7 +568,6 @@ __device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour)
     const MgGeom &g = L.g;
     const uint32_t hm = (uint32_t)(g.m + 1) >> 1;
     const uint32_t count = hm * (uint32_t)g.n * (uint32_t)g.l;
-    const uint32_t sy = (uint32_t)g.PX, sz = (uint32_t)g.PX * (uint32_t)g.Y;
     for (uint32_t base = threadIdx.x; base < count; base += (uint32_t)MG_TAIL_THREADS * MG_TAIL_U) {
         T v[MG_TAIL_U];
         int I[MG_TAIL_U], J[MG_TAIL_U], K[MG_TAIL_U];
@@ -578,13 +587,13 @@ __device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour)
             act[u] = true;
             const uint32_t o = tail_off(g, I[u], J[u], K[u]);
             T s = L.r[o];
-            s = s - L.A.cE[o] * L.e[o + 1];
-            s = s - L.A.cW[o] * L.e[o - 1];
-            s = s - L.A.cN[o] * L.e[o + sy];
-            s = s - L.A.cS[o] * L.e[o - sy];
+            s = s - L.A.cE[o] * L.e[tail_off(g, I[u] + 1, J[u], K[u])];
+            s = s - L.A.cW[o] * L.e[tail_off(g, I[u] - 1, J[u], K[u])];
+            s = s - L.A.cN[o] * L.e[tail_off(g, I[u], J[u] + 1, K[u])];
+            s = s - L.A.cS[o] * L.e[tail_off(g, I[u], J[u] - 1, K[u])];
             if (g.dim == 3) {
-                s = s - L.A.cT[o] * L.e[o + sz];
-                s = s - L.A.cB[o] * L.e[o - sz];
+                s = s - L.A.cT[o] * L.e[tail_off(g, I[u], J[u], K[u] + 1)];
+                s = s - L.A.cB[o] * L.e[tail_off(g, I[u], J[u], K[u] - 1)];
             }
             v[u] = s / L.A.ap[o];
         }
@@ -597,16 +606,16 @@ __device__ __forceinline__ void tail_smooth(const MgTailLevel<T> &L, int colour)
 }
 
 template <typename T>
-__device__ __forceinline__ T tail_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, uint32_t o) {
-    const uint32_t sy = (uint32_t)g.PX, sz = (uint32_t)g.PX * (uint32_t)g.Y;
+__device__ __forceinline__ T tail_ax(const MgGeom &g, const MgOp<T> &A, const T *__restrict__ e, int I, int J, int K,
+                                     uint32_t o) {
     T s = A.ap[o] * e[o];
-    s = s + A.cE[o] * e[o + 1];
-    s = s + A.cW[o] * e[o - 1];
-    s = s + A.cN[o] * e[o + sy];
-    s = s + A.cS[o] * e[o - sy];
+    s = s + A.cE[o] * e[tail_off(g, I + 1, J, K)];
+    s = s + A.cW[o] * e[tail_off(g, I - 1, J, K)];
+    s = s + A.cN[o] * e[tail_off(g, I, J + 1, K)];
+    s = s + A.cS[o] * e[tail_off(g, I, J - 1, K)];
     if (g.dim == 3) {
-        s = s + A.cT[o] * e[o + sz];
-        s = s + A.cB[o] * e[o - sz];
+        s = s + A.cT[o] * e[tail_off(g, I, J, K + 1)];
+        s = s + A.cB[o] * e[tail_off(g, I, J, K - 1)];
     }
     return s;
 }
@@ -632,7 +641,7 @@ __device__ __forceinline__ void tail_restrict(const MgTailLevel<T> &F, const MgT
                         if (i >= f.m || j >= f.n || k >= f.l) continue;
                         const uint32_t o = tail_off(f, i, j, k);
                         if (!F.keep[o]) continue;
-                        sum = sum + (F.r[o] - tail_ax(f, F.A, F.e, o));
+                        sum = sum + (F.r[o] - tail_ax(f, F.A, F.e, i, j, k, o));
                     }
             v[u] = sum;
         }
@@ -1357,8 +1366,9 @@ static void coarse_cycle(MgSolver<T> &S, hipStream_t st, size_t q) {
     }
     const unsigned nb = blocks_for(cells);
     const unsigned nbr = blocks_for((int64_t)g.m * (g.l > g.n ? g.l : g.n));
+    const unsigned nbs = blocks_for((int64_t)((g.m + 1) / 2) * g.n * g.l);
     auto smooth = [&](int colour) {
-        hipLaunchKernelGGL((k_mg_smooth<T>), dim3(nb), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e, colour);
+        hipLaunchKernelGGL((k_mg_smooth<T>), dim3(nbs), dim3(256), 0, st, g, op_of(L), (const T *)L.r, L.e, colour);
         if (g.ref_y || g.ref_z) hipLaunchKernelGGL((k_mg_refresh<T>), dim3(nbr), dim3(256), 0, st, g, L.e);
     };
     for (int s = 0; s < S.prm.nuc; ++s) {
