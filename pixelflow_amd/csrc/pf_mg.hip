@@ -920,20 +920,23 @@ __global__ __launch_bounds__(256) void k_bicg_init(int64_t n, const T *__restric
     block_sum2(d, 0., part);
 }
 
+// The two updates without a reduction take one element per thread (the launch covers n):
+// a grid-stride loop over MG_NPART blocks streams at 4.5-5 TB/s where this form reaches a
+// plain copy's 6.1-6.5 TB/s.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bicg_p(int64_t n, const T *__restrict__ r, T *__restrict__ p,
                                                 const T *__restrict__ v, const double *S) {
     const T beta = (T)S[3], omega = (T)S[2];
-    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256)
-        p[q] = r[q] + beta * (p[q] - omega * v[q]);
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) p[q] = r[q] + beta * (p[q] - omega * v[q]);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bicg_s(int64_t n, const T *__restrict__ r, const T *__restrict__ v,
                                                 T *__restrict__ s, const double *S) {
     const T alpha = (T)S[1];
-    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256)
-        s[q] = r[q] - alpha * v[q];
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) s[q] = r[q] - alpha * v[q];
 }
 
 template <typename T>
@@ -1012,11 +1015,14 @@ __global__ __launch_bounds__(256) void k_bicg_scalar(const double *part, int nb,
     }
 }
 
+// One element per thread (the launch covers n).
 __global__ void k_mg_zero_t(float *p, int64_t n) {
-    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) p[q] = 0.f;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) p[q] = 0.f;
 }
 __global__ void k_mg_zero_t(double *p, int64_t n) {
-    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) p[q] = 0.;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) p[q] = 0.;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1398,7 +1404,7 @@ static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
     StepArgs a1 = S.c.a;
     a1.omega = 1.0;
     a1.omega_c = 0.0;
-    hipLaunchKernelGGL(k_mg_zero_t, dim3(flat_blocks(2 * g.HS)), dim3(256), 0, st, e, 2 * g.HS);
+    hipLaunchKernelGGL(k_mg_zero_t, dim3(blocks_for(2 * g.HS)), dim3(256), 0, st, e, 2 * g.HS);
     for (int s = 0; s < S.prm.nu1; ++s) {
         sweep0(S, st, a1, 0, e, rhs);
         sweep0(S, st, a1, 1, e, rhs);
@@ -1493,12 +1499,12 @@ static void enqueue_iteration(MgSolver<T> &S, hipStream_t st) {
     const int64_t n = 2 * S.c.g.HS;
     const unsigned fb = flat_blocks(n);
     const int nap = (int)apply_blocks<T>(S.c.g);
-    hipLaunchKernelGGL((k_bicg_p<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, S.p, (const T *)S.v,
+    hipLaunchKernelGGL((k_bicg_p<T>), dim3(blocks_for(n)), dim3(256), 0, st, n, (const T *)S.r, S.p, (const T *)S.v,
                        (const double *)S.scal);
     vcycle(S, st, (const T *)S.p, S.y);
     apply0<T, 0>(S, st, S.y, S.v, S.rh);
     hipLaunchKernelGGL(k_bicg_scalar, dim3(1), dim3(256), 0, st, (const double *)S.part, nap, S.scal, 1);
-    hipLaunchKernelGGL((k_bicg_s<T>), dim3(fb), dim3(256), 0, st, n, (const T *)S.r, (const T *)S.v, S.s,
+    hipLaunchKernelGGL((k_bicg_s<T>), dim3(blocks_for(n)), dim3(256), 0, st, n, (const T *)S.r, (const T *)S.v, S.s,
                        (const double *)S.scal);
     vcycle(S, st, (const T *)S.s, S.z);
     apply0<T, 1>(S, st, S.z, S.t, S.s);

@@ -75,10 +75,11 @@ __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s
 
 // p += e over every slot of both colours: interior cells and the periodic ghost slots
 // (which hold e of their source cell, so the ghost stays equal to its source).
+// One element per thread (the launch covers n).
 template <typename TE>
 __global__ void k_update(const TE *__restrict__ E, double *__restrict__ P, int64_t n) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        P[t] = P[t] + (double)E[t];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) P[t] = P[t] + (double)E[t];
 }
 
 // p += e as k_update, with the max |e| of the update (NaN kept) folded into the slots of
@@ -172,10 +173,10 @@ void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs 
 }
 
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P) {
-    hipLaunchKernelGGL(k_update<float>, dim3(256 * 8), dim3(256), 0, st, E, P, 2 * g.HS);
+    hipLaunchKernelGGL(k_update<float>, dim3((unsigned)((2 * g.HS + 255) / 256)), dim3(256), 0, st, E, P, 2 * g.HS);
 }
 void launch_update(hipStream_t st, const PfGeom &g, const double *E, double *P) {
-    hipLaunchKernelGGL(k_update<double>, dim3(256 * 8), dim3(256), 0, st, E, P, 2 * g.HS);
+    hipLaunchKernelGGL(k_update<double>, dim3((unsigned)((2 * g.HS + 255) / 256)), dim3(256), 0, st, E, P, 2 * g.HS);
 }
 // nslots: the slots available (four per block are used)
 template <typename TE>
