@@ -322,7 +322,7 @@ def tolerance_large(steps: int, warmup: int, tol: float):
             os.environ["PF_PHI_CODE"] = code
             with Solver(p, phi) as s:
                 wall, sweep_us, _ = _timed_run(s, steps, warmup)
-                runs[code] = (wall, sweep_us, s.sor_coded())
+                runs[code] = (wall, sweep_us, s.sor_coded(), s.sor_unit_cells())
     finally:
         if saved is None:
             os.environ.pop("PF_PHI_CODE", None)
@@ -335,23 +335,25 @@ def tolerance_large(steps: int, warmup: int, tol: float):
                                "ms_per_step_porosity_fp64": runs["0"][0] * 1e3 / steps,
                                "avg_colour_sweep_us_porosity_fp64": runs["0"][1]}
 
-    def roofline(sweep_us, bytes_per_cell, kernel, traffic):
-        bytes_ = bytes_per_cell * p.cells
+    def roofline(sweep_us, bytes_per_cell, unit_bytes, unit, kernel, traffic):
+        # bytes_per_cell per grid cell and launch, less unit_bytes per cell of either colour
+        # whose waves skip the porosity loads
+        bytes_ = bytes_per_cell * p.cells - unit_bytes * unit
         achieved = bytes_ / (sweep_us * 1e-6) / 1e9
         out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch_algorithmic": bytes_,
-               "avg_launch_us": sweep_us}
+               "avg_launch_us": sweep_us, "unit_porosity_cells": unit}
         if traffic:
             # the bytes the counters saw (L2 line fetches and writes) per launch time: the
             # rate the memory system actually delivers, to set beside a plain copy's
             out["traffic_GBps"] = traffic / (sweep_us * 1e-6) / 1e9
         return out
 
-    res["sweep_roofline"] = roofline(runs["0"][1], 24, "k_sor_phi (red/black colour sweep, FP64)",
+    res["sweep_roofline"] = roofline(runs["0"][1], 24, 8, runs["0"][3], "k_sor_phi (red/black colour sweep, FP64)",
                                      measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}", "uncoded_kernel"))
     if runs["1"][2]:
         res["sweep_roofline"]["coded"] = roofline(
-            runs["1"][1], 20, "k_sor_phi (red/black colour sweep, FP64, porosity coded in 4 bytes)",
+            runs["1"][1], 20, 4, runs["1"][3], "k_sor_phi (red/black colour sweep, FP64, porosity coded in 4 bytes)",
             measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"))
     return res
 
@@ -448,6 +450,7 @@ def main(argv=None):
     wall = time.perf_counter() - t0
     sweep_us, launches = s.last_sweep_time_us()
     fused = s.sor_fused()
+    unit = s.sor_unit_cells()
     s.close()
     log(f"[bench] {args.steps} steps in {wall:.3f} s; colour sweep avg {sweep_us:.2f} us over {launches} sweeps"
         f" ({'fused: one launch per iteration' if fused else 'one launch per colour sweep'})")
@@ -461,11 +464,13 @@ def main(argv=None):
         # read once (4 FP64 words per cell)
         kernel, launch_us, launch_bytes = "k_sor_fused (one red/black SOR iteration)", 2 * sweep_us, 4 * 8 * cells
     else:
-        # one launch = one colour: 3 FP64 words per cell
-        kernel, launch_us, launch_bytes = "k_sor_phi (red/black colour sweep)", sweep_us, 3 * 8 * cells
+        # one launch = one colour: 3 FP64 words per cell, less the porosity word of the cells
+        # whose waves skip it (unit porosity: 2 of the 3 words per updated cell, so 1 word
+        # per cell of either colour over a launch pair)
+        kernel, launch_us, launch_bytes = "k_sor_phi (red/black colour sweep)", sweep_us, 3 * 8 * cells - 8 * unit
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9 if launch_us > 0 else 0.0
-    sor_words = 4 if fused else 6   # per SOR iteration and cell: fused launch / two colour sweeps
-    step_bytes = 8 * ((2 * 3 + 2) + sor_words * p.iter_max + (2 * 3 + 1)) * cells
+    sor_bytes = 4 * 8 * cells if fused else 2 * launch_bytes   # per SOR iteration
+    step_bytes = 8 * ((2 * 3 + 2) + (2 * 3 + 1)) * cells + sor_bytes * p.iter_max
     res = {
         "metric": METRIC,
         "value": value,
@@ -495,6 +500,7 @@ def main(argv=None):
             "traffic": measured_traffic(kernel.split()[0], workload),
             "bytes_per_launch_algorithmic": launch_bytes,
             "avg_launch_us": launch_us,
+            "unit_porosity_cells": unit,
         },
     }
     if args.mixed_steps > 0:

@@ -55,6 +55,14 @@ struct pf_ctx {
     float *PHC = nullptr;
     double *PHTAB = nullptr;
     bool phi_code_on = false, phi_coded = false;
+    bool phi_code_forced = false;   // PF_PHI_CODE=1: code even where unit waves dominate
+    // FP64 sweeps: one byte per 32 colour slots of a row, 1 where every porosity operand of
+    // those cells is 1 (unit_index in pf_layout.h); waves of flagged chunks skip their
+    // porosity loads.  unit_cells: cells of both colours in such waves (PF_UNIT_PHI=0 in
+    // the environment turns the flags off)
+    uint8_t *UNIT = nullptr;
+    int64_t unit_cells = 0;
+    uint8_t *UNIT32 = nullptr;      // mixed precision: the same flags of the FP32 porosity
     int err_cap = 0, nparts = 0;
     bool have_phi = false, have_init = false;
     bool timing = false;
@@ -116,6 +124,13 @@ static void tb2_params(const PfGeom &g, const pf_desc *d, int &K, int &R) {
     }
 }
 
+// The colour sweeps' unit-porosity flags, of the FP64 porosity for FP64 and of the FP32
+// porosity for mixed precision (off with PF_UNIT_PHI=0 in the environment).
+static bool unit_wanted(const pf_desc *d) {
+    const char *u = std::getenv("PF_UNIT_PHI");
+    return !(u && *u && std::atoi(u) == 0);
+}
+
 extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     std::string why;
     if (!desc_ok(d, why)) return -1;
@@ -134,7 +149,8 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     int tk = 0, tr = 0;
     tb2_params(g, d, tk, tr);
     const int64_t p2 = (tk > 0) ? (int64_t)8 * 2 * g.HS : 0;
-    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + p2 + 4096;
+    const int64_t unit = unit_wanted(d) ? unit_flags_size(g) : 0;
+    return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + p2 + unit + 4096;
 }
 
 static int alloc(pf_ctx *c, double **p, int64_t n) {
@@ -161,6 +177,8 @@ static void release(pf_ctx *c) {
     for (float **p : f32)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (c->PHTAB) { (void)hipFree(c->PHTAB); c->PHTAB = nullptr; }
+    if (c->UNIT) { (void)hipFree(c->UNIT); c->UNIT = nullptr; }
+    if (c->UNIT32) { (void)hipFree(c->UNIT32); c->UNIT32 = nullptr; }
     if (c->mg32) { pf::mg_destroy(*c->mg32); delete c->mg32; c->mg32 = nullptr; }
     if (c->mg64) { pf::mg_destroy(*c->mg64); delete c->mg64; c->mg64 = nullptr; }
     for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
@@ -241,11 +259,15 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
     {
         // coded porosity where the FP64 sweep's arrays (24 B per cell) are well past the
         // last-level cache: 512x256x256 179 against 184 us per sweep; at 256x128x128, inside
-        // the cache, it was slower (18.5 against 17.9 us)
+        // the cache, it was slower (18.5 against 17.9 us).  pf_set_porosity leaves it unused
+        // where unit-porosity waves hold half the cells or more (see derive_from_porosity)
         const int64_t cells = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
         c->phi_code_on = cells * 24 > ((int64_t)512 << 20);
         const char *pc = std::getenv("PF_PHI_CODE");
-        if (pc && *pc) c->phi_code_on = std::atoi(pc) != 0;
+        if (pc && *pc) {
+            c->phi_code_on = std::atoi(pc) != 0;
+            c->phi_code_forced = c->phi_code_on;
+        }
     }
     if (desc->precision == PF_PREC_FP64 && c->phi_code_on) {
         if (hipMalloc((void **)&c->PHC, sizeof(float) * (size_t)(2 * g.HS)) != hipSuccess ||
@@ -253,6 +275,12 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
             rc = fail(c, PF_ENOMEM, "hipMalloc of the coded porosity failed");
             goto bad;
         }
+    }
+    if (unit_wanted(desc) &&
+        hipMalloc((void **)(desc->precision == PF_PREC_MIXED ? &c->UNIT32 : &c->UNIT), (size_t)unit_flags_size(g)) !=
+            hipSuccess) {
+        rc = fail(c, PF_ENOMEM, "hipMalloc of the unit-porosity flags failed");
+        goto bad;
     }
     if (hipStreamSynchronize(c->st) != hipSuccess) {
         rc = fail(c, PF_EHIP, "stream synchronize failed after allocation");
@@ -293,6 +321,8 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
         mc.PHI32 = c->PHI32;
         mc.PHC = c->PHC;
         mc.PHTAB = c->PHTAB;
+        mc.UNIT = c->UNIT;
+        mc.UNIT32 = c->UNIT32;
         mc.kb32 = c->sweep_kb32;
         mc.kb64 = c->sweep_kb;
         mc.xw = c->sweep_xw;
@@ -361,8 +391,24 @@ static int derive_from_porosity(pf_ctx *c) {
     PF_TRY_HIP(c, hipGetLastError());
     c->sor_variant = (hbad == 0) ? 2 : 1;
     if (c->sor_forced >= 0 && c->sor_forced <= 2 && !(c->sor_forced == 2 && hbad != 0)) c->sor_variant = c->sor_forced;
+    c->unit_cells = 0;
+    if (c->UNIT) {
+        pf::launch_unit_flags(c->st, g, c->PHIs, c->UNIT);
+        std::vector<uint8_t> h((size_t)unit_flags_size(g));
+        PF_TRY_HIP(c, hipMemcpyAsync(h.data(), c->UNIT, h.size(), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        // counted where the two-launch colour sweeps run (not the stored-face, fused or 2D
+        // blocked sweeps, which read porosity or coefficients everywhere)
+        if (c->sor_variant != 0 && !c->fused && c->tb2_k == 0)
+            c->unit_cells = pf::unit_wave_cells(g, h, c->sweep_xw, g.dim == 3 ? c->sweep_kb : 1);
+    }
+    if (c->UNIT32 && c->PHI32) pf::launch_unit_flags(c->st, g, c->PHI32, c->UNIT32);
+    // coded porosity only where unit waves hold less than half the cells: a unit wave reads
+    // no porosity, and at 512x256x256 with 65 % of the cells in unit waves the sweep on
+    // stored FP64 porosity took 165 us against 195 us coded
     c->phi_coded = false;
-    if (c->PHC && c->PHTAB) {
+    const int64_t ncell = (int64_t)g.m * g.n * (g.dim == 3 ? g.l : 1);
+    if (c->PHC && c->PHTAB && (c->phi_code_forced || 2 * c->unit_cells < ncell)) {
         std::vector<double> h((size_t)(2 * g.HS));
         PF_TRY_HIP(c, hipMemcpyAsync(h.data(), c->PHIs, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
         PF_TRY_HIP(c, hipStreamSynchronize(c->st));
@@ -544,7 +590,7 @@ static void enqueue_sor(pf_ctx *c, unsigned long long *slot, int iters) {
                 pf::launch_sor(c->st, g, c->a, s, c->P, c->AW, c->AS, c->AB, c->BB, c->phi, slot);
             else
                 pf::launch_sor_phi(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, slot, c->sor_variant == 2,
-                                   c->phi_coded ? c->PHC : nullptr, c->PHTAB);
+                                   c->phi_coded ? c->PHC : nullptr, c->PHTAB, c->UNIT);
             if (ry || rz) pf::launch_ghost_refresh(c->st, g, ry, rz, c->P);
         }
     }
@@ -617,7 +663,7 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
                 s.xcd_remap = c->xcd_remap;
                 s.kb = c->sweep_kb32;
                 s.xw = c->sweep_xw;
-                pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot);
+                pf::launch_sor32(c->st, g, c->a, s, c->E32, c->PHI32, c->R32, c->phi, slot, c->UNIT32);
                 if (ry || rz) pf::launch_ghost_refresh32(c->st, g, ry, rz, c->E32);
             }
         }
@@ -832,6 +878,12 @@ extern "C" int pf_last_sweep_time_us(const pf_ctx *c, double *avg_us, int64_t *l
 extern "C" int pf_sor_kernel(const pf_ctx *c) { return c ? c->sor_variant : PF_EINVAL; }
 
 extern "C" int pf_sor_coded(const pf_ctx *c) { return c ? (c->phi_coded ? 1 : 0) : PF_EINVAL; }
+
+extern "C" int pf_sor_unit_cells(const pf_ctx *c, int64_t *cells) {
+    if (!c || !cells) return PF_EINVAL;
+    *cells = c->unit_cells;
+    return PF_OK;
+}
 
 // Bandwidth calibration: copies `bytes` (rounded down to 16) device to device `iters` times
 // on a private stream and returns the average GB/s (read + write bytes per second).

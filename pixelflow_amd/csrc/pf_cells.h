@@ -372,6 +372,33 @@ __device__ __forceinline__ double sor_value(const PfGeom &g, const StepArgs &a, 
     return sor_apply<DIM>(a, c, ap, bb, pc, pW, pE, pS, pN, pB, pT);
 }
 
+// The SOR update of a cell whose seven porosity operands are 1: the six faces are the
+// per-axis values StepArgs unit_ax/ay/az (face_any of two unit porosities, which the
+// host evaluates with the same IEEE operations; the fast path's faces were checked
+// against IEEE division), ap is unit_ap, the boundary rows' rules apply as in sor_coefs,
+// and num / ap is taken by dv with the wave's divisor kap.  The caller evaluates the cell
+// with sor_value where dv reports a dividend outside its range.
+template <int DIM, class DIV>
+__device__ __forceinline__ double sor_unit_value(const PfGeom &g, const StepArgs &a, int i, int j, int k,
+                                                 double pc, double pW, double pE, double pS, double pN, double pB,
+                                                 double pT, double bb, const double *__restrict__ phi, DIV &dv,
+                                                 const DivK &kap) {
+    Coef c;
+    c.ae = a.unit_ax;
+    c.aw = a.unit_ax;
+    c.an = a.unit_ay;
+    c.as = a.unit_ay;
+    c.at = (DIM == 3) ? a.unit_az : 0.;
+    c.ab = (DIM == 3) ? a.unit_az : 0.;
+    if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    double num;
+    if (DIM == 3)
+        num = (((((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS)) - (c.at * pT)) - (c.ab * pB);
+    else
+        num = (((bb - (c.ae * pE)) - (c.aw * pW)) - (c.an * pN)) - (c.as * pS);
+    return (dv(num, kap) * a.omega) + (pc * a.omega_c);
+}
+
 // The operands of one colour-sweep cell: the unknown at the cell and at its six neighbours
 // (other colour), porosity at the same seven places, and the right-hand side.
 template <typename T>
@@ -444,6 +471,91 @@ __device__ __forceinline__ void sweep_gather_coded(const PfGeom &g, int sh, int6
         ct = C[o + HPY];
     } else {
         x.b = x.t = 0.;
+    }
+    x.fc = phi_decode(cc, tab);
+    x.fw = phi_decode(cw, tab);
+    x.fe = phi_decode(ce, tab);
+    x.fs = phi_decode(cs, tab);
+    x.fn = phi_decode(cn, tab);
+    if (DIM == 3) {
+        x.fb = phi_decode(cb, tab);
+        x.ft = phi_decode(ct, tab);
+    } else {
+        x.fb = x.ft = 0.;
+    }
+}
+
+// sweep_gather for a wave whose every porosity operand is 1: the same pressure and
+// right-hand-side loads, no porosity loads, and 1 in the seven porosity operands.  The
+// update then evaluates the same IEEE operations on the same values as with the loaded
+// ones, so its result is the same double.
+template <int DIM, typename T>
+__device__ __forceinline__ void sweep_gather_unit(const PfGeom &g, int sh, int64_t o, const T *__restrict__ Xoth,
+                                                  const T *__restrict__ Xown, const T *__restrict__ Ro, SweepIn<T> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    const auto X = gmem(Xoth);
+    x.c = gmem(Xown)[o];
+    x.w = X[o - 1 + sh];
+    x.e = X[o + sh];
+    x.s = X[o - HP];
+    x.n = X[o + HP];
+    x.rhs = gmem(Ro)[o];
+    if (DIM == 3) {
+        x.b = X[o - HPY];
+        x.t = X[o + HPY];
+    } else {
+        x.b = x.t = (T)0;
+    }
+    x.fc = x.fw = x.fe = x.fs = x.fn = (T)1;
+    x.fb = x.ft = (DIM == 3) ? (T)1 : (T)0;
+}
+
+// The porosity operands alone (the rest of sweep_gather), as stored FP64 and coded.
+template <int DIM>
+__device__ __forceinline__ void sweep_gather_phi(const PfGeom &g, int sh, int64_t o, const double *__restrict__ Fo,
+                                                 const double *__restrict__ Fx, SweepIn<double> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    const auto F = gmem(Fx);
+    x.fc = gmem(Fo)[o];
+    x.fw = F[o - 1 + sh];
+    x.fe = F[o + sh];
+    x.fs = F[o - HP];
+    x.fn = F[o + HP];
+    if (DIM == 3) {
+        x.fb = F[o - HPY];
+        x.ft = F[o + HPY];
+    } else {
+        x.fb = x.ft = 0.;
+    }
+}
+template <int DIM>
+__device__ __forceinline__ void sweep_gather_phi32(const PfGeom &g, int sh, int64_t o, const float *__restrict__ Fo,
+                                                   const float *__restrict__ Fx, SweepIn<float> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    const auto F = gmem(Fx);
+    x.fc = gmem(Fo)[o];
+    x.fw = F[o - 1 + sh];
+    x.fe = F[o + sh];
+    x.fs = F[o - HP];
+    x.fn = F[o + HP];
+    if (DIM == 3) {
+        x.fb = F[o - HPY];
+        x.ft = F[o + HPY];
+    } else {
+        x.fb = x.ft = 0.f;
+    }
+}
+template <int DIM>
+__device__ __forceinline__ void sweep_gather_phi_coded(const PfGeom &g, int sh, int64_t o, const float *__restrict__ Co,
+                                                       const float *__restrict__ Cx, const double *__restrict__ tab,
+                                                       SweepIn<double> &x) {
+    const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+    const auto C = gmem(Cx);
+    const float cc = gmem(Co)[o], cw = C[o - 1 + sh], ce = C[o + sh], cs = C[o - HP], cn = C[o + HP];
+    float cb = 0.f, ct = 0.f;
+    if (DIM == 3) {
+        cb = C[o - HPY];
+        ct = C[o + HPY];
     }
     x.fc = phi_decode(cc, tab);
     x.fw = phi_decode(cw, tab);

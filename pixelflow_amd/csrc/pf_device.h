@@ -230,27 +230,32 @@ struct SweepThread {
 
 // Block (64, 4): threadIdx.x runs over xw slots of rpw = 64 / xw rows; 3D: threadIdx.y =
 // (row pair offset, plane parity), 2D: threadIdx.y = row group.
+// The cell of thread (tx, ty) of logical block L (host and device).
 template <int DIM, int KB>
-__device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const SweepArgs &s) {
-    const int nb = (int)gridDim.x;
-    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
+PF_HD SweepThread sweep_thread_at(const PfGeom &g, const SweepArgs &s, int L, int tx, int ty) {
     const int bx = L % s.gx, byz = L / s.gx;
     const int by = byz % s.gy, bz = byz / s.gy;
     const int rpw = (s.xw == 32) ? 2 : 1;
-    const int sub = (s.xw == 32) ? (int)(threadIdx.x >> 5) : 0;
-    const int xl = (s.xw == 32) ? (int)(threadIdx.x & 31) : (int)threadIdx.x;
+    const int sub = (s.xw == 32) ? (tx >> 5) : 0;
+    const int xl = (s.xw == 32) ? (tx & 31) : tx;
     SweepThread t;
     if (DIM == 3) {
-        t.j = by * 2 * rpw + (int)(threadIdx.y >> 1) * rpw + sub + 1;
-        t.kbase = bz * 2 * KB + 1 + (int)(threadIdx.y & 1);
+        t.j = by * 2 * rpw + (ty >> 1) * rpw + sub + 1;
+        t.kbase = bz * 2 * KB + 1 + (ty & 1);
     } else {
-        t.j = by * 4 * rpw + (int)threadIdx.y * rpw + sub + 1;
+        t.j = by * 4 * rpw + ty * rpw + sub + 1;
         t.kbase = 0;
     }
     t.sh = (s.colour + t.j + t.kbase) & 1;   // the same for kbase + 2q
     t.h = bx * s.xw + xl + 1 - t.sh;
     t.i = 2 * t.h + t.sh;
     return t;
+}
+template <int DIM, int KB>
+__device__ __forceinline__ SweepThread sweep_thread(const PfGeom &g, const SweepArgs &s) {
+    const int nb = (int)gridDim.x;
+    const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
+    return sweep_thread_at<DIM, KB>(g, s, L, (int)threadIdx.x, (int)threadIdx.y);
 }
 
 // Max of 64-bit patterns over the wave, returned in every lane.  Cross-lane moves are DPP

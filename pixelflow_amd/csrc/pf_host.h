@@ -68,6 +68,12 @@ inline StepArgs make_step_args(const pf_desc &d) {
     a.v_in = d.inlet_velocity * std::sin(ang);
     a.inlet_velocity = d.inlet_velocity;
     a.outlet_pressure = d.outlet_pressure;
+    // face_coef(1, 1, dt, d): max(small, (1 + 1) * 0.5) is 1 and dt * 1 is dt, exactly
+    a.unit_ax = (a.dt / a.dx) / a.dx;
+    a.unit_ay = (a.dt / a.dy) / a.dy;
+    a.unit_az = (d.dim == 3) ? (a.dt / a.dz) / a.dz : 0.;
+    a.unit_ap = (d.dim == 3) ? (((((-a.unit_ax) - a.unit_ax) - a.unit_ay) - a.unit_ay) - a.unit_az) - a.unit_az
+                             : (((-a.unit_ax) - a.unit_ax) - a.unit_ay) - a.unit_ay;
     return a;
 }
 

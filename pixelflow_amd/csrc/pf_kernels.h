@@ -26,6 +26,10 @@ struct StepArgs {
     double omega, omega_c;        // relux_factor and (1 - relux_factor)
     double u_in, v_in;            // inlet velocity components (host libm cos/sin)
     double inlet_velocity, outlet_pressure;
+    // the Poisson faces between two cells of porosity 1, ((dt * 1) / d) / d per axis (0 in
+    // z for 2D), and the ap they sum to in solve_p's order: the coefficients of a colour
+    // sweep's unit-porosity waves
+    double unit_ax, unit_ay, unit_az, unit_ap;
 };
 
 struct SweepArgs {
@@ -50,9 +54,20 @@ void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepA
                 const double *AW, const double *AS, const double *AB, const double *BB, const double *phi,
                 unsigned long long *err_slot);
 // PHC, tab: porosity coded in 4 bytes (phi_decode in pf_cells.h); nullptr reads PHIs.
+// unit: the unit-porosity chunk flags (launch_unit_flags) or nullptr to read porosity everywhere.
 void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                     const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
-                    bool fast, const float *PHC = nullptr, const double *tab = nullptr);
+                    bool fast, const float *PHC = nullptr, const double *tab = nullptr,
+                    const uint8_t *unit = nullptr);
+// Unit-porosity chunk flags (unit_index in pf_layout.h) of a colour-split porosity: `unit`
+// holds unit_flags_size(g) bytes.
+void launch_unit_flags(hipStream_t st, const PfGeom &g, const double *PHIs, uint8_t *unit);
+// The same for the FP32 porosity of the mixed-precision sweeps (values that round to 1).
+void launch_unit_flags(hipStream_t st, const PfGeom &g, const float *PHI32, uint8_t *unit);
+// Cells of both colours that a colour sweep of this shape (xw slots per wave row, kb cells
+// per thread) updates in waves that skip porosity, from a host copy of the flags (the block
+// order does not change which cells share a wave).
+int64_t unit_wave_cells(const PfGeom &g, const std::vector<uint8_t> &unit, int xw, int kb);
 // Code a colour-split FP64 porosity (2 * HS values) in 4 bytes per value: `code` receives
 // the floats, `tab` the distinct values FP32 cannot hold.  Returns false when there are
 // more of those than the code's index holds (`max_tab`), leaving the outputs unspecified.
@@ -101,8 +116,9 @@ void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, float *R, float *E,
                      unsigned long long *slots, bool fast);
+// unit: the unit-porosity chunk flags of the FP32 porosity (launch_unit_flags), or nullptr.
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
-                  const float *R, const double *phi, unsigned long long *err_slot);
+                  const float *R, const double *phi, unsigned long long *err_slot, const uint8_t *unit = nullptr);
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, double *R, double *E,
                      unsigned long long *slots, bool fast);

@@ -238,26 +238,67 @@ __global__ void k_reduce_err(unsigned long long *slots, int nslots, unsigned lon
 
 // PT = double: porosity as stored FP64; PT = float: porosity coded in 4 bytes
 // (phi_decode, table `tab`), the same doubles for 4 fewer bytes per cell and launch.
-template <int DIM, bool FAST, int KB, typename PT>
+// UNIT: `unit` holds the unit-porosity chunk flags (unit_index in pf_layout.h); a wave whose
+// lanes all lie in flagged chunks skips its porosity loads and the face arithmetic.
+template <int DIM, bool FAST, int KB, typename PT, bool UNIT>
 __global__ __launch_bounds__(256) void k_sor_phi(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ Poth,
                                                  double *__restrict__ Pown, const PT *__restrict__ PHIo,
                                                  const PT *__restrict__ PHIx, const double *__restrict__ BBo,
                                                  const double *__restrict__ phi, unsigned long long *err_slot,
-                                                 const double *__restrict__ tab) {
+                                                 const double *__restrict__ tab, const uint8_t *__restrict__ unit) {
     const SweepThread t = sweep_thread<DIM, KB>(g, s);
+    const bool active = t.j <= g.n && t.i <= g.m;
     double err = 0.;
-    if (t.j <= g.n && t.i <= g.m) {
-        SweepIn<double> x[KB];
-        int64_t o[KB];
+    SweepIn<double> x[KB];
+    int64_t o[KB];
+    int ok = 1;
+    if (active) {
+        // the cells' unit flags first (loads return in order: the vote waits for these
+        // only), then the pressure and RHS operands, all in flight together; the flags are
+        // combined after the operand loads are issued
+        uint32_t fl[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
+            fl[q] = UNIT ? (uint32_t)gmem(unit)[unit_index(g, s.colour, t.j, k, t.h + t.sh - 1)] : 0u;
+        }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             // a plane past the last one loads plane l instead (in bounds) and is not updated
             const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
             o[q] = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
+            sweep_gather_unit<DIM>(g, t.sh, o[q], Poth, Pown, BBo, x[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            if (!(DIM == 3 && t.kbase + 2 * q > g.l)) ok &= (int)fl[q];
+    }
+    // the same in every lane, so the branches below are the wave's
+    const bool all_unit = UNIT && __all(ok) != 0;
+    if (active && all_unit) {
+        // every porosity operand is 1: the unit faces, and num / ap by the wave's divisor (ap
+        // is the same for every cell of such a wave)
+        const DivK kap = divk_make(a.unit_ap);
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const int k = t.kbase + 2 * q;
+            if (DIM == 3 && k > g.l) break;
+            const SweepIn<double> &c = x[q];
+            DivFast dv;
+            double pn = sor_unit_value<DIM>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.rhs, phi, dv, kap);
+            if (dv.slow)
+                pn = sor_value<DIM, FAST>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, 1., 1., 1., 1., 1.,
+                                          (DIM == 3) ? 1. : 0., (DIM == 3) ? 1. : 0., c.rhs, phi);
+            sweep_store<DIM>(g, s, t.j, k, o[q], pn, Pown);
+            err = max_keep_nan(err, fabs(pn - c.c));
+        }
+    } else if (active) {
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
             if constexpr (std::is_same<PT, float>::value)
-                sweep_gather_coded<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, tab, BBo, x[q]);
+                sweep_gather_phi_coded<DIM>(g, t.sh, o[q], PHIo, PHIx, tab, x[q]);
             else
-                sweep_gather<DIM>(g, t.sh, o[q], Poth, Pown, PHIo, PHIx, BBo, x[q]);
+                sweep_gather_phi<DIM>(g, t.sh, o[q], PHIo, PHIx, x[q]);
         }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
@@ -435,6 +476,94 @@ __global__ void k_split(PfGeom g, const double *__restrict__ src, double *__rest
     const int k = blockIdx.z;
     if (i > g.m + 1 || j > g.n + 1) return;
     dst[pf_soff(g, i, j, k)] = src[pf_off(g, i, j, k)];
+}
+
+// Unit-porosity chunk flags (unit_index in pf_layout.h) of a colour-split porosity of
+// type T (the FP64 one, or the FP32 one of the mixed-precision sweeps, whose unit values
+// are those that round to 1): one thread per chunk; rows and planes outside the interior
+// get 0 (no sweep reads them).
+template <typename T>
+__global__ void k_unit_flags(PfGeom g, const T *__restrict__ PHIs, uint8_t *__restrict__ unit) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= unit_flags_size(g)) return;
+    const int NQ = unit_chunks(g);
+    const int q = (int)(t % NQ);
+    const int64_t r = t / NQ;
+    const int j = (int)(r % g.Y);
+    const int64_t r2 = r / g.Y;
+    const int k = (int)(r2 % g.Z);
+    const int c = (int)(r2 / g.Z);
+    const bool interior = j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
+    uint8_t ok = 0;
+    if (interior) {
+        ok = 1;
+        const int sh = (c + j + k) & 1;
+        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+        const T *own = PHIs + (int64_t)c * g.HS, *oth = PHIs + (int64_t)(1 - c) * g.HS;
+        const T one = (T)1;
+        for (int x = q * 32; x < q * 32 + 32; ++x) {
+            const int h = x + 1 - sh, i = 2 * h + sh;   // i >= 1 for every x >= 0
+            if (i > g.m) break;
+            const int64_t o = pf_hoff(g, h, j, k);
+            bool u = own[o] == one && oth[o - 1 + sh] == one && oth[o + sh] == one && oth[o - HP] == one &&
+                     oth[o + HP] == one;
+            if (g.dim == 3) u = u && oth[o - HPY] == one && oth[o + HPY] == one;
+            if (!u) {
+                ok = 0;
+                break;
+            }
+        }
+    }
+    unit[t] = ok;
+}
+
+void launch_unit_flags(hipStream_t st, const PfGeom &g, const double *PHIs, uint8_t *unit) {
+    const int64_t n = unit_flags_size(g);
+    hipLaunchKernelGGL(k_unit_flags<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHIs, unit);
+}
+void launch_unit_flags(hipStream_t st, const PfGeom &g, const float *PHI32, uint8_t *unit) {
+    const int64_t n = unit_flags_size(g);
+    hipLaunchKernelGGL(k_unit_flags<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHI32, unit);
+}
+
+template <int DIM, int KB>
+static int64_t unit_wave_cells_t(const PfGeom &g, const std::vector<uint8_t> &unit, int xw) {
+    int64_t cells = 0;
+    for (int colour = 0; colour < 2; ++colour) {
+        SweepArgs s{};
+        s.colour = colour;
+        s.xw = xw;
+        s.kb = KB;
+        const SweepShape sp = sweep_shape(g, KB, xw);
+        s.gx = sp.gx;
+        s.gy = sp.gy;
+        for (int L = 0; L < sp.blocks(); ++L)
+            for (int ty = 0; ty < 4; ++ty) {   // one wave: 64 threads of one threadIdx.y
+                bool all = true;
+                int64_t nw = 0;
+                for (int tx = 0; tx < 64; ++tx) {
+                    const SweepThread t = sweep_thread_at<DIM, KB>(g, s, L, tx, ty);
+                    if (t.j > g.n || t.i > g.m) continue;
+                    for (int q = 0; q < KB; ++q) {
+                        const int k = t.kbase + 2 * q;
+                        if (DIM == 3 && k > g.l) break;
+                        all = all && unit[(size_t)unit_index(g, colour, t.j, k, t.h + t.sh - 1)] != 0;
+                        ++nw;
+                    }
+                }
+                if (all) cells += nw;
+            }
+    }
+    return cells;
+}
+
+int64_t unit_wave_cells(const PfGeom &g, const std::vector<uint8_t> &unit, int xw, int kb) {
+    if ((int64_t)unit.size() != unit_flags_size(g)) return 0;
+    xw = (xw == 32) ? 32 : 64;
+    if (g.dim == 2) return unit_wave_cells_t<2, 1>(g, unit, xw);
+    if (kb == 2) return unit_wave_cells_t<3, 2>(g, unit, xw);
+    if (kb == 4) return unit_wave_cells_t<3, 4>(g, unit, xw);
+    return unit_wave_cells_t<3, 1>(g, unit, xw);
 }
 
 // Compares the fast face formula with IEEE division for every face k_faces stored; any
@@ -723,7 +852,7 @@ void launch_sor(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepA
 
 void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const SweepArgs &s, double *P,
                     const double *PHIs, const double *BB, const double *phi, unsigned long long *err_slot,
-                    bool fast, const float *PHC, const double *tab) {
+                    bool fast, const float *PHC, const double *tab, const uint8_t *unit) {
     const int c = s.colour;
     const int64_t own = (int64_t)c * g.HS, oth = (int64_t)(1 - c) * g.HS;
     const int KB = (g.dim == 3) ? s.kb : 1;
@@ -733,10 +862,13 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
     s1.gx = sp.gx;
     s1.gy = sp.gy;
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
-#define PF_SOR_PHI(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K, double>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
-                                               PHIs + own, PHIs + oth, BB + own, phi, err_slot, (const double *)nullptr)
-#define PF_SOR_PHC(D, F, K) hipLaunchKernelGGL((k_sor_phi<D, F, K, float>), gr, b, 0, st, g, a, s1, P + oth, P + own, \
-                                               PHC + own, PHC + oth, BB + own, phi, err_slot, tab)
+#define PF_SOR_PHI_U(D, F, K, U) hipLaunchKernelGGL((k_sor_phi<D, F, K, double, U>), gr, b, 0, st, g, a, s1, P + oth, \
+                                                   P + own, PHIs + own, PHIs + oth, BB + own, phi, err_slot, \
+                                                   (const double *)nullptr, unit)
+#define PF_SOR_PHC_U(D, F, K, U) hipLaunchKernelGGL((k_sor_phi<D, F, K, float, U>), gr, b, 0, st, g, a, s1, P + oth, \
+                                                   P + own, PHC + own, PHC + oth, BB + own, phi, err_slot, tab, unit)
+#define PF_SOR_PHI(D, F, K) do { if (unit) PF_SOR_PHI_U(D, F, K, true); else PF_SOR_PHI_U(D, F, K, false); } while (0)
+#define PF_SOR_PHC(D, F, K) do { if (unit) PF_SOR_PHC_U(D, F, K, true); else PF_SOR_PHC_U(D, F, K, false); } while (0)
     if (PHC && tab && KB == 1) {
         if (g.dim == 3) { if (fast) PF_SOR_PHC(3, true, 1); else PF_SOR_PHC(3, false, 1); }
         else { if (fast) PF_SOR_PHC(2, true, 1); else PF_SOR_PHC(2, false, 1); }
@@ -751,6 +883,8 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
     }
 #undef PF_SOR_PHI
 #undef PF_SOR_PHC
+#undef PF_SOR_PHI_U
+#undef PF_SOR_PHC_U
 }
 
 bool code_porosity(const std::vector<double> &phis, std::vector<float> &code, std::vector<double> &tab,

@@ -63,3 +63,15 @@ inline PfGeom pf_make_geom(int dim, int m, int n, int l) {
     g.HS = (int64_t)g.HP * g.Y * g.Z;
     return g;
 }
+
+// Unit-porosity chunks of the colour sweeps.  One byte per 32 colour slots of a row,
+// (colour, plane, row, chunk), is 1 when every cell of that colour the chunk holds
+// (1 <= i <= m) and its six neighbours have porosity exactly 1.  A sweep lane's chunk is
+// its slot position h + sh - 1 (0 for the first lane of the row) over 32, so a wave of 64
+// slots spans two chunks of one row and a wave of 32 slots one chunk of each of its two
+// rows.
+PF_HD int unit_chunks(const PfGeom &g) { return ((g.m + 1) / 2 + 63) / 64 * 2; }
+PF_HD int64_t unit_flags_size(const PfGeom &g) { return (int64_t)2 * g.Z * g.Y * unit_chunks(g); }
+PF_HD int64_t unit_index(const PfGeom &g, int c, int j, int k, int x) {
+    return (((int64_t)c * g.Z + k) * g.Y + j) * unit_chunks(g) + (x >> 5);
+}

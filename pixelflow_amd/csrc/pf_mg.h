@@ -106,6 +106,8 @@ struct MgCase {
     const float *PHI32 = nullptr;    // colour-split FP32 porosity (FP32 solver)
     const float *PHC = nullptr;      // FP64 solver: the porosity coded in 4 bytes, and its table
     const double *PHTAB = nullptr;   //   (used by the level-0 sweeps while `coded`)
+    const uint8_t *UNIT = nullptr;   // FP64 solver: the unit-porosity chunk flags, or nullptr
+    const uint8_t *UNIT32 = nullptr; // FP32 solver: the same flags of the FP32 porosity, or nullptr
     bool coded = false;
     bool fast = false;               // FP64 faces by the checked constant-divisor division
     int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings

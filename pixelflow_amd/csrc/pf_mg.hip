@@ -1316,7 +1316,7 @@ static void sweep0(const MgSolver<float> &S, hipStream_t st, const StepArgs &a1,
     sw.xcd_remap = S.c.remap32;
     sw.kb = S.c.kb32;
     sw.xw = S.c.xw;
-    launch_sor32(st, g, a1, sw, e, S.c.PHI32, rhs, S.c.phi, nullptr);
+    launch_sor32(st, g, a1, sw, e, S.c.PHI32, rhs, S.c.phi, nullptr, S.c.UNIT32);
     if (l0.ref_y || l0.ref_z) launch_ghost_refresh32(st, g, l0.ref_y, l0.ref_z, e);
 }
 static void sweep0(const MgSolver<double> &S, hipStream_t st, const StepArgs &a1, int colour, double *e,
@@ -1332,7 +1332,7 @@ static void sweep0(const MgSolver<double> &S, hipStream_t st, const StepArgs &a1
     sw.kb = S.c.kb64;
     sw.xw = S.c.xw;
     launch_sor_phi(st, g, a1, sw, e, S.c.PHIs, rhs, S.c.phi, nullptr, S.c.fast, S.c.coded ? S.c.PHC : nullptr,
-                   S.c.PHTAB);
+                   S.c.PHTAB, S.c.UNIT);
     if (l0.ref_y || l0.ref_z) launch_ghost_refresh(st, g, l0.ref_y, l0.ref_z, e);
 }
 static void refresh0(const MgSolver<float> &S, hipStream_t st, float *e) {

@@ -40,31 +40,56 @@ __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArg
 // FP32 red/black SOR half-sweep on A e = r (same structure as the FP64 sweep).
 // ------------------------------------------------------------------------------------
 
-template <int DIM, int KB>
+// UNIT: `unit` holds the unit-porosity chunk flags of the FP32 porosity; a wave whose lanes
+// all lie in flagged chunks skips its porosity loads and uses 1 for them (the same floats).
+template <int DIM, int KB, bool UNIT>
 __global__ __launch_bounds__(256) void k_sor32(PfGeom g, StepArgs a, SweepArgs s, const float *__restrict__ Eoth,
                                                float *__restrict__ Eown, const float *__restrict__ F32o,
                                                const float *__restrict__ F32x, const float *__restrict__ Ro,
                                                const double *__restrict__ phi, float cdx, float cdy, float cdz,
-                                               unsigned long long *err_slot) {
+                                               unsigned long long *err_slot, const uint8_t *__restrict__ unit) {
     const SweepThread t = sweep_thread<DIM, KB>(g, s);
+    const bool active = t.j <= g.n && t.i <= g.m;
     double err = 0.;
-    if (t.j <= g.n && t.i <= g.m) {
-        SweepIn<float> x[KB];
-        int64_t o[KB];
+    SweepIn<float> x[KB];
+    int64_t o[KB];
+    int ok = 1;
+    if (active) {
+        // the unit flags first (the vote waits for these only), then the correction and
+        // residual operands
+        uint32_t fl[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
+            fl[q] = UNIT ? (uint32_t)gmem(unit)[unit_index(g, s.colour, t.j, k, t.h + t.sh - 1)] : 0u;
+        }
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             // a plane past the last one loads plane l instead (in bounds) and is not updated
             const int k = (DIM == 3 && t.kbase + 2 * q > g.l) ? g.l : t.kbase + 2 * q;
             o[q] = (int64_t)t.h + (int64_t)g.HP * ((int64_t)t.j + (int64_t)g.Y * k);
-            sweep_gather<DIM>(g, t.sh, o[q], Eoth, Eown, F32o, F32x, Ro, x[q]);
+            sweep_gather_unit<DIM>(g, t.sh, o[q], Eoth, Eown, Ro, x[q]);
         }
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            if (!(DIM == 3 && t.kbase + 2 * q > g.l)) ok &= (int)fl[q];
+    }
+    const bool all_unit = UNIT && __all(ok) != 0;   // the wave's branch
+    if (active && !all_unit) {
+#pragma unroll
+        for (int q = 0; q < KB; ++q) sweep_gather_phi32<DIM>(g, t.sh, o[q], F32o, F32x, x[q]);
+    }
+    if (active) {
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
             const int k = t.kbase + 2 * q;
             if (DIM == 3 && k > g.l) break;
             const SweepIn<float> &c = x[q];
-            const float en = sor32_value<DIM>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw, c.fe,
-                                              c.fs, c.fn, c.fb, c.ft, c.rhs, phi, cdx, cdy, cdz);
+            const float en = all_unit
+                                 ? sor32_value<DIM>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, 1.f, 1.f, 1.f,
+                                                    1.f, 1.f, 1.f, 1.f, c.rhs, phi, cdx, cdy, cdz)
+                                 : sor32_value<DIM>(g, a, t.i, t.j, k, c.c, c.w, c.e, c.s, c.n, c.b, c.t, c.fc, c.fw,
+                                                    c.fe, c.fs, c.fn, c.fb, c.ft, c.rhs, phi, cdx, cdy, cdz);
             sweep_store<DIM>(g, s, t.j, k, o[q], en, Eown);
             const double e = fabs((double)en - (double)c.c);
             err = max_keep_nan(err, e);
@@ -152,7 +177,7 @@ void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepAr
 }
 
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
-                  const float *R, const double *phi, unsigned long long *err_slot) {
+                  const float *R, const double *phi, unsigned long long *err_slot, const uint8_t *unit) {
     const int KB = (g.dim == 3) ? s.kb : 1;
     s.xw = (s.xw == 32) ? 32 : 64;   // an unset width means one row per wave
     const SweepShape sp = sweep_shape(g, KB, s.xw);
@@ -162,14 +187,16 @@ void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs 
     const int64_t own = (int64_t)s.colour * g.HS, oth = (int64_t)(1 - s.colour) * g.HS;
     const float cdx = (float)(a.dt / (a.dx * a.dx)), cdy = (float)(a.dt / (a.dy * a.dy)),
                 cdz = (float)(a.dt / (a.dz * a.dz));
-#define PF_S32(D, K) hipLaunchKernelGGL((k_sor32<D, K>), gr, b, 0, st, g, a, s, E + oth, E + own, F32 + own, F32 + oth, \
-                                        R + own, phi, cdx, cdy, cdz, err_slot)
+#define PF_S32_U(D, K, U) hipLaunchKernelGGL((k_sor32<D, K, U>), gr, b, 0, st, g, a, s, E + oth, E + own, F32 + own, \
+                                             F32 + oth, R + own, phi, cdx, cdy, cdz, err_slot, unit)
+#define PF_S32(D, K) do { if (unit) PF_S32_U(D, K, true); else PF_S32_U(D, K, false); } while (0)
     if (g.dim == 3) {
         if (KB == 1) PF_S32(3, 1); else if (KB == 2) PF_S32(3, 2); else PF_S32(3, 4);
     } else {
         PF_S32(2, 1);
     }
 #undef PF_S32
+#undef PF_S32_U
 }
 
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P) {

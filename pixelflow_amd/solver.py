@@ -62,7 +62,8 @@ _lib: Optional[ctypes.CDLL] = None
 # Every symbol include/pixelflow.h declares (checked by tests/test_abi.py).
 EXPORTS = ("pf_abi_version", "pf_create", "pf_destroy", "pf_last_error", "pf_set_porosity", "pf_initial",
            "pf_upload", "pf_download", "pf_step", "pf_divergence", "pf_force", "pf_force_noalloc", "pf_footprint_bytes", "pf_sync",
-           "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_sor_fused", "pf_sor_coded", "pf_probe_copy",
+           "pf_set_timing", "pf_last_sweep_time_us", "pf_sor_kernel", "pf_sor_fused", "pf_sor_coded", "pf_sor_unit_cells",
+           "pf_probe_copy",
            "pf_batch_create", "pf_batch_destroy", "pf_batch_last_error", "pf_batch_case_bytes", "pf_batch_free_bytes",
            "pf_batch_add", "pf_batch_remove", "pf_batch_count", "pf_batch_step", "pf_batch_download",
            "pf_batch_divergence", "pf_batch_set_config", "pf_batch_get_config", "pf_batch_sync", "pf_batch_case_steps",
@@ -123,6 +124,7 @@ def lib() -> ctypes.CDLL:
     L.pf_sor_kernel.argtypes = [vp]
     L.pf_sor_fused.argtypes = [vp]
     L.pf_sor_coded.argtypes = [vp]
+    L.pf_sor_unit_cells.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.pf_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int, dp]
     L.pf_batch_create.argtypes = [ctypes.POINTER(PfDesc), ctypes.c_int, ctypes.c_int64, ctypes.POINTER(vp)]
     L.pf_batch_destroy.argtypes = [vp]
@@ -271,6 +273,13 @@ class Solver:
     def sor_coded(self) -> bool:
         """True when the FP64 sweeps read the porosity coded in 4 bytes per value."""
         return bool(lib().pf_sor_coded(self._h))
+
+    def sor_unit_cells(self) -> int:
+        """Cells of both colours whose FP64 colour sweeps skip the porosity loads (their wave's
+        cells and neighbours all have porosity 1); 0 when the unit-porosity flags are off."""
+        n = ctypes.c_int64()
+        self._check(lib().pf_sor_unit_cells(self._h, ctypes.byref(n)), "pf_sor_unit_cells")
+        return int(n.value)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

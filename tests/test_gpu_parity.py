@@ -437,3 +437,99 @@ def test_gpu_mixed_block_order_identical(name, monkeypatch):
     for f in _fields(p):
         np.testing.assert_array_equal(out["1"][0][f], out["0"][0][f], err_msg=f"{name}:{f}")
     np.testing.assert_array_equal(out["1"][1], out["0"][1])
+
+
+def _sparse_porosity(p, seed):
+    """Porosity 1 with isolated cells of other values: on the domain's first and last rows,
+    planes and columns (whose ghosts the periodic and outflow copies spread), on both
+    sides of the 32-slot chunk boundaries (i = 63..66 and 127..130), and at random."""
+    rng = np.random.default_rng(seed)
+    shape = (p.l, p.n, p.m) if p.dim == 3 else (p.n, p.m)
+    phi = np.ones(shape)
+    if p.dim == 3:
+        spots = [(0, 0, 0), (p.l - 1, p.n - 1, p.m - 1), (5, 0, 63), (7, p.n - 1, 64), (0, 17, 65),
+                 (p.l - 1, 3, 66), (11, 20, 127), (3, 9, 128), (9, 30, 129), (2, 12, 1)]
+    else:
+        spots = [(0, 0), (p.n - 1, p.m - 1), (0, 63), (p.n - 1, 64), (17, 65), (3, 66), (20, 127), (9, 128),
+                 (30, 129), (12, 1)]
+    spots += [tuple(int(rng.integers(0, s)) for s in shape) for _ in range(6)]
+    for c in spots:
+        phi[c] = rng.uniform(0.3, 0.95)
+    return phi
+
+
+@pytest.mark.parametrize("variant", ["default", "coded", "kb2", "xw64", "flags_off", "2d"])
+def test_gpu_unit_porosity_waves_match_oracle(variant, monkeypatch):
+    """FP64 colour sweeps skip the porosity loads of waves whose cells and neighbours all
+    have porosity exactly 1 (and use 1 for those operands).  A field of porosity 1 with
+    isolated other values at the faces, at the chunk boundaries and at random: a wave taken
+    as unit where one of its operands is not 1 would change the pressure.  Three steps
+    against the oracle, bit for bit, with each launch shape and porosity storage."""
+    import pyoracle
+    env = {"default": {}, "coded": {"PF_PHI_CODE": "1"}, "kb2": {"PF_SWEEP_KB": "2"}, "xw64": {"PF_SWEEP_XW": "64"},
+           "flags_off": {"PF_UNIT_PHI": "0"}, "2d": {"PF_SOR_TB2": "0,0"}}[variant]
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    if variant == "2d":
+        p = cases.CaseParams(dim=2, m=134, n=34, width=1.34, height=0.34, time=3e-4, istep_max=3, iter_max=20)
+    else:
+        p = cases.CaseParams(dim=3, m=134, n=34, l=22, width=1.34, height=0.34, depth=0.22, time=3e-4, istep_max=3,
+                             iter_max=20)
+    phi = _sparse_porosity(p, 11)
+    print(f"[gpu] unit-porosity waves: {variant}", flush=True)
+    o = pyoracle.OracleRun(p, phi)
+    e_ref = o.step(p.istep_max)
+    with _solver(p, phi) as s:
+        unit = s.sor_unit_cells()
+        if variant == "flags_off":
+            assert unit == 0
+        else:
+            assert 0 < unit < p.cells
+        st = s.step(p.istep_max)
+        np.testing.assert_array_equal(st["sor_error"], e_ref)
+        for f in _fields(p):
+            np.testing.assert_array_equal(s.interior(f), o.interior(f), err_msg=f"{variant}:{f}")
+
+
+def test_gpu_unit_porosity_cell_count_extremes(monkeypatch):
+    """All cells in unit waves when the porosity is 1 everywhere (ghosts included), none when
+    no value is 1, none with the flags turned off."""
+    p = cases.CaseParams(dim=3, m=64, n=32, l=16, width=1.0, height=0.5, depth=0.25, time=1e-4, istep_max=1,
+                         iter_max=5)
+    ones = np.ones((p.l, p.n, p.m))
+    with _solver(p, ones) as s:
+        assert s.sor_unit_cells() == p.cells
+    rnd = np.random.default_rng(3).uniform(0.2, 0.99, size=ones.shape)
+    with _solver(p, rnd) as s:
+        assert s.sor_unit_cells() == 0
+    monkeypatch.setenv("PF_UNIT_PHI", "0")
+    with _solver(p, ones) as s:
+        assert s.sor_unit_cells() == 0
+
+
+@pytest.mark.parametrize("mode", ["mixed_fixed", "mixed_mg", "fp64_mg"])
+def test_gpu_unit_porosity_solvers_identical(mode, monkeypatch):
+    """The mixed-precision FP32 sweeps (flags of the FP32 porosity, whose unit values are
+    those that round to 1) and the multigrid's level-0 smoothers give the same bits with
+    and without unit-porosity waves, on the sparse field above plus values within an FP32
+    rounding of 1."""
+    from pixelflow_amd import solver as S
+    p = cases.CaseParams(dim=3, m=134, n=34, l=22, width=1.34, height=0.34, depth=0.22, time=3e-4, istep_max=3,
+                         iter_max=20)
+    phi = _sparse_porosity(p, 5)
+    phi[4, 10, 40] = 1.0 - 2.0 ** -30   # 1.0 in FP32, not in FP64
+    phi[9, 22, 99] = 1.0 - 2.0 ** -20
+    kw = {"mixed_fixed": dict(precision=S.PF_PREC_MIXED, inner_iters=10),
+          "mixed_mg": dict(precision=S.PF_PREC_MIXED, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30),
+          "fp64_mg": dict(precision=S.PF_PREC_FP64, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30)}[mode]
+    print(f"[gpu] unit porosity, {mode}", flush=True)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PF_UNIT_PHI", flag)
+        with S.Solver(p, phi, **kw) as s:
+            st = s.step(p.istep_max)
+            out[flag] = ({f: s.interior(f) for f in _fields(p)}, st)
+    for f in _fields(p):
+        np.testing.assert_array_equal(out["1"][0][f], out["0"][0][f], err_msg=f"{mode}:{f}")
+    for key in ("sor_error", "residual", "sor_iters"):
+        np.testing.assert_array_equal(out["1"][1][key], out["0"][1][key], err_msg=f"{mode}:{key}")
