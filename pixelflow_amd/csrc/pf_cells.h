@@ -826,8 +826,8 @@ __device__ __forceinline__ CellIdx sweep_cell(const PfGeom &g, const SweepArgs &
     CellIdx c;
     const int nb = (int)gridDim.x;
     const int L = s.xcd_remap ? xcd_logical_block((int)blockIdx.x, nb) : (int)blockIdx.x;
-    const int bx = L % s.gx, byz = L / s.gx;
-    const int by = byz % s.gy, bz = byz / s.gy;
+    const int byz = (int)grid_div((uint32_t)L, (uint32_t)s.gx, s.mgx), bx = L - byz * s.gx;
+    const int bz = (int)grid_div((uint32_t)byz, (uint32_t)s.gy, s.mgy), by = byz - bz * s.gy;
     c.j = by * blockDim.y + threadIdx.y + 1;
     c.k = (DIM == 3) ? bz + 1 : 0;
     c.sh = (s.colour + c.j + c.k) & 1;

@@ -230,11 +230,29 @@ struct SweepThread {
 
 // Block (64, 4): threadIdx.x runs over xw slots of rpw = 64 / xw rows; 3D: threadIdx.y =
 // (row pair offset, plane parity), 2D: threadIdx.y = row group.
+// Block index / grid extent by multiply-high: m = floor(2^32 / d) + 1 gives
+// umulhi(x, m) = x / d exactly for x * d < 2^32 (the error x * (m - 2^32 / d) / 2^32 stays
+// below 1 / d).  grid_magic returns 0 (divide instead) for d <= 1 or a grid past that bound.
+PF_HD uint32_t grid_magic(uint32_t d, uint32_t blocks) {
+    if (d <= 1 || (uint64_t)blocks * d >= ((uint64_t)1 << 32)) return 0u;
+    return (uint32_t)((((uint64_t)1) << 32) / d + 1);
+}
+PF_HD uint32_t grid_div(uint32_t x, uint32_t d, uint32_t m) {
+    return m ? (uint32_t)(((uint64_t)x * m) >> 32) : (d <= 1 ? x : x / d);
+}
+// gx, gy and their multipliers for a grid of `blocks` blocks.
+PF_HD void sweep_set_grid(SweepArgs &s, int gx, int gy, int blocks) {
+    s.gx = gx;
+    s.gy = gy;
+    s.mgx = grid_magic((uint32_t)gx, (uint32_t)blocks);
+    s.mgy = grid_magic((uint32_t)gy, (uint32_t)blocks);
+}
+
 // The cell of thread (tx, ty) of logical block L (host and device).
 template <int DIM, int KB>
 PF_HD SweepThread sweep_thread_at(const PfGeom &g, const SweepArgs &s, int L, int tx, int ty) {
-    const int bx = L % s.gx, byz = L / s.gx;
-    const int by = byz % s.gy, bz = byz / s.gy;
+    const int byz = (int)grid_div((uint32_t)L, (uint32_t)s.gx, s.mgx), bx = L - byz * s.gx;
+    const int bz = (int)grid_div((uint32_t)byz, (uint32_t)s.gy, s.mgy), by = byz - bz * s.gy;
     const int rpw = (s.xw == 32) ? 2 : 1;
     const int sub = (s.xw == 32) ? (tx >> 5) : 0;
     const int xl = (s.xw == 32) ? (tx & 31) : tx;

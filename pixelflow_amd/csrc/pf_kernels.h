@@ -38,6 +38,7 @@ struct SweepArgs {
     int track_err;      // reduce max |dp| into the step's error slot
     int xcd_remap;      // renumber blocks XCD-contiguously
     int gx, gy;         // block grid extents of the sweep (set by the launcher)
+    uint32_t mgx, mgy;  // x / gx and x / gy as umulhi(x, m) for every block index x (grid_magic; 0: divide)
     int kb;             // 3D: cells per thread, two planes apart (1, 2 or 4)
     int xw;             // colour slots of one row a wave covers: 64 (one row) or 32 (two rows)
 };

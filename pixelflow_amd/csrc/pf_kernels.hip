@@ -535,8 +535,7 @@ static int64_t unit_wave_cells_t(const PfGeom &g, const std::vector<uint8_t> &un
         s.xw = xw;
         s.kb = KB;
         const SweepShape sp = sweep_shape(g, KB, xw);
-        s.gx = sp.gx;
-        s.gy = sp.gy;
+        sweep_set_grid(s, sp.gx, sp.gy, sp.blocks());
         for (int L = 0; L < sp.blocks(); ++L)
             for (int ty = 0; ty < 4; ++ty) {   // one wave: 64 threads of one threadIdx.y
                 bool all = true;
@@ -859,8 +858,7 @@ void launch_sor_phi(hipStream_t st, const PfGeom &g, const StepArgs &a, const Sw
     SweepArgs s1 = s;
     s1.xw = (s.xw == 32) ? 32 : 64;   // an unset width means one row per wave
     const SweepShape sp = sweep_shape(g, KB, s1.xw);
-    s1.gx = sp.gx;
-    s1.gy = sp.gy;
+    sweep_set_grid(s1, sp.gx, sp.gy, sp.blocks());
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
 #define PF_SOR_PHI_U(D, F, K, U) hipLaunchKernelGGL((k_sor_phi<D, F, K, double, U>), gr, b, 0, st, g, a, s1, P + oth, \
                                                    P + own, PHIs + own, PHIs + oth, BB + own, phi, err_slot, \

@@ -149,8 +149,7 @@ static inline dim3 sweep_grid(const PfGeom &g, SweepArgs &s, dim3 b) {
     const unsigned gx = (unsigned)(((g.m + 1) / 2 + (int)b.x - 1) / (int)b.x);
     const unsigned gy = (unsigned)((g.n + (int)b.y - 1) / (int)b.y);
     const unsigned gz = (unsigned)(g.dim == 3 ? g.l : 1);
-    s.gx = (int)gx;
-    s.gy = (int)gy;
+    sweep_set_grid(s, (int)gx, (int)gy, (int)(gx * gy * gz));
     return dim3(gx * gy * gz);
 }
 
@@ -181,8 +180,7 @@ void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs 
     const int KB = (g.dim == 3) ? s.kb : 1;
     s.xw = (s.xw == 32) ? 32 : 64;   // an unset width means one row per wave
     const SweepShape sp = sweep_shape(g, KB, s.xw);
-    s.gx = sp.gx;
-    s.gy = sp.gy;
+    sweep_set_grid(s, sp.gx, sp.gy, sp.blocks());
     dim3 b(64, 4, 1), gr((unsigned)sp.blocks());
     const int64_t own = (int64_t)s.colour * g.HS, oth = (int64_t)(1 - s.colour) * g.HS;
     const float cdx = (float)(a.dt / (a.dx * a.dx)), cdy = (float)(a.dt / (a.dy * a.dy)),
