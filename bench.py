@@ -231,6 +231,7 @@ def mixed_comparison(p, phi, steps: int, warmup: int, inner: int, fp64_value: fl
     b.sync()
     wall = time.perf_counter() - t0
     sweep_us, _ = b.last_sweep_time_us()
+    unit = b.sor_unit_cells()
     a = Solver(p, phi)
     a.step(warmup + steps)
     errs = {}
@@ -241,15 +242,17 @@ def mixed_comparison(p, phi, steps: int, warmup: int, inner: int, fp64_value: fl
     a.close()
     b.close()
     v = p.cells * steps / wall
-    # mixed bytes per cell-iteration: 2 FP32 half-sweeps of 3 words, per cycle an FP64
-    # residual (p 8, bb 8, porosity 8 read, r 4 + e 4 written) and an update (e 4, p 8 + 8)
+    # mixed bytes per cell-iteration: 2 FP32 half-sweeps of 3 words (less 2 words per cell of
+    # a unit-porosity wave over the two: its own porosity and its neighbours'), per cycle an
+    # FP64 residual (p 8, bb 8, porosity 8 read, r 4 + e 4 written) and an update (e 4, p 8 + 8)
     K = inner or p.iter_max
     cyc = (p.iter_max + K - 1) // K
-    sor_bytes_cell = p.iter_max * 2 * 3 * 4 + cyc * (8 * 3 + 4 + 4 + 4 + 16)
+    solve_bytes = p.cells * (p.iter_max * 2 * 3 * 4 + cyc * (8 * 3 + 4 + 4 + 4 + 16)) - p.iter_max * 8 * unit
     return {"value": v, "unit": "cell-steps/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
             "speedup_vs_fp64": v / fp64_value, "sor_cell_updates_per_s": v * p.iter_max,
             "inner_iters": K, "cycles_per_step": cyc, "avg_solve_us_per_iteration_half": sweep_us,
-            "solve_roofline_frac": (sor_bytes_cell * p.cells / (sweep_us * 2 * p.iter_max * 1e-6) / 1e9) / HBM_PEAK_GBS,
+            "solve_roofline_frac": (solve_bytes / (sweep_us * 2 * p.iter_max * 1e-6) / 1e9) / HBM_PEAK_GBS,
+            "unit_porosity_cells": unit,
             "field_rel_l2_vs_fp64": errs, "max_abs_divergence_diff_vs_fp64": ddiv,
             "mode": "fixed-count (iter_max FP32 SOR iterations in FP64 defect-correction cycles)"}
 
@@ -312,9 +315,10 @@ def tolerance_large(steps: int, warmup: int, tol: float):
                                 "vcycles_per_step": float(np.mean(st["sor_iters"])),
                                 "cycles_per_step": float(np.mean(st["cycles"]))}
     res["speedup_vs_fp64_single_solve"] = res["fp64_single_solve"]["ms_per_step"] / res["mixed"]["ms_per_step"]
-    # the fixed-count FP64 path twice: porosity read as stored FP64 (the roofline figure:
-    # 24 B per grid cell and launch) and coded in 4 bytes (the library's default at this
-    # size: 20 B)
+    # the fixed-count FP64 path twice: porosity read as stored FP64 (the library's default
+    # here, where unit-porosity waves hold most cells; the roofline figure: 24 B per grid
+    # cell and launch, less 8 B per cell in a unit wave) and coded in 4 bytes (PF_PHI_CODE=1:
+    # 20 B, less 4 B per such cell)
     runs = {}
     saved = os.environ.get("PF_PHI_CODE")
     try:
@@ -328,12 +332,13 @@ def tolerance_large(steps: int, warmup: int, tol: float):
             os.environ.pop("PF_PHI_CODE", None)
         else:
             os.environ["PF_PHI_CODE"] = saved
-    wall, sweep_us, _ = runs["1"]
+    wall, sweep_us = runs["0"][0], runs["0"][1]
     res["fixed_count_fp64"] = {"ms_per_step": wall * 1e3 / steps, "cell_steps_per_s": p.cells * steps / wall,
                                "iter_max": p.iter_max, "avg_colour_sweep_us": sweep_us,
-                               "porosity": "coded in 4 bytes" if runs["1"][2] else "stored FP64",
-                               "ms_per_step_porosity_fp64": runs["0"][0] * 1e3 / steps,
-                               "avg_colour_sweep_us_porosity_fp64": runs["0"][1]}
+                               "porosity": "coded in 4 bytes" if runs["0"][2] else "stored FP64",
+                               "unit_porosity_cells": runs["0"][3],
+                               "ms_per_step_porosity_coded": runs["1"][0] * 1e3 / steps,
+                               "avg_colour_sweep_us_porosity_coded": runs["1"][1]}
 
     def roofline(sweep_us, bytes_per_cell, unit_bytes, unit, kernel, traffic):
         # bytes_per_cell per grid cell and launch, less unit_bytes per cell of either colour
@@ -350,11 +355,11 @@ def tolerance_large(steps: int, warmup: int, tol: float):
         return out
 
     res["sweep_roofline"] = roofline(runs["0"][1], 24, 8, runs["0"][3], "k_sor_phi (red/black colour sweep, FP64)",
-                                     measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}", "uncoded_kernel"))
+                                     measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"))
     if runs["1"][2]:
         res["sweep_roofline"]["coded"] = roofline(
             runs["1"][1], 20, 4, runs["1"][3], "k_sor_phi (red/black colour sweep, FP64, porosity coded in 4 bytes)",
-            measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}"))
+            measured_traffic("k_sor_phi", f"boxes3d_{p.m}x{p.n}x{p.l}", "coded_kernel"))
     return res
 
 
@@ -464,9 +469,10 @@ def main(argv=None):
         # read once (4 FP64 words per cell)
         kernel, launch_us, launch_bytes = "k_sor_fused (one red/black SOR iteration)", 2 * sweep_us, 4 * 8 * cells
     else:
-        # one launch = one colour: 3 FP64 words per cell, less the porosity word of the cells
-        # whose waves skip it (unit porosity: 2 of the 3 words per updated cell, so 1 word
-        # per cell of either colour over a launch pair)
+        # one launch = one colour: 3 FP64 words per grid cell, i.e. 6 per updated cell, 2 of
+        # them porosity (the cell's own and its share of its neighbours'); a unit-porosity
+        # wave skips those 2, which over a launch pair is 1 word per launch for each cell,
+        # of either colour, in such waves
         kernel, launch_us, launch_bytes = "k_sor_phi (red/black colour sweep)", sweep_us, 3 * 8 * cells - 8 * unit
     achieved = launch_bytes / (launch_us * 1e-6) / 1e9 if launch_us > 0 else 0.0
     sor_bytes = 4 * 8 * cells if fused else 2 * launch_bytes   # per SOR iteration

@@ -165,11 +165,13 @@ int pf_sor_fused(const pf_ctx *ctx);
  * (smaller grids, too many distinct values, or a precision other than FP64).  Same
  * results.  No reference counterpart. */
 int pf_sor_coded(const pf_ctx *ctx);
-/* Cells of both colours that the FP64 colour sweeps update in waves skipping the porosity
+/* Cells of both colours that the colour sweeps update in waves skipping the porosity
  * loads: waves whose cells and their six neighbours all have porosity exactly 1 take 1 for
- * those operands (the same doubles, 8 bytes fewer per updated cell and sweep; one flag byte
- * per 32 colour slots of a row marks them, built by pf_set_porosity; PF_UNIT_PHI=0 in the
- * environment turns it off, *cells is then 0).  Same results.  No reference counterpart. */
+ * those operands (the same values, 8 bytes fewer per updated cell and sweep in FP64; one
+ * flag byte per 32 colour slots of a row marks them, built by pf_set_porosity).  FP64
+ * contexts count the FP64 sweeps, mixed-precision ones the FP32 sweeps (whose unit values
+ * are those that round to 1).  PF_UNIT_PHI=0 in the environment turns it off (*cells is
+ * then 0).  Same results.  No reference counterpart. */
 int pf_sor_unit_cells(const pf_ctx *ctx, int64_t *cells);
 int pf_last_sweep_time_us(const pf_ctx *ctx, double *avg_us, int64_t *launches);
 

@@ -58,8 +58,9 @@ struct pf_ctx {
     bool phi_code_forced = false;   // PF_PHI_CODE=1: code even where unit waves dominate
     // FP64 sweeps: one byte per 32 colour slots of a row, 1 where every porosity operand of
     // those cells is 1 (unit_index in pf_layout.h); waves of flagged chunks skip their
-    // porosity loads.  unit_cells: cells of both colours in such waves (PF_UNIT_PHI=0 in
-    // the environment turns the flags off)
+    // porosity loads.  unit_cells: cells of both colours in such waves, of the FP64 sweeps
+    // or, in mixed precision, of the FP32 sweeps (PF_UNIT_PHI=0 in the environment turns
+    // the flags off)
     uint8_t *UNIT = nullptr;
     int64_t unit_cells = 0;
     uint8_t *UNIT32 = nullptr;      // mixed precision: the same flags of the FP32 porosity
@@ -402,7 +403,13 @@ static int derive_from_porosity(pf_ctx *c) {
         if (c->sor_variant != 0 && !c->fused && c->tb2_k == 0)
             c->unit_cells = pf::unit_wave_cells(g, h, c->sweep_xw, g.dim == 3 ? c->sweep_kb : 1);
     }
-    if (c->UNIT32 && c->PHI32) pf::launch_unit_flags(c->st, g, c->PHI32, c->UNIT32);
+    if (c->UNIT32 && c->PHI32) {
+        pf::launch_unit_flags(c->st, g, c->PHI32, c->UNIT32);
+        std::vector<uint8_t> h((size_t)unit_flags_size(g));
+        PF_TRY_HIP(c, hipMemcpyAsync(h.data(), c->UNIT32, h.size(), hipMemcpyDeviceToHost, c->st));
+        PF_TRY_HIP(c, hipStreamSynchronize(c->st));
+        if (!c->fused) c->unit_cells = pf::unit_wave_cells(g, h, c->sweep_xw, g.dim == 3 ? c->sweep_kb32 : 1);
+    }
     // coded porosity only where unit waves hold less than half the cells: a unit wave reads
     // no porosity, and at 512x256x256 with 65 % of the cells in unit waves the sweep on
     // stored FP64 porosity took 165 us against 195 us coded

@@ -480,50 +480,53 @@ __global__ void k_split(PfGeom g, const double *__restrict__ src, double *__rest
 
 // Unit-porosity chunk flags (unit_index in pf_layout.h) of a colour-split porosity of
 // type T (the FP64 one, or the FP32 one of the mixed-precision sweeps, whose unit values
-// are those that round to 1): one thread per chunk; rows and planes outside the interior
-// get 0 (no sweep reads them).
+// are those that round to 1).  One thread per slot position of every (colour, plane, row);
+// a row holds an even number of chunks, so each wave covers two whole chunks of one row and
+// a ballot of the failing lanes gives both flags.  Rows and planes outside the interior get
+// 0 (no sweep reads them).
 template <typename T>
-__global__ void k_unit_flags(PfGeom g, const T *__restrict__ PHIs, uint8_t *__restrict__ unit) {
+__global__ __launch_bounds__(256) void k_unit_flags(PfGeom g, const T *__restrict__ PHIs, uint8_t *__restrict__ unit,
+                                                    int64_t nslots) {
+    // nslots is a multiple of 64: every wave lies wholly inside the slots or wholly past them
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= unit_flags_size(g)) return;
-    const int NQ = unit_chunks(g);
-    const int q = (int)(t % NQ);
-    const int64_t r = t / NQ;
+    const int RW = unit_chunks(g) * 32;
+    const int x = (int)(t % RW);
+    const int64_t r = t / RW;
     const int j = (int)(r % g.Y);
     const int64_t r2 = r / g.Y;
     const int k = (int)(r2 % g.Z);
     const int c = (int)(r2 / g.Z);
-    const bool interior = j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
-    uint8_t ok = 0;
+    const bool interior = t < nslots && j >= 1 && j <= g.n && (g.dim == 2 || (k >= 1 && k <= g.l));
+    bool fail = !interior;
     if (interior) {
-        ok = 1;
         const int sh = (c + j + k) & 1;
-        const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
-        const T *own = PHIs + (int64_t)c * g.HS, *oth = PHIs + (int64_t)(1 - c) * g.HS;
-        const T one = (T)1;
-        for (int x = q * 32; x < q * 32 + 32; ++x) {
-            const int h = x + 1 - sh, i = 2 * h + sh;   // i >= 1 for every x >= 0
-            if (i > g.m) break;
+        const int h = x + 1 - sh, i = 2 * h + sh;   // i >= 1 for every x >= 0
+        if (i <= g.m) {
+            const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
+            const T *own = PHIs + (int64_t)c * g.HS, *oth = PHIs + (int64_t)(1 - c) * g.HS;
+            const T one = (T)1;
             const int64_t o = pf_hoff(g, h, j, k);
             bool u = own[o] == one && oth[o - 1 + sh] == one && oth[o + sh] == one && oth[o - HP] == one &&
                      oth[o + HP] == one;
             if (g.dim == 3) u = u && oth[o - HPY] == one && oth[o + HPY] == one;
-            if (!u) {
-                ok = 0;
-                break;
-            }
+            fail = !u;
         }
     }
-    unit[t] = ok;
+    const unsigned long long bad = __ballot(fail);
+    const int lane = (int)(threadIdx.x & 63);
+    if (t < nslots && (lane == 0 || lane == 32)) {
+        const unsigned half = (lane == 0) ? (unsigned)bad : (unsigned)(bad >> 32);
+        unit[t >> 5] = interior ? (half == 0u ? 1 : 0) : 0;
+    }
 }
 
 void launch_unit_flags(hipStream_t st, const PfGeom &g, const double *PHIs, uint8_t *unit) {
-    const int64_t n = unit_flags_size(g);
-    hipLaunchKernelGGL(k_unit_flags<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHIs, unit);
+    const int64_t n = unit_flags_size(g) * 32;   // slots: a multiple of 64 (two chunks per row at least)
+    hipLaunchKernelGGL(k_unit_flags<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHIs, unit, n);
 }
 void launch_unit_flags(hipStream_t st, const PfGeom &g, const float *PHI32, uint8_t *unit) {
-    const int64_t n = unit_flags_size(g);
-    hipLaunchKernelGGL(k_unit_flags<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHI32, unit);
+    const int64_t n = unit_flags_size(g) * 32;
+    hipLaunchKernelGGL(k_unit_flags<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, PHI32, unit, n);
 }
 
 template <int DIM, int KB>

@@ -275,8 +275,9 @@ class Solver:
         return bool(lib().pf_sor_coded(self._h))
 
     def sor_unit_cells(self) -> int:
-        """Cells of both colours whose FP64 colour sweeps skip the porosity loads (their wave's
-        cells and neighbours all have porosity 1); 0 when the unit-porosity flags are off."""
+        """Cells of both colours whose colour sweeps skip the porosity loads (their wave's
+        cells and neighbours all have porosity 1; FP32 sweeps for mixed precision); 0 when the
+        unit-porosity flags are off."""
         n = ctypes.c_int64()
         self._check(lib().pf_sor_unit_cells(self._h, ctypes.byref(n)), "pf_sor_unit_cells")
         return int(n.value)

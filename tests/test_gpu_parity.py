@@ -492,8 +492,9 @@ def test_gpu_unit_porosity_waves_match_oracle(variant, monkeypatch):
 
 
 def test_gpu_unit_porosity_cell_count_extremes(monkeypatch):
-    """All cells in unit waves when the porosity is 1 everywhere (ghosts included), none when
-    no value is 1, none with the flags turned off."""
+    """All cells in unit waves when the porosity is 1 everywhere (ghosts included; FP64 and
+    the mixed-precision FP32 sweeps), none when no value is 1, none with the flags turned
+    off."""
     p = cases.CaseParams(dim=3, m=64, n=32, l=16, width=1.0, height=0.5, depth=0.25, time=1e-4, istep_max=1,
                          iter_max=5)
     ones = np.ones((p.l, p.n, p.m))
@@ -502,6 +503,9 @@ def test_gpu_unit_porosity_cell_count_extremes(monkeypatch):
     rnd = np.random.default_rng(3).uniform(0.2, 0.99, size=ones.shape)
     with _solver(p, rnd) as s:
         assert s.sor_unit_cells() == 0
+    from pixelflow_amd.solver import PF_PREC_MIXED, Solver
+    with Solver(p, ones, precision=PF_PREC_MIXED, inner_iters=5) as s:   # the FP32 sweeps' flags
+        assert s.sor_unit_cells() == p.cells
     monkeypatch.setenv("PF_UNIT_PHI", "0")
     with _solver(p, ones) as s:
         assert s.sor_unit_cells() == 0
