@@ -429,10 +429,16 @@ static int derive_from_porosity(pf_ctx *c) {
         }
     }
     // the multigrid operators follow the porosity (rebuilt at the next solve)
-    if (c->mg32) { c->mg32->c.fast = (c->sor_variant == 2); c->mg32->built = false; }
+    const bool unit_ops = 2 * c->unit_cells >= ncell && c->unit_cells > 0;
+    if (c->mg32) {
+        c->mg32->c.fast = (c->sor_variant == 2);
+        c->mg32->c.unit_ops = unit_ops;
+        c->mg32->built = false;
+    }
     if (c->mg64) {
         c->mg64->c.fast = (c->sor_variant == 2);
         c->mg64->c.coded = c->phi_coded;
+        c->mg64->c.unit_ops = unit_ops;
         c->mg64->built = false;
     }
     c->have_phi = true;

@@ -108,6 +108,8 @@ struct MgCase {
     const double *PHTAB = nullptr;   //   (used by the level-0 sweeps while `coded`)
     const uint8_t *UNIT = nullptr;   // FP64 solver: the unit-porosity chunk flags, or nullptr
     const uint8_t *UNIT32 = nullptr; // FP32 solver: the same flags of the FP32 porosity, or nullptr
+    bool unit_ops = false;           // the level-0 operator kernels use the flags too (set where
+                                     // unit-porosity waves hold half the cells or more)
     bool coded = false;
     bool fast = false;               // FP64 faces by the checked constant-divisor division
     int kb32 = 1, kb64 = 1, xw = 64, remap32 = 1, remap64 = 1;   // colour-sweep launch settings

@@ -82,6 +82,39 @@ __device__ __forceinline__ T op0(const PfGeom &g, const StepArgs &a, const MgK<T
     return ap;
 }
 
+// op0 at a cell whose seven porosity operands are all 1 (a unit-porosity chunk): the same
+// couplings and ap without the porosity loads.  FP64: the faces of two unit porosities from
+// StepArgs (unit_ax/ay/az, unit_ap: the values face_any and op0's sum give for them); FP32:
+// face32 of two unit porosities, which the compiler evaluates exactly as for loaded ones.
+template <typename T, int DIM>
+__device__ __forceinline__ T op0_unit(const PfGeom &g, const StepArgs &a, const MgK<T> &K,
+                                      const double *__restrict__ phi, int i, int j, int k, CoefT<T> &c) {
+    T ap;
+    if constexpr (std::is_same<T, float>::value) {
+        c.ae = face32(1.f, 1.f, K.cdx);
+        c.aw = face32(1.f, 1.f, K.cdx);
+        c.an = face32(1.f, 1.f, K.cdy);
+        c.as = face32(1.f, 1.f, K.cdy);
+        c.at = 0.f;
+        c.ab = 0.f;
+        if (DIM == 3) {
+            c.at = face32(1.f, 1.f, K.cdz);
+            c.ab = face32(1.f, 1.f, K.cdz);
+        }
+        ap = -(c.ae + c.aw + c.an + c.as + c.at + c.ab);
+    } else {
+        c.ae = a.unit_ax;
+        c.aw = a.unit_ax;
+        c.an = a.unit_ay;
+        c.as = a.unit_ay;
+        c.at = (DIM == 3) ? a.unit_az : 0.;
+        c.ab = (DIM == 3) ? a.unit_az : 0.;
+        ap = a.unit_ap;
+    }
+    if (is_edge_row(g, a, i, j, k)) row_rules<false, T>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
+    return ap;
+}
+
 // A row that the boundary rules decouple (all six couplings zero: the outlet rows) takes
 // no part in the coarse space.
 __device__ __forceinline__ bool dec0(const PfGeom &g, const StepArgs &a, int i, int j, int k, const double *phi) {
@@ -92,12 +125,12 @@ __device__ __forceinline__ bool dec0(const PfGeom &g, const StepArgs &a, int i, 
     return c.ae == 0.f && c.aw == 0.f && c.an == 0.f && c.as == 0.f && c.at == 0.f && c.ab == 0.f;
 }
 
-// (A y)(i, j, k) at level 0.
-template <typename T, int DIM, bool FAST>
+// (A y)(i, j, k) at level 0; UNIT: the cell's porosity operands are all 1 (op0_unit).
+template <typename T, int DIM, bool FAST, bool UNIT = false>
 __device__ __forceinline__ T apply0_at(const PfGeom &g, const StepArgs &a, const MgK<T> &K, const T *F,
                                        const double *phi, const T *__restrict__ y_, int i, int j, int k) {
     CoefT<T> c;
-    const T ap = op0<T, DIM, FAST>(g, a, K, F, phi, i, j, k, c);
+    const T ap = UNIT ? op0_unit<T, DIM>(g, a, K, phi, i, j, k, c) : op0<T, DIM, FAST>(g, a, K, F, phi, i, j, k, c);
     const auto y = gmem(y_);
     T s = ap * y[pf_soff(g, i, j, k)];
     s = s + c.ae * y[pf_soff(g, i + 1, j, k)];
@@ -140,15 +173,37 @@ PF_HD void mg_cell(const MgGeom &g, int64_t q, int &I, int &J, int &K) {
 // Cells of colour `zero_colour` are left out: the pre-smoothing ends with a Gauss-Seidel
 // (omega = 1) sweep of that colour, after which their residual is zero but for rounding.
 // ------------------------------------------------------------------------------------
-template <typename T, int DIM, bool FAST>
+// UNIT: `unit` holds the unit-porosity chunk flags of F; a wave whose fine cells all lie in
+// flagged chunks takes op0_unit for them.
+template <typename T, int DIM, bool FAST, bool UNIT>
 __global__ __launch_bounds__(256) void k_mg_restrict0(PfGeom g, StepArgs a, MgK<T> K, const T *F,
                                                       const double *phi, const T *__restrict__ rhs,
                                                       const T *__restrict__ e, MgGeom c1, T *__restrict__ r1,
-                                                      T *__restrict__ e1, int zero_colour) {
+                                                      T *__restrict__ e1, int zero_colour,
+                                                      const uint8_t *__restrict__ unit) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)c1.m * c1.n * c1.l) return;
-    int I, J, Kc;
-    mg_cell(c1, q, I, J, Kc);
+    const bool active = q < (int64_t)c1.m * c1.n * c1.l;
+    int I = 0, J = 0, Kc = 0;
+    if (active) mg_cell(c1, q, I, J, Kc);
+    // the fine cells this thread sums, and whether all of them lie in unit-porosity chunks
+    int ok = 1;
+    if (UNIT && active) {
+#pragma unroll
+        for (int dz = 0; dz < (DIM == 3 ? 2 : 1); ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int i = 2 * I + dx + 1, j = 2 * J + dy + 1, k = (DIM == 3) ? 2 * Kc + dz + 1 : 0;
+                    if (i > g.m || j > g.n || (DIM == 3 && k > g.l)) continue;
+                    const int c = (i + j + k) & 1;
+                    if (c == zero_colour) continue;
+                    // slot position h + sh - 1 of cell i = 2 h + sh
+                    ok &= (int)gmem(unit)[unit_index(g, c, j, k, (i >> 1) + (i & 1) - 1)];
+                }
+    }
+    const bool all_unit = UNIT && __all(ok) != 0;   // the wave's branch
+    if (!active) return;
     T sum = T(0);
 #pragma unroll
     for (int dz = 0; dz < (DIM == 3 ? 2 : 1); ++dz)
@@ -160,7 +215,9 @@ __global__ __launch_bounds__(256) void k_mg_restrict0(PfGeom g, StepArgs a, MgK<
                 if (i > g.m || j > g.n || (DIM == 3 && k > g.l)) continue;
                 if (((i + j + k) & 1) == zero_colour) continue;
                 if (dec0(g, a, i, j, k, phi)) continue;
-                sum = sum + (gmem(rhs)[pf_soff(g, i, j, k)] - apply0_at<T, DIM, FAST>(g, a, K, F, phi, e, i, j, k));
+                const T Ae = all_unit ? apply0_at<T, DIM, FAST, true>(g, a, K, F, phi, e, i, j, k)
+                                      : apply0_at<T, DIM, FAST>(g, a, K, F, phi, e, i, j, k);
+                sum = sum + (gmem(rhs)[pf_soff(g, i, j, k)] - Ae);
             }
     r1[mg_off(c1, I, J, Kc)] = sum;
     mg_store(c1, e1, I, J, Kc, T(0), c1.per, c1.per);
@@ -215,10 +272,13 @@ __global__ __launch_bounds__(256) void k_mg_prolong0(PfGeom g, StepArgs a, const
 }
 
 // out = A y at level 0; MODE 0: partial (w, out); MODE 1: partials (out, w) and (out, out).
-template <typename T, int DIM, bool FAST, int MODE>
+// UNIT: `unit` holds the unit-porosity chunk flags of F (unit_index in pf_layout.h); a
+// wave whose 64 cells all lie in flagged chunks takes op0_unit.
+template <typename T, int DIM, bool FAST, int MODE, bool UNIT>
 __global__ __launch_bounds__(256) void k_mg_apply0(PfGeom g, StepArgs a, MgK<T> K, const T *F, const double *phi,
                                                    const T *__restrict__ y, T *__restrict__ out,
-                                                   const T *__restrict__ w, double *__restrict__ part) {
+                                                   const T *__restrict__ w, double *__restrict__ part,
+                                                   const uint8_t *__restrict__ unit) {
     const int rows = g.n * (DIM == 3 ? g.l : 1);
     double d0 = 0., d1 = 0.;
     for (int row = (int)blockIdx.x * 4 + (int)threadIdx.y; row < rows; row += (int)gridDim.x * 4) {
@@ -230,8 +290,13 @@ __global__ __launch_bounds__(256) void k_mg_apply0(PfGeom g, StepArgs a, MgK<T> 
             const int sh = (c + j + k) & 1;
             for (int h0 = 0; h0 < (g.m + 1) / 2; h0 += 64) {
                 const int i = 2 * ((1 - sh) + h0 + (int)threadIdx.x) + sh;
+                // the lane's chunk is its slot position h + sh - 1 = h0 + lane (the sweeps' own)
+                bool all_unit = false;
+                if constexpr (UNIT)
+                    all_unit = __all(i > g.m || gmem(unit)[unit_index(g, c, j, k, h0 + (int)threadIdx.x)] != 0) != 0;
                 if (i > g.m) continue;
-                const T Ay = apply0_at<T, DIM, FAST>(g, a, K, F, phi, y, i, j, k);
+                const T Ay = all_unit ? apply0_at<T, DIM, FAST, true>(g, a, K, F, phi, y, i, j, k)
+                                      : apply0_at<T, DIM, FAST>(g, a, K, F, phi, y, i, j, k);
                 const int64_t so = pf_soff(g, i, j, k);
                 out[so] = Ay;
                 const double wv = (double)w[so];
@@ -1195,6 +1260,20 @@ template <>
 const float *fine_phi<float>(const MgSolver<float> &S) { return S.c.PHI32; }
 template <>
 const double *fine_phi<double>(const MgSolver<double> &S) { return S.c.PHIs; }
+// The unit-porosity chunk flags of that porosity for the level-0 operator kernels, where
+// unit-porosity waves hold half the cells or more (nullptr: none).  Measured per step to
+// the rule on one box, flags on / off: sphere 256x128x128 FP64 9.23 / 9.36 ms, boxes
+// 512x256x256 FP64 72.1 / 73.6 ms, room 192^3 (few unit cells) 23.3 / 23.0 ms.
+// PF_MG_UNIT=0/1 in the environment overrides the rule.
+template <typename T>
+static const uint8_t *fine_unit(const MgSolver<T> &S) {
+    static const int forced = [] {
+        const char *v = std::getenv("PF_MG_UNIT");
+        return (v && *v) ? (std::atoi(v) != 0 ? 1 : 0) : -1;
+    }();
+    if (forced == 0 || (forced < 0 && !S.c.unit_ops)) return nullptr;
+    return std::is_same<T, float>::value ? S.c.UNIT32 : S.c.UNIT;
+}
 
 template <typename T>
 static MgOp<T> op_of(const MgLevel<T> &L) {
@@ -1417,11 +1496,14 @@ static void vcycle(MgSolver<T> &S, hipStream_t st, const T *rhs, T *e) {
         // with an odd periodic extent a boundary cell's periodic neighbour has its own colour
         // and is refreshed after the sweep, so its residual is not zero: all cells count
         const int zero_colour = (S.prm.nu1 > 0 && !S.lev[0].g.ref_y && !S.lev[0].g.ref_z) ? 1 : -1;
-#define PF_R0(D, F_) hipLaunchKernelGGL((k_mg_restrict0<T, D, F_>), dim3(nb), dim3(256), 0, st, g, S.c.a, K, F, S.c.phi, \
-                                        rhs, (const T *)e, C.g, C.r, C.e, zero_colour)
+        const uint8_t *unit = fine_unit(S);
+#define PF_R0_U(D, F_, U) hipLaunchKernelGGL((k_mg_restrict0<T, D, F_, U>), dim3(nb), dim3(256), 0, st, g, S.c.a, K, F, \
+                                             S.c.phi, rhs, (const T *)e, C.g, C.r, C.e, zero_colour, unit)
+#define PF_R0(D, F_) do { if (unit) PF_R0_U(D, F_, true); else PF_R0_U(D, F_, false); } while (0)
         if (g.dim == 3) { if (S.c.fast) PF_R0(3, true); else PF_R0(3, false); }
         else { if (S.c.fast) PF_R0(2, true); else PF_R0(2, false); }
 #undef PF_R0
+#undef PF_R0_U
         coarse_cycle(S, st, 1);
         const MgGeom &l0 = S.lev[0].g;
         dim3 b(64, 4, 1);
@@ -1454,11 +1536,17 @@ static void apply0(MgSolver<T> &S, hipStream_t st, const T *y, T *out, const T *
     const MgK<T> K = mg_k<T>(S.c.a);
     const T *F = fine_phi(S);
     const dim3 b(64, 4, 1), gr(apply_blocks<T>(g));
-#define PF_AP(D, F_) hipLaunchKernelGGL((k_mg_apply0<T, D, F_, MODE>), gr, b, 0, st, g, S.c.a, K, F, S.c.phi, y, out, \
-                                        w, S.part)
+    // unit-porosity waves for the FP64 operator only: 512x256x256 FP64 360 against 395 us,
+    // FP32 245 against 204 us (the flag's load ahead of the operands costs the FP32 form more
+    // than the porosity bytes it saves)
+    const uint8_t *unit = std::is_same<T, double>::value ? fine_unit(S) : nullptr;
+#define PF_AP_U(D, F_, U) hipLaunchKernelGGL((k_mg_apply0<T, D, F_, MODE, U>), gr, b, 0, st, g, S.c.a, K, F, S.c.phi, y, \
+                                             out, w, S.part, unit)
+#define PF_AP(D, F_) do { if (unit) PF_AP_U(D, F_, true); else PF_AP_U(D, F_, false); } while (0)
     if (g.dim == 3) { if (S.c.fast) PF_AP(3, true); else PF_AP(3, false); }
     else { if (S.c.fast) PF_AP(2, true); else PF_AP(2, false); }
 #undef PF_AP
+#undef PF_AP_U
 }
 
 // ------------------------------------------------------------------------------------
