@@ -1263,7 +1263,7 @@ const double *fine_phi<double>(const MgSolver<double> &S) { return S.c.PHIs; }
 // The unit-porosity chunk flags of that porosity for the level-0 operator kernels, where
 // unit-porosity waves hold half the cells or more (nullptr: none).  Measured per step to
 // the rule on one box, flags on / off: sphere 256x128x128 FP64 9.23 / 9.36 ms, boxes
-// 512x256x256 FP64 72.1 / 73.6 ms, room 192^3 (few unit cells) 23.3 / 23.0 ms.
+// 512x256x256 FP64 71.9 / 73.8 ms, room 192^3 (fewer unit cells) 23.3 / 22.7 ms.
 // PF_MG_UNIT=0/1 in the environment overrides the rule.
 template <typename T>
 static const uint8_t *fine_unit(const MgSolver<T> &S) {
