@@ -511,19 +511,26 @@ def test_gpu_unit_porosity_cell_count_extremes(monkeypatch):
         assert s.sor_unit_cells() == 0
 
 
-@pytest.mark.parametrize("mode", ["mixed_fixed", "mixed_mg", "fp64_mg"])
+@pytest.mark.parametrize("mode", ["mixed_fixed", "mixed_mg", "fp64_mg", "mixed_fixed_2d"])
 def test_gpu_unit_porosity_solvers_identical(mode, monkeypatch):
     """The mixed-precision FP32 sweeps (flags of the FP32 porosity, whose unit values are
     those that round to 1) and the multigrid's level-0 smoothers give the same bits with
     and without unit-porosity waves, on the sparse field above plus values within an FP32
     rounding of 1."""
     from pixelflow_amd import solver as S
-    p = cases.CaseParams(dim=3, m=134, n=34, l=22, width=1.34, height=0.34, depth=0.22, time=3e-4, istep_max=3,
-                         iter_max=20)
-    phi = _sparse_porosity(p, 5)
-    phi[4, 10, 40] = 1.0 - 2.0 ** -30   # 1.0 in FP32, not in FP64
-    phi[9, 22, 99] = 1.0 - 2.0 ** -20
+    if mode.endswith("_2d"):
+        p = cases.CaseParams(dim=2, m=134, n=34, width=1.34, height=0.34, time=3e-4, istep_max=3, iter_max=20)
+        phi = _sparse_porosity(p, 5)
+        phi[10, 40] = 1.0 - 2.0 ** -30   # 1.0 in FP32, not in FP64
+        phi[22, 99] = 1.0 - 2.0 ** -20
+    else:
+        p = cases.CaseParams(dim=3, m=134, n=34, l=22, width=1.34, height=0.34, depth=0.22, time=3e-4, istep_max=3,
+                             iter_max=20)
+        phi = _sparse_porosity(p, 5)
+        phi[4, 10, 40] = 1.0 - 2.0 ** -30   # 1.0 in FP32, not in FP64
+        phi[9, 22, 99] = 1.0 - 2.0 ** -20
     kw = {"mixed_fixed": dict(precision=S.PF_PREC_MIXED, inner_iters=10),
+          "mixed_fixed_2d": dict(precision=S.PF_PREC_MIXED, inner_iters=10),
           "mixed_mg": dict(precision=S.PF_PREC_MIXED, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30),
           "fp64_mg": dict(precision=S.PF_PREC_FP64, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30)}[mode]
     print(f"[gpu] unit porosity, {mode}", flush=True)
