@@ -64,6 +64,8 @@ struct pf_ctx {
     uint8_t *UNIT = nullptr;
     int64_t unit_cells = 0;
     uint8_t *UNIT32 = nullptr;      // mixed precision: the same flags of the FP32 porosity
+    bool unit_half = false;         // unit waves hold half the cells or more: the residual and
+                                    // the multigrid's level-0 kernels use the flags too
     int err_cap = 0, nparts = 0;
     bool have_phi = false, have_init = false;
     bool timing = false;
@@ -150,7 +152,7 @@ extern "C" int64_t pf_footprint_bytes(const pf_desc *d) {
     int tk = 0, tr = 0;
     tb2_params(g, d, tk, tr);
     const int64_t p2 = (tk > 0) ? (int64_t)8 * 2 * g.HS : 0;
-    const int64_t unit = unit_wanted(d) ? unit_flags_size(g) : 0;
+    const int64_t unit = unit_wanted(d) ? unit_flags_size(g) * (d->precision == PF_PREC_MIXED ? 2 : 1) : 0;
     return (int64_t)8 * (nat * g.NS + spl * 2 * g.HS + interior) + mixed + mg + coded + p2 + unit + 4096;
 }
 
@@ -277,9 +279,12 @@ extern "C" int pf_create(const pf_desc *desc, pf_ctx **out) {
             goto bad;
         }
     }
+    // FP64 flags for every precision (the FP64 residual reads the FP64 porosity), FP32 flags
+    // for the mixed-precision sweeps
     if (unit_wanted(desc) &&
-        hipMalloc((void **)(desc->precision == PF_PREC_MIXED ? &c->UNIT32 : &c->UNIT), (size_t)unit_flags_size(g)) !=
-            hipSuccess) {
+        (hipMalloc((void **)&c->UNIT, (size_t)unit_flags_size(g)) != hipSuccess ||
+         (desc->precision == PF_PREC_MIXED &&
+          hipMalloc((void **)&c->UNIT32, (size_t)unit_flags_size(g)) != hipSuccess))) {
         rc = fail(c, PF_ENOMEM, "hipMalloc of the unit-porosity flags failed");
         goto bad;
     }
@@ -430,6 +435,7 @@ static int derive_from_porosity(pf_ctx *c) {
     }
     // the multigrid operators follow the porosity (rebuilt at the next solve)
     const bool unit_ops = 2 * c->unit_cells >= ncell && c->unit_cells > 0;
+    c->unit_half = unit_ops;
     if (c->mg32) {
         c->mg32->c.fast = (c->sor_variant == 2);
         c->mg32->c.unit_ops = unit_ops;
@@ -635,7 +641,8 @@ static int enqueue_mixed(pf_ctx *c, unsigned long long *slot, double *res, int *
             s.fuse_z = fz;
             s.track_err = norm ? 1 : 0;
             s.xcd_remap = c->xcd_remap;
-            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, c->R32, c->E32, c->res_part, fast);
+            pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, c->R32, c->E32, c->res_part, fast,
+                                c->unit_half ? c->UNIT : nullptr);
         }
         if (!norm) return PF_OK;
         pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
@@ -726,7 +733,7 @@ static int enqueue_fp64_tol(pf_ctx *c, double *res, int *iters) {
             s.track_err = 1;
             s.xcd_remap = c->xcd_remap;
             pf::launch_residual(c->st, g, c->a, s, c->P, c->PHIs, c->BB, c->phi, (float *)nullptr, (float *)nullptr, c->res_part,
-                                fast);
+                                fast, c->unit_half ? c->UNIT : nullptr);
         }
         pf::launch_reduce_err(c->st, c->res_part, c->nparts, c->res_val);
         unsigned long long bits = 0;

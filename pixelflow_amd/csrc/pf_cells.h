@@ -846,7 +846,7 @@ __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs 
                                                 int sh, int64_t o, const double *__restrict__ P_,
                                                 const double *__restrict__ PHIs_, const double *__restrict__ BB_,
                                                 const double *__restrict__ phi, TR *__restrict__ R_,
-                                                TR *__restrict__ E_) {
+                                                TR *__restrict__ E_, bool unit = false) {
     const auto P = gmem(P_);
     const auto PHIs = gmem(PHIs_);
     const auto BB = gmem(BB_);
@@ -856,23 +856,36 @@ __device__ __forceinline__ double residual_cell(const PfGeom &g, const StepArgs 
     const int64_t HP = g.HP, HPY = (int64_t)g.HP * g.Y;
     const auto Po = P + own, Px = P + oth, Fo = PHIs + own, Fx = PHIs + oth;
     const double pc = Po[o], pW = Px[o - 1 + sh], pE = Px[o + sh], pS = Px[o - HP], pN = Px[o + HP];
-    const double fc = Fo[o], fW = Fx[o - 1 + sh], fE = Fx[o + sh], fS = Fx[o - HP], fN = Fx[o + HP];
-    Coef c;
-    c.ae = face_any<FAST>(fE, fc, a.dt, a.dx, a.rdx);
-    c.aw = face_any<FAST>(fc, fW, a.dt, a.dx, a.rdx);
-    c.an = face_any<FAST>(fN, fc, a.dt, a.dy, a.rdy);
-    c.as = face_any<FAST>(fc, fS, a.dt, a.dy, a.rdy);
     double pB = 0., pT = 0.;
-    c.at = 0.;
-    c.ab = 0.;
     if (DIM == 3) {
         pB = Px[o - HPY];
         pT = Px[o + HPY];
-        c.at = face_any<FAST>(Fx[o + HPY], fc, a.dt, a.dz, a.rdz);
-        c.ab = face_any<FAST>(fc, Fx[o - HPY], a.dt, a.dz, a.rdz);
     }
-    const double ap = (DIM == 3) ? (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab
-                                 : (((-c.ae) - c.aw) - c.an) - c.as;
+    Coef c;
+    double ap;
+    if (unit) {
+        // every porosity operand is 1 (a unit-porosity chunk): the unit faces and their sum
+        c.ae = a.unit_ax;
+        c.aw = a.unit_ax;
+        c.an = a.unit_ay;
+        c.as = a.unit_ay;
+        c.at = (DIM == 3) ? a.unit_az : 0.;
+        c.ab = (DIM == 3) ? a.unit_az : 0.;
+        ap = a.unit_ap;
+    } else {
+        const double fc = Fo[o], fW = Fx[o - 1 + sh], fE = Fx[o + sh], fS = Fx[o - HP], fN = Fx[o + HP];
+        c.ae = face_any<FAST>(fE, fc, a.dt, a.dx, a.rdx);
+        c.aw = face_any<FAST>(fc, fW, a.dt, a.dx, a.rdx);
+        c.an = face_any<FAST>(fN, fc, a.dt, a.dy, a.rdy);
+        c.as = face_any<FAST>(fc, fS, a.dt, a.dy, a.rdy);
+        c.at = 0.;
+        c.ab = 0.;
+        if (DIM == 3) {
+            c.at = face_any<FAST>(Fx[o + HPY], fc, a.dt, a.dz, a.rdz);
+            c.ab = face_any<FAST>(fc, Fx[o - HPY], a.dt, a.dz, a.rdz);
+        }
+        ap = (DIM == 3) ? (((((-c.ae) - c.aw) - c.an) - c.as) - c.at) - c.ab : (((-c.ae) - c.aw) - c.an) - c.as;
+    }
     if (is_edge_row(g, a, i, j, k)) row_rules<false>(g, a, i, j, k, c, nullptr, nullptr, phi, 0.);
     double r = BB[own + o];
     r = r - c.ae * pE;

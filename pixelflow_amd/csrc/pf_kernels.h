@@ -114,15 +114,16 @@ void launch_force(hipStream_t st, const PfGeom &g, const StepArgs &a, const doub
                   const double *v, const double *w, const double *P, double *partial, double *out);
 void launch_copy_probe(hipStream_t st, const double *src, double *dst, int64_t n);
 // mixed precision (pf_mixed.hip)
+// unit: the unit-porosity chunk flags of the FP64 porosity PHIs, or nullptr.
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, float *R, float *E,
-                     unsigned long long *slots, bool fast);
+                     unsigned long long *slots, bool fast, const uint8_t *unit = nullptr);
 // unit: the unit-porosity chunk flags of the FP32 porosity (launch_unit_flags), or nullptr.
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,
                   const float *R, const double *phi, unsigned long long *err_slot, const uint8_t *unit = nullptr);
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, double *R, double *E,
-                     unsigned long long *slots, bool fast);
+                     unsigned long long *slots, bool fast, const uint8_t *unit = nullptr);
 void launch_update(hipStream_t st, const PfGeom &g, const float *E, double *P);
 void launch_update(hipStream_t st, const PfGeom &g, const double *E, double *P);
 void launch_update_track(hipStream_t st, const PfGeom &g, const float *E, double *P, unsigned long long *slots,

@@ -1567,7 +1567,8 @@ static void enqueue_residual(MgSolver<T> &S, hipStream_t st) {
         sw.colour = half;
         sw.track_err = 1;
         sw.xcd_remap = c.remap32;
-        launch_residual(st, c.g, c.a, sw, c.P, c.PHIs, c.BB, c.phi, S.r, (T *)nullptr, S.res_slots, c.fast);
+        launch_residual(st, c.g, c.a, sw, c.P, c.PHIs, c.BB, c.phi, S.r, (T *)nullptr, S.res_slots, c.fast,
+                        c.unit_ops ? c.UNIT : nullptr);
     }
     launch_reduce_err(st, S.res_slots, S.nslots, S.res_val);
     (void)hipMemcpyAsync(S.h_scal + 8, S.res_val, 8, hipMemcpyDeviceToHost, st);

@@ -25,14 +25,21 @@ namespace pf {
 // wave max of |r / ap| into the wave's slot.
 // ------------------------------------------------------------------------------------
 
-template <int DIM, bool FAST, typename TR>
+// UNIT: `unit` holds the unit-porosity chunk flags of the FP64 porosity; a wave (one row of
+// 64 slots) whose cells all lie in flagged chunks takes the unit faces.
+template <int DIM, bool FAST, typename TR, bool UNIT>
 __global__ __launch_bounds__(256) void k_residual(PfGeom g, StepArgs a, SweepArgs s, const double *__restrict__ P,
                                                   const double *__restrict__ PHIs, const double *__restrict__ BB,
                                                   const double *__restrict__ phi, TR *__restrict__ R,
-                                                  TR *__restrict__ E, unsigned long long *slots) {
+                                                  TR *__restrict__ E, unsigned long long *slots,
+                                                  const uint8_t *__restrict__ unit) {
     const CellIdx q = sweep_cell<DIM>(g, s);
+    bool all_unit = false;
+    if constexpr (UNIT)
+        all_unit = __all(!q.active || gmem(unit)[unit_index(g, s.colour, q.j, q.k, q.h + q.sh - 1)] != 0) != 0;
     double res = 0.;
-    if (q.active) res = residual_cell<DIM, FAST, TR>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E);
+    if (q.active)
+        res = residual_cell<DIM, FAST, TR>(g, a, s.colour, q.i, q.j, q.k, q.sh, q.o, P, PHIs, BB, phi, R, E, all_unit);
     if (s.track_err) wave_max_to_slot(res, slots);
 }
 
@@ -156,23 +163,26 @@ static inline dim3 sweep_grid(const PfGeom &g, SweepArgs &s, dim3 b) {
 template <typename TR>
 static void residual_any(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                          const double *PHIs, const double *BB, const double *phi, TR *R, TR *E,
-                         unsigned long long *slots, bool fast) {
+                         unsigned long long *slots, bool fast, const uint8_t *unit) {
     dim3 b(64, 4, 1);
     dim3 gr = sweep_grid(g, s, b);
-#define PF_RES(D, F) hipLaunchKernelGGL((k_residual<D, F, TR>), gr, b, 0, st, g, a, s, P, PHIs, BB, phi, R, E, slots)
+#define PF_RES_U(D, F, U) hipLaunchKernelGGL((k_residual<D, F, TR, U>), gr, b, 0, st, g, a, s, P, PHIs, BB, phi, R, E, \
+                                             slots, unit)
+#define PF_RES(D, F) do { if (unit) PF_RES_U(D, F, true); else PF_RES_U(D, F, false); } while (0)
     if (g.dim == 3) { if (fast) PF_RES(3, true); else PF_RES(3, false); }
     else { if (fast) PF_RES(2, true); else PF_RES(2, false); }
 #undef PF_RES
+#undef PF_RES_U
 }
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, float *R, float *E,
-                     unsigned long long *slots, bool fast) {
-    residual_any<float>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast);
+                     unsigned long long *slots, bool fast, const uint8_t *unit) {
+    residual_any<float>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast, unit);
 }
 void launch_residual(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, const double *P,
                      const double *PHIs, const double *BB, const double *phi, double *R, double *E,
-                     unsigned long long *slots, bool fast) {
-    residual_any<double>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast);
+                     unsigned long long *slots, bool fast, const uint8_t *unit) {
+    residual_any<double>(st, g, a, s, P, PHIs, BB, phi, R, E, slots, fast, unit);
 }
 
 void launch_sor32(hipStream_t st, const PfGeom &g, const StepArgs &a, SweepArgs s, float *E, const float *F32,

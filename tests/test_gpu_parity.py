@@ -511,12 +511,12 @@ def test_gpu_unit_porosity_cell_count_extremes(monkeypatch):
         assert s.sor_unit_cells() == 0
 
 
-@pytest.mark.parametrize("mode", ["mixed_fixed", "mixed_mg", "fp64_mg", "mixed_fixed_2d"])
+@pytest.mark.parametrize("mode", ["mixed_fixed", "mixed_mg", "fp64_mg", "fp64_tol", "mixed_fixed_2d"])
 def test_gpu_unit_porosity_solvers_identical(mode, monkeypatch):
     """The mixed-precision FP32 sweeps (flags of the FP32 porosity, whose unit values are
-    those that round to 1) and the multigrid's level-0 smoothers give the same bits with
-    and without unit-porosity waves, on the sparse field above plus values within an FP32
-    rounding of 1."""
+    those that round to 1), the FP64 residual (flags of the FP64 porosity) and the
+    multigrid's level-0 kernels give the same bits with and without unit-porosity waves,
+    on the sparse field above plus values within an FP32 rounding of 1."""
     from pixelflow_amd import solver as S
     if mode.endswith("_2d"):
         p = cases.CaseParams(dim=2, m=134, n=34, width=1.34, height=0.34, time=3e-4, istep_max=3, iter_max=20)
@@ -532,7 +532,8 @@ def test_gpu_unit_porosity_solvers_identical(mode, monkeypatch):
     kw = {"mixed_fixed": dict(precision=S.PF_PREC_MIXED, inner_iters=10),
           "mixed_fixed_2d": dict(precision=S.PF_PREC_MIXED, inner_iters=10),
           "mixed_mg": dict(precision=S.PF_PREC_MIXED, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30),
-          "fp64_mg": dict(precision=S.PF_PREC_FP64, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30)}[mode]
+          "fp64_mg": dict(precision=S.PF_PREC_FP64, residual_tol=1e-9, solver=S.PF_SOLVER_MG, max_cycles=30),
+          "fp64_tol": dict(precision=S.PF_PREC_FP64, residual_tol=1e-7, inner_iters=10, max_cycles=4)}[mode]
     print(f"[gpu] unit porosity, {mode}", flush=True)
     out = {}
     for flag in ("1", "0"):
